@@ -1,0 +1,154 @@
+/*
+ * mcrt.h -- C ABI of the MI355X photon-packet engine (libmcrt.so).
+ *
+ * The reference (evc9pu/src3_mpi_cfitsio, Fortran 90 + MPI) has no plugin
+ * API: its packet path is a set of Fortran subroutines sharing module-global
+ * arrays.  Each entry point below replaces one of those routines or the
+ * module state it reads, with plain pointers and sizes so that a Fortran
+ * `interface ... bind(C)` block (src3_mpi_cfitsio_amd/fortran/mcrt_mod.f90)
+ * or ctypes can call it.  Citations are relative to src3_mpi_cfitsio/.
+ *
+ * Conventions
+ *  - every call returns 0 on success, a nonzero MCRT_E_* code otherwise
+ *    (the reference `stop`s); mcrt_last_error() describes the failure;
+ *  - host buffers are caller-owned and copied; device buffers are owned by
+ *    the context and stay resident between calls (upload once per model);
+ *  - walls keep the reference's 1-based meaning: array element k-1 holds
+ *    wall k; cell (ir,it,ip) is element (ir-1)+(nrg-1)*((it-1)+(ntg-1)*(ip-1));
+ *  - units are the reference's: lengths in stellar radii, photon energy in
+ *    eV, temperature in K/11605, density in g/cm^3, volume in cm^3;
+ *  - a context is used by one host thread at a time (the reference runs one
+ *    thread per MPI rank); work is queued on the context's stream and every
+ *    call that returns data to the host synchronizes that stream.
+ */
+#ifndef MCRT_H
+#define MCRT_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcrt_ctx mcrt_ctx;
+
+enum { MCRT_OK = 0, MCRT_E_ARG = 1, MCRT_E_STATE = 2, MCRT_E_HIP = 3, MCRT_E_NODEVICE = 4 };
+enum { MCRT_FP64 = 0, MCRT_MIXED = 1 };           /* transport precision */
+enum { MCRT_T_REFERENCE = 0, MCRT_T_PERCELL = 1 }; /* Lucy temperature mode */
+
+/* Context on one GPU; stands in for one MPI rank's module state
+ * (ttsre_mpi_mod.f90, grid_mod_5.f90, dust_mod_5.f90, tts_mod_5.f90). */
+int mcrt_create(int device, mcrt_ctx **ctx);
+int mcrt_destroy(mcrt_ctx *ctx);
+const char *mcrt_last_error(const mcrt_ctx *ctx);
+/* Queue work on an external HIP stream (hipStream_t as void*); NULL = own stream. */
+int mcrt_set_stream(mcrt_ctx *ctx, void *stream);
+int mcrt_synchronize(mcrt_ctx *ctx);
+
+/* Grid walls and cell volumes: replaces grid_allocate + the wall section of
+ * gridset (grid_mod_5.f90:69-122, gridset_5_model13new.f90:516,706-781,1152).
+ * rarr,r2arr: nrg; thetarr,costarr,tan2arr: ntg (tan2arr=-1 flags the
+ * equator, ntg odd); phiarr,aarr,barr: npg; vcell: cells. */
+int mcrt_set_grid(mcrt_ctx *ctx, int32_t nrg, int32_t ntg, int32_t npg,
+                  const double *rarr, const double *r2arr,
+                  const double *thetarr, const double *costarr, const double *tan2arr,
+                  const double *phiarr, const double *aarr, const double *barr,
+                  const double *vcell);
+
+/* Medium: densarr(nrg,ntg,npg,ndg+2) and findopac (grid_mod_5.f90:29-63)
+ * compacted to one thermal species per cell, as select_dust_2.f90:22-34 and
+ * tfinal_5_test1.f90:244-247 require.  species[c] in 0..10 (0 = empty),
+ * species_opac[0..10] = findopac(.,.,.,id) opacity index (0-based). */
+int mcrt_set_medium(mcrt_ctx *ctx, const double *rho, const uint8_t *species,
+                    const int32_t *species_opac);
+
+/* Dust tables (dust_mod_5.f90:7-21, filled by dustinit/setemiss,
+ * dust_5.f90:345-699): per opacity index, nudust descending in eV with
+ * kappa/kappav, albedo, g, pl; the Planck tables lTint,Tint (nT), nuint (nnu),
+ * kapint[(iop*nnu + j)*nT + i] normalized CDFs and kappap[iop*nT + i]. */
+int mcrt_set_dust(mcrt_ctx *ctx, int32_t nopac, int32_t nlam,
+                  const double *nudust, const double *kappad, const double *adust,
+                  const double *gdust, const double *pldust, const double *kappav,
+                  int32_t nT, int32_t nnu, const double *lTint, const double *Tint,
+                  const double *nuint, const double *kapint, const double *kappap);
+
+/* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
+int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);
+
+/* Observer: peel-off directions and image/SED binning (tts_mod_5.f90:22-62,
+ * setup.f90:36-41,115-130, filt_mod.f90).  thete/phie in radians (npeel);
+ * images nx*nx over [-rmaxi, rmaxi]; nband filter curves of band_nw[b] lines
+ * stored at band_freq/phi[b*nwmax + k] (Hz, ascending); SEDs nfreq x nmu x
+ * nph x nap with aperture2 (R*^2) and u_mu (nmu bin centres of cos theta). */
+int mcrt_set_observer(mcrt_ctx *ctx, int32_t ipeel, int32_t npeel,
+                      const double *thete, const double *phie,
+                      int32_t nx, double rmaxi,
+                      int32_t nband, int32_t nwmax, const int32_t *band_nw,
+                      const double *band_freq, const double *band_phi,
+                      int32_t nfreq, int32_t nmu, int32_t nph, int32_t nap,
+                      const double *aperture2, const double *u_mu,
+                      double numin, double lnurat);
+
+/* Dust temperature per cell used for re-emission: tdust(nrg,ntg,npg)
+ * (grid_mod_5.f90; set from tdave in gridset_5_model13new.f90:1857-1868). */
+int mcrt_set_tdust(mcrt_ctx *ctx, const double *tdust);
+
+/* Zero every tally (initarr_2_new.f90:123-192 / the reset before disk). */
+int mcrt_zero_tallies(mcrt_ctx *ctx);
+
+/* The packet loop of disk (newdisktherm_5_fuv.f90:220-491, stellar photons)
+ * with propagate_5_fuv.f90, tauint_3d_2_fuv.f90, stokes.f90 and
+ * peeloff_3d_2_test.f90, for packets first_packet .. first_packet+npackets-1
+ * of the run seeded by `seed` (counter-based RNG: results do not depend on
+ * launch shape).  Tallies accumulate across calls until mcrt_zero_tallies. */
+int mcrt_run_packets(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
+                     uint64_t seed, int32_t precision);
+
+/* Post-ALLREDUCE tallies (newdisktherm_5_fuv.f90:524-602): any pointer may be
+ * NULL.  Layouts: dtauabs/nabs/dbig/killed per cell; sed
+ * [10][8][nap][nph][nmu][nfreq] = si,sq,su,sv,si2,sq2,su2,sv2,nums,aveinc;
+ * psed [9][8][nap][npeel][nfreq] = ti..tv,ti2..tv2,numt; image
+ * [8][nband][nx][nx][npeel] = image_b_{i,q,u,v,d,s,e,o}; counters[16]. */
+int mcrt_get_tallies(mcrt_ctx *ctx, double *dtauabs, int32_t *nabs, int32_t *dbig,
+                     int32_t *killed, float *sed, float *psed, float *image,
+                     double *counters);
+
+/* Lucy temperature (tfinal_5_test1.f90 + dusttemp_5.f90) from the device
+ * dtauabs tally, FP64.  nphot = number of stellar packets of the iteration.
+ * mode MCRT_T_REFERENCE: upper hemisphere at ip=1 averaged with its mirror,
+ * mirrored and copied to all phi; MCRT_T_PERCELL: every cell on its own.
+ * update_tdust != 0 makes the result the re-emission temperature of the next
+ * run.  tdave may be NULL. */
+int mcrt_dusttemp(mcrt_ctx *ctx, double nphot, int32_t mode, int32_t update_tdust,
+                  double *tdave);
+
+/* Deterministic entries for parity tests (no counterpart in the reference;
+ * they expose tauint's face loop and the stokes/stokespeel Mueller update).
+ * trace_rays: n rays from pos (3n) along u (3n) in cells ijk (3n) until
+ * optical depth tau_stop (rho is taken as opacity per unit length), the grid
+ * edge or the star.  faces (n*max_steps) and cells (3*n*max_steps) record each
+ * crossing; status: 0 stopped, 1 escaped, 2 star, 3 max steps, -1 no wall. */
+int mcrt_trace_rays(mcrt_ctx *ctx, int32_t precision, int32_t n, const double *pos,
+                    const double *u, const int32_t *ijk, const uint8_t *on_wall,
+                    const double *tau_stop, int32_t max_steps, int32_t *faces,
+                    int32_t *cells, double *pos_out, int32_t *ijk_out, double *tau_out,
+                    int32_t *nsteps, int32_t *status);
+/* kind 0: stokes.f90:117-336 for an accepted (bmu, ri1); kind 1: stokespeel.f90.
+ * par (9n): g, pl, bmu, ri1, cost, sint, phi, costp, sintp; dir (5n) in/out
+ * cost,sint,phi,cosp,sinp (kind 0); stokes (4n) in/out I,Q,U,V. */
+int mcrt_mueller(mcrt_ctx *ctx, int32_t precision, int32_t n, int32_t kind,
+                 const double *par, double *dir, double *stokes);
+
+/* Per-packet record of the next mcrt_run_packets calls (8 doubles per packet,
+ * indexed by packet - first_packet): final nub, cost, phi, I, Q, U,
+ * interactions, exit code (1 escaped, 2 killed, -1 error).  n = 0 disables. */
+int mcrt_enable_trace(mcrt_ctx *ctx, int64_t n);
+int mcrt_get_trace(mcrt_ctx *ctx, double *trace, int64_t n);
+
+/* Duration in ms of the last mcrt_run_packets launch, from HIP events
+ * recorded on the context stream (synchronizes). */
+int mcrt_last_run_ms(mcrt_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

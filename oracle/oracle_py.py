@@ -1,0 +1,246 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so) and of the reference leaf
+build (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class OrcModel(C.Structure):
+    _fields_ = [
+        ("nrg", C.c_int32), ("ntg", C.c_int32), ("npg", C.c_int32),
+        ("rarr", _dp), ("r2arr", _dp), ("thetarr", _dp), ("costarr", _dp), ("tan2arr", _dp),
+        ("phiarr", _dp), ("aarr", _dp), ("barr", _dp), ("vcell", _dp),
+        ("rho", _dp), ("species", _u8p), ("species_opac", C.c_int32 * 11),
+        ("nopac", C.c_int32), ("nlam", C.c_int32),
+        ("nudust", _dp), ("kappad", _dp), ("adust", _dp), ("gdust", _dp), ("pldust", _dp), ("kappav", _dp),
+        ("nT", C.c_int32), ("nnu", C.c_int32),
+        ("lTint", _dp), ("Tint", _dp), ("nuint", _dp), ("kapint", _dp), ("kappap", _dp),
+        ("rstar", C.c_double), ("tstar", C.c_double), ("rmin", C.c_double),
+        ("ipeel", C.c_int32), ("npeel", C.c_int32),
+        ("coste", _dp), ("sinte", _dp), ("cospe", _dp), ("sinpe", _dp), ("phie", _dp),
+        ("nx", C.c_int32), ("rmaxi", C.c_double),
+        ("nband", C.c_int32), ("nwmax", C.c_int32), ("band_nw", _ip), ("band_freq", _dp), ("band_phi", _dp),
+        ("nfreq", C.c_int32), ("nmu", C.c_int32), ("nph", C.c_int32), ("nap", C.c_int32),
+        ("aperture2", _dp), ("u_mu", _dp), ("sini_mu", _dp),
+        ("numin", C.c_double), ("lnurat", C.c_double),
+        ("iintmax", C.c_int32), ("taumax", C.c_double),
+    ]
+
+
+class OrcTallies(C.Structure):
+    _fields_ = [("dtauabs", _dp), ("nabs", _ip), ("dbig", _ip), ("killed", _ip),
+                ("sed", _fp), ("psed", _fp), ("image", _fp), ("counters", C.c_double * 16)]
+
+
+class OrcWalls(C.Structure):
+    _fields_ = [("tn1", C.c_double), ("tp1", C.c_double), ("tp2", C.c_double),
+                ("in1", C.c_int32), ("ip1", C.c_int32), ("ip2", C.c_int32)]
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def build_ref():
+    """Compile the reference's MPI-free leaves (needs /root/reference)."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "ref")], check=True)
+
+
+_orc = None
+_ref = None
+
+
+def oracle():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = C.CDLL(ORACLE_SO)
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_packet_word.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32]
+        L.orc_packet_word.restype = C.c_uint32
+        L.orc_lecuyer_set_all_seeds.argtypes = [C.c_int32, C.c_int32]
+        L.orc_lecuyer_set_generator.argtypes = [C.c_int32]
+        L.orc_lecuyer_ran.restype = C.c_double
+        L.orc_bbfreq_lecuyer.argtypes = [C.c_double]
+        L.orc_bbfreq_lecuyer.restype = C.c_double
+        W = C.POINTER(OrcWalls)
+        L.orc_reset_t.argtypes = [W]
+        L.orc_insert_t.argtypes = [W, C.c_double, C.c_int32]
+        L.orc_find_next_wall.argtypes = [W, C.c_int, _dp, _ip]
+        L.orc_find_next_wall.restype = C.c_int
+        L.orc_radface.argtypes = [W] + [C.c_double] * 8
+        L.orc_thetaface.argtypes = [W] + [C.c_double] * 8
+        L.orc_phiface.argtypes = [W] + [C.c_double] * 10
+        L.orc_locate.argtypes = [_dp, C.c_int32, C.c_double]
+        L.orc_locate.restype = C.c_int32
+        L.orc_findangle3.argtypes = [C.c_double] * 10 + [_dp] * 4
+        L.orc_dustmat.argtypes = [C.c_double] * 4 + [_dp]
+        L.orc_stokes_apply.argtypes = [C.c_double] * 4 + [_dp, _dp]
+        L.orc_stokespeel.argtypes = [C.c_double, C.c_double, _dp] + [C.c_double] * 6
+        L.orc_lucydustfreq.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_int32, C.c_double]
+        L.orc_lucydustfreq.restype = C.c_double
+        L.orc_dusttemp.argtypes = [C.POINTER(OrcModel), C.c_double, C.c_double, C.c_int32, C.c_double]
+        L.orc_dusttemp.restype = C.c_double
+        L.orc_tfinal.argtypes = [C.POINTER(OrcModel), _dp, C.c_double, _dp]
+        L.orc_trace_ray.argtypes = [C.POINTER(OrcModel), _dp, _dp, _ip, C.c_int, C.c_double, C.c_int32,
+                                    _ip, _ip, _dp, _ip]
+        L.orc_trace_ray.restype = C.c_int32
+        L.orc_run_packets.argtypes = [C.POINTER(OrcModel), _dp, C.POINTER(OrcTallies), C.c_uint64,
+                                      C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _dp]
+        L.orc_run_packets.restype = C.c_int32
+        _orc = L
+    return _orc
+
+
+def ref():
+    """The reference's own compiled leaf routines, or None when unbuilt."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        L = C.CDLL(REF_SO)
+        L.ref_walls_state.argtypes = [_dp, _ip]
+        L.ref_radface.argtypes = [_dp, _dp, C.c_double, C.c_double]
+        L.ref_thetaface.argtypes = [_dp, _dp, C.c_double, C.c_double]
+        L.ref_phiface.argtypes = [C.c_double] * 4 + [_dp, _dp]
+        L.ref_find_next_wall.argtypes = [C.c_int, _dp, _ip]
+        L.ref_findangle3.argtypes = [_dp, _dp]
+        L.ref_check.argtypes = [C.c_double]
+        L.ref_check.restype = C.c_double
+        L.ref_locate.argtypes = [_dp, C.c_int32, C.c_double]
+        L.ref_locate.restype = C.c_int32
+        L.ref_user_set_all.argtypes = [C.c_int32, C.c_int32, C.c_int32]
+        L.ref_ran.restype = C.c_double
+        L.ref_bbfreq.argtypes = [C.c_double]
+        L.ref_bbfreq.restype = C.c_double
+        L.ref_bnu.argtypes = [C.c_double, C.c_double]
+        L.ref_bnu.restype = C.c_double
+        L.ref_open_log()
+        _ref = L
+    return _ref
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OracleModel:
+    """Holds the numpy arrays alive and exposes the C struct."""
+
+    def __init__(self, m):
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        self.arrs = dict(
+            rarr=f(m.rarr), r2arr=f(m.r2arr), thetarr=f(m.thetarr), costarr=f(m.costarr),
+            tan2arr=f(m.tan2arr), phiarr=f(m.phiarr), aarr=f(m.aarr), barr=f(m.barr), vcell=f(m.vcell),
+            rho=f(m.rho), species=np.ascontiguousarray(m.species, dtype=np.uint8),
+            nudust=f(m.nudust), kappad=f(m.kappad), adust=f(m.adust), gdust=f(m.gdust),
+            pldust=f(m.pldust), kappav=f(m.kappav), lTint=f(m.lTint), Tint=f(m.Tint),
+            nuint=f(m.nuint), kapint=f(m.kapint), kappap=f(m.kappap),
+            coste=f(np.cos(m.thete)), sinte=f(np.sin(m.thete)), cospe=f(np.cos(m.phie)),
+            sinpe=f(np.sin(m.phie)), phie=f(m.phie),
+            band_nw=np.ascontiguousarray(m.band_nw, dtype=np.int32), band_freq=f(m.band_freq),
+            band_phi=f(m.band_phi), aperture2=f(m.aperture2), u_mu=f(m.u_mu), sini_mu=f(m.sini_mu))
+        s = OrcModel()
+        for k, v in self.arrs.items():
+            t = C.c_uint8 if v.dtype == np.uint8 else (C.c_int32 if v.dtype == np.int32 else C.c_double)
+            setattr(s, k, _p(v, t))
+        s.nrg, s.ntg, s.npg = m.nrg, m.ntg, m.npg
+        for i in range(11):
+            s.species_opac[i] = int(m.species_opac[i])
+        s.nopac, s.nlam, s.nT, s.nnu = m.nopac, m.nlam, m.nT, m.nnu
+        s.rstar, s.tstar, s.rmin = m.rstar, m.tstar, m.rmin
+        s.ipeel, s.npeel, s.nx, s.rmaxi = m.ipeel, m.npeel, m.nx, m.rmaxi
+        s.nband, s.nwmax = m.nband, m.nwmax
+        s.nfreq, s.nmu, s.nph, s.nap = m.nfreq, m.nmu, m.nph, m.nap
+        s.numin, s.lnurat = m.numin, m.lnurat
+        s.iintmax, s.taumax = 10000, 25.0
+        self.s = s
+        self.m = m
+
+
+def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_mode=0, trace=False):
+    """Oracle packet loop; returns tallies named like Engine.tallies()."""
+    L = oracle()
+    om = OracleModel(m)
+    cs = m.cell_shape
+    out = dict(dtauabs=np.zeros(cs), nabs=np.zeros(cs, np.int32), dbig=np.zeros(cs, np.int32),
+               killed=np.zeros(cs, np.int32))
+    sed = np.zeros((10, 8, m.nap, m.nph, m.nmu, m.nfreq), np.float32)
+    psed = np.zeros((9, 8, m.nap, max(m.npeel, 1), m.nfreq), np.float32)
+    img = np.zeros((8, max(m.nband, 1), m.nx, m.nx, max(m.npeel, 1)), np.float32)
+    t = OrcTallies()
+    t.dtauabs = _p(out["dtauabs"], C.c_double)
+    t.nabs = _p(out["nabs"], C.c_int32)
+    t.dbig = _p(out["dbig"], C.c_int32)
+    t.killed = _p(out["killed"], C.c_int32)
+    t.sed, t.psed, t.image = _p(sed, C.c_float), _p(psed, C.c_float), _p(img, C.c_float)
+    td = np.ascontiguousarray(m.tdust_initial() if tdust is None else tdust, dtype=np.float64)
+    tr = np.zeros((npackets, 8)) if trace else None
+    rc = L.orc_run_packets(C.byref(om.s), _p(td, C.c_double), C.byref(t), int(first_packet), int(npackets),
+                           C.c_uint64(int(seed)), int(rng_mode), int(nthreads),
+                           _p(tr, C.c_double) if trace else None)
+    if rc != 0:
+        raise RuntimeError(f"orc_run_packets failed: {rc}")
+    from src3_mpi_cfitsio_amd.engine import COUNTERS, IMAGE_NAMES, PSED_NAMES, SED_NAMES
+    for i, n in enumerate(SED_NAMES):
+        out[n] = sed[i]
+    for i, n in enumerate(PSED_NAMES):
+        out[n] = psed[i]
+    for i, n in enumerate(IMAGE_NAMES):
+        out[n] = img[i]
+    out["counters"] = {k: t.counters[i] for i, k in enumerate(COUNTERS)}
+    if trace:
+        out["trace"] = tr
+    return out
+
+
+def tfinal(m, dtauabs, nphot):
+    L = oracle()
+    om = OracleModel(m)
+    d = np.ascontiguousarray(dtauabs, dtype=np.float64)
+    out = np.zeros(m.cell_shape)
+    L.orc_tfinal(C.byref(om.s), _p(d, C.c_double), float(nphot), _p(out, C.c_double))
+    return out
+
+
+def dusttemp(m, na, nphot, iop, mcell):
+    om = OracleModel(m)
+    return oracle().orc_dusttemp(C.byref(om.s), float(na), float(nphot), int(iop), float(mcell))
+
+
+def lucydustfreq(m, T0, iop, xi):
+    om = OracleModel(m)
+    return oracle().orc_lucydustfreq(C.byref(om.s), float(T0), int(iop), float(xi))
+
+
+def trace_ray(m, pos, u, ijk, on_wall, tau_stop, max_steps=4096, om=None):
+    L = oracle()
+    om = om or OracleModel(m)
+    p = np.array(pos, dtype=np.float64)
+    uu = np.array(u, dtype=np.float64)
+    ij = np.array(ijk, dtype=np.int32)
+    faces = np.zeros(max_steps, np.int32)
+    cells = np.zeros((max_steps, 3), np.int32)
+    tau = C.c_double()
+    st = C.c_int32()
+    n = L.orc_trace_ray(C.byref(om.s), _p(p, C.c_double), _p(uu, C.c_double), _p(ij, C.c_int32), int(on_wall),
+                        float(tau_stop), int(max_steps), _p(faces, C.c_int32), _p(cells, C.c_int32),
+                        C.byref(tau), C.byref(st))
+    return dict(n=n, faces=faces[:max(n, 0)], cells=cells[:max(n, 0)], pos=p, ijk=ij, tau=tau.value,
+                status=st.value)

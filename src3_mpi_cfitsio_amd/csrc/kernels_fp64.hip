@@ -1,0 +1,52 @@
+// kernels_fp64.hip -- FP64 instantiation (built with -ffp-contract=off so
+// every product and sum rounds exactly as in the reference's Fortran), plus
+// the FP64-only Lucy temperature and fill kernels.
+#define MCRT_FP64_ONLY_KERNELS 1
+#include "mcrt_kernels.h"
+#include "mcrt_launch.h"
+
+namespace mcrt {
+
+hipError_t launch_packets_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_packets<double>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_trace<double>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mueller_fp64(int n, int kind, const double* par, double* dir, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_mueller<double>, dim3((n + 255) / 256), dim3(256), 0, s, n, kind, par, dir, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s) {
+  size_t ncell = (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1);
+  if (mode == 0) {
+    int nh = (m.nrg - 1) * ((m.ntg - 1) / 2);
+    hipLaunchKernelGGL(k_tfinal_half, dim3((nh + 255) / 256), dim3(256), 0, s, m, dtauabs, nphot, tdave);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tfinal_fill, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, m, tdave);
+  } else {
+    hipLaunchKernelGGL(k_tcell, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, m, dtauabs, nphot, tdave);
+  }
+  return hipGetLastError();
+}
+
+template <class V>
+static hipError_t fill(V* p, size_t n, V v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill<V>, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s) { return fill(p, n, v, s); }
+hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s) { return fill(p, n, v, s); }
+hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t s) { return fill(p, n, v, s); }
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s) { return fill(p, n, v, s); }
+
+}  // namespace mcrt

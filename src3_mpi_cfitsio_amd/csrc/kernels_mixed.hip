@@ -1,0 +1,24 @@
+// kernels_mixed.hip -- mixed-precision instantiation: FP32 transport,
+// sampling and Stokes algebra; FP64 radial-shell roots (radface) and FP64
+// absorbed-energy atomics (dtauabs).
+#include "mcrt_kernels.h"
+#include "mcrt_launch.h"
+
+namespace mcrt {
+
+hipError_t launch_packets_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_packets<float>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_trace<float>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mueller_mixed(int n, int kind, const double* par, double* dir, double* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_mueller<float>, dim3((n + 255) / 256), dim3(256), 0, s, n, kind, par, dir, st);
+  return hipGetLastError();
+}
+
+}  // namespace mcrt

@@ -1,0 +1,543 @@
+// mcrt_abi.hip -- the extern "C" boundary (include/mcrt.h): context,
+// device-resident model state, launches, tally read-back.
+#include "../../include/mcrt.h"
+#include "mcrt_kernels.h"
+#include "mcrt_launch.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace mcrt;
+
+struct mcrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  std::string err;
+  std::map<std::string, void*> buf;       // device allocations by name
+  std::map<std::string, size_t> bytes;
+  DeviceModel dm{};
+  DeviceTallies dt{};
+  bool has_grid = false, has_medium = false, has_dust = false, has_source = false, has_observer = false, has_tdust = false;
+  size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
+  int64_t trace_n = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool ev_valid = false;
+  int cu_count = 256;
+};
+
+namespace {
+
+int fail(mcrt_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(mcrt_ctx* c, hipError_t e, const char* what) {
+  return fail(c, MCRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHECK(c, expr)                                  \
+  do {                                                     \
+    hipError_t _e = (expr);                                \
+    if (_e != hipSuccess) return hipfail((c), _e, #expr);  \
+  } while (0)
+
+// (re)allocate a named device buffer and optionally upload host data
+int dev_alloc(mcrt_ctx* c, const std::string& name, size_t nbytes, void** out) {
+  auto it = c->buf.find(name);
+  if (it != c->buf.end()) {
+    if (c->bytes[name] == nbytes) { *out = it->second; return MCRT_OK; }
+    HIPCHECK(c, hipFree(it->second));
+    c->buf.erase(it);
+  }
+  void* p = nullptr;
+  HIPCHECK(c, hipMalloc(&p, nbytes ? nbytes : 16));
+  c->buf[name] = p;
+  c->bytes[name] = nbytes;
+  *out = p;
+  return MCRT_OK;
+}
+
+template <class V>
+int upload(mcrt_ctx* c, const std::string& name, const V* host, size_t n, const V** out) {
+  void* p = nullptr;
+  int rc = dev_alloc(c, name, n * sizeof(V), &p);
+  if (rc) return rc;
+  if (n) HIPCHECK(c, hipMemcpyAsync(p, host, n * sizeof(V), hipMemcpyHostToDevice, c->stream));
+  *out = (const V*)p;
+  return MCRT_OK;
+}
+
+template <class V>
+int upload_cast(mcrt_ctx* c, const std::string& name, const double* host, size_t n, const V** out) {
+  std::vector<V> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (V)host[i];
+  int rc = upload<V>(c, name, tmp.data(), n, out);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));   // tmp goes out of scope
+  return MCRT_OK;
+}
+
+// random.f90:74-92: partial sums of 1/j^4, smallest term first
+std::vector<double> zeta_partial_sums() {
+  std::vector<double> z(145, 0.0);
+  for (int j = 145; j >= 1; --j) {
+    double temp = 1.0 / (double)(j * j * j * j);
+    for (int i = 145; i >= j; --i) z[i - 1] += temp;
+  }
+  return z;
+}
+
+int ensure_tallies(mcrt_ctx* c) {
+  if (!(c->has_grid && c->has_observer)) return fail(c, MCRT_E_STATE, "grid and observer must be set before tallies");
+  DeviceModel& m = c->dm;
+  c->ncell = (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1);
+  c->nsed = (size_t)10 * m.nfreq * m.nmu * m.nph * m.nap * kNo;
+  c->npsed = (size_t)9 * m.nfreq * m.npeel * m.nap * kNo;
+  c->nimage = (size_t)8 * m.nband * m.nx * m.nx * m.npeel;
+  void* p;
+  int rc;
+  if ((rc = dev_alloc(c, "t.dtauabs", c->ncell * 8, &p))) return rc;
+  c->dt.dtauabs = (double*)p;
+  if ((rc = dev_alloc(c, "t.nabs", c->ncell * 4, &p))) return rc;
+  c->dt.nabs = (int*)p;
+  if ((rc = dev_alloc(c, "t.dbig", c->ncell * 4, &p))) return rc;
+  c->dt.dbig = (int*)p;
+  if ((rc = dev_alloc(c, "t.killed", c->ncell * 4, &p))) return rc;
+  c->dt.killed = (int*)p;
+  if ((rc = dev_alloc(c, "t.sed", c->nsed * 4, &p))) return rc;
+  c->dt.sed = (float*)p;
+  if ((rc = dev_alloc(c, "t.psed", c->npsed * 4, &p))) return rc;
+  c->dt.psed = (float*)p;
+  if ((rc = dev_alloc(c, "t.image", c->nimage * 4, &p))) return rc;
+  c->dt.image = (float*)p;
+  if ((rc = dev_alloc(c, "t.counters", C_NCOUNTERS * 8, &p))) return rc;
+  c->dt.counters = (unsigned long long*)p;
+  return MCRT_OK;
+}
+
+int zero_tallies(mcrt_ctx* c) {
+  int rc = ensure_tallies(c);
+  if (rc) return rc;
+  HIPCHECK(c, launch_fill_f64(c->dt.dtauabs, c->ncell, 0.0, c->stream));
+  HIPCHECK(c, launch_fill_i32(c->dt.nabs, c->ncell, 0, c->stream));
+  HIPCHECK(c, launch_fill_i32(c->dt.dbig, c->ncell, 0, c->stream));
+  HIPCHECK(c, launch_fill_i32(c->dt.killed, c->ncell, 0, c->stream));
+  HIPCHECK(c, launch_fill_f32(c->dt.sed, c->nsed, 0.f, c->stream));
+  HIPCHECK(c, launch_fill_f32(c->dt.psed, c->npsed, 0.f, c->stream));
+  HIPCHECK(c, launch_fill_f32(c->dt.image, c->nimage, 0.f, c->stream));
+  HIPCHECK(c, launch_fill_u64(c->dt.counters, C_NCOUNTERS, 0ull, c->stream));
+  return MCRT_OK;
+}
+
+bool ready(mcrt_ctx* c) {
+  return c->has_grid && c->has_medium && c->has_dust && c->has_source && c->has_observer && c->has_tdust;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcrt_create(int device, mcrt_ctx** out) {
+  if (!out) return MCRT_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MCRT_E_NODEVICE;
+  if (device < 0 || device >= n) return MCRT_E_ARG;
+  mcrt_ctx* c = new mcrt_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete c; return MCRT_E_HIP; }
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCRT_E_HIP; }
+  c->stream = c->own_stream;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->cu_count = prop.multiProcessorCount;
+  (void)hipEventCreate(&c->ev0);
+  (void)hipEventCreate(&c->ev1);
+  c->dm.max_segments = 1000000;
+  c->dm.iintmax = 10000;
+  c->dm.taumax = 25.0;
+  std::vector<double> z = zeta_partial_sums();
+  const double* dz;
+  if (upload<double>(c, "inczeta", z.data(), z.size(), &dz) != MCRT_OK || hipStreamSynchronize(c->stream) != hipSuccess) {
+    mcrt_destroy(c);
+    return MCRT_E_HIP;
+  }
+  *out = c;
+  return MCRT_OK;
+}
+
+int mcrt_destroy(mcrt_ctx* c) {
+  if (!c) return MCRT_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->buf) (void)hipFree(kv.second);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return MCRT_OK;
+}
+
+const char* mcrt_last_error(const mcrt_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mcrt_set_stream(mcrt_ctx* c, void* stream) {
+  if (!c) return MCRT_E_ARG;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  return MCRT_OK;
+}
+
+int mcrt_synchronize(mcrt_ctx* c) {
+  if (!c) return MCRT_E_ARG;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+int mcrt_set_grid(mcrt_ctx* c, int32_t nrg, int32_t ntg, int32_t npg, const double* rarr, const double* r2arr,
+                  const double* thetarr, const double* costarr, const double* tan2arr, const double* phiarr,
+                  const double* aarr, const double* barr, const double* vcell) {
+  if (!c) return MCRT_E_ARG;
+  if (nrg < 3 || ntg < 2 || npg < 2 || !rarr || !r2arr || !thetarr || !costarr || !tan2arr || !phiarr || !aarr || !barr || !vcell)
+    return fail(c, MCRT_E_ARG, "mcrt_set_grid: need nrg>=3, ntg>=2, npg>=2 and all arrays");
+  if (ntg > 2 && (ntg % 2) == 0) return fail(c, MCRT_E_ARG, "mcrt_set_grid: ntg must be odd (equator wall)");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DeviceModel& m = c->dm;
+  m.nrg = nrg; m.ntg = ntg; m.npg = npg;
+  size_t ncell = (size_t)(nrg - 1) * (ntg - 1) * (npg - 1);
+  int rc = 0;
+  rc |= upload(c, "rarr", rarr, nrg, &m.rarr);
+  rc |= upload(c, "r2arr", r2arr, nrg, &m.r2arr);
+  rc |= upload(c, "thetarr", thetarr, ntg, &m.thetarr);
+  rc |= upload(c, "costarr", costarr, ntg, &m.costarr);
+  rc |= upload(c, "tan2arr", tan2arr, ntg, &m.tan2arr);
+  rc |= upload(c, "phiarr", phiarr, npg, &m.phiarr);
+  rc |= upload(c, "aarr", aarr, npg, &m.aarr);
+  rc |= upload(c, "barr", barr, npg, &m.barr);
+  rc |= upload(c, "vcell", vcell, ncell, &m.vcell);
+  rc |= upload_cast<float>(c, "r2arr_f", r2arr, nrg, &m.r2arr_f);
+  rc |= upload_cast<float>(c, "costarr_f", costarr, ntg, &m.costarr_f);
+  rc |= upload_cast<float>(c, "tan2arr_f", tan2arr, ntg, &m.tan2arr_f);
+  rc |= upload_cast<float>(c, "phiarr_f", phiarr, npg, &m.phiarr_f);
+  rc |= upload_cast<float>(c, "aarr_f", aarr, npg, &m.aarr_f);
+  rc |= upload_cast<float>(c, "barr_f", barr, npg, &m.barr_f);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_grid = true;
+  c->has_medium = c->has_tdust = false;   // per-cell arrays must match the new grid
+  c->ncell = ncell;
+  return MCRT_OK;
+}
+
+int mcrt_set_medium(mcrt_ctx* c, const double* rho, const uint8_t* species, const int32_t* species_opac) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->has_grid) return fail(c, MCRT_E_STATE, "mcrt_set_medium: set the grid first");
+  if (!rho || !species || !species_opac) return fail(c, MCRT_E_ARG, "mcrt_set_medium: null array");
+  for (int i = 0; i <= 10; ++i)
+    if (species_opac[i] < 0 || species_opac[i] >= kMaxOpac) return fail(c, MCRT_E_ARG, "mcrt_set_medium: species_opac out of range");
+  for (size_t i = 0; i < c->ncell; ++i) {
+    int s = species[i];
+    if (s > 10) return fail(c, MCRT_E_ARG, "mcrt_set_medium: species code above 10");
+    if (s >= 5 && s <= 8 && rho[i] > 0.0) return fail(c, MCRT_E_ARG, "mcrt_set_medium: small-grain species (5-8) are not supported");
+    if (!(rho[i] >= 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_medium: negative or NaN density");
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  DeviceModel& m = c->dm;
+  int rc = 0;
+  rc |= upload(c, "rho64", rho, c->ncell, &m.rho64);
+  rc |= upload_cast<float>(c, "rho32", rho, c->ncell, &m.rho32);
+  rc |= upload(c, "species", species, c->ncell, &m.species);
+  if (rc) return rc;
+  for (int i = 0; i <= 10; ++i) m.species_opac[i] = species_opac[i];
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_medium = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust, const double* kappad, const double* adust,
+                  const double* gdust, const double* pldust, const double* kappav, int32_t nT, int32_t nnu,
+                  const double* lTint, const double* Tint, const double* nuint, const double* kapint, const double* kappap) {
+  if (!c) return MCRT_E_ARG;
+  if (nopac < 1 || nopac > kMaxOpac) return fail(c, MCRT_E_ARG, "mcrt_set_dust: nopac must be 1..4");
+  if (nlam < 2 || nT < 2 || nnu < 2) return fail(c, MCRT_E_ARG, "mcrt_set_dust: nlam, nT, nnu must be >= 2");
+  if (!nudust || !kappad || !adust || !gdust || !pldust || !kappav || !lTint || !Tint || !nuint || !kapint || !kappap)
+    return fail(c, MCRT_E_ARG, "mcrt_set_dust: null array");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DeviceModel& m = c->dm;
+  m.nopac = nopac; m.nlam = nlam; m.nT = nT; m.nnu = nnu;
+  size_t nl = (size_t)nopac * nlam;
+  int rc = 0;
+  rc |= upload(c, "nudust", nudust, nl, &m.nudust);
+  rc |= upload(c, "kappad", kappad, nl, &m.kappad);
+  rc |= upload(c, "adust", adust, nl, &m.adust);
+  rc |= upload(c, "gdust", gdust, nl, &m.gdust);
+  rc |= upload(c, "pldust", pldust, nl, &m.pldust);
+  rc |= upload(c, "lTint", lTint, nT, &m.lTint);
+  rc |= upload(c, "Tint", Tint, nT, &m.Tint);
+  rc |= upload(c, "nuint", nuint, nnu, &m.nuint);
+  rc |= upload(c, "kapint", kapint, (size_t)nopac * nT * nnu, &m.kapint);
+  rc |= upload(c, "kappap", kappap, (size_t)nopac * nT, &m.kappap);
+  if (rc) return rc;
+  for (int i = 0; i < kMaxOpac; ++i) m.kappav[i] = (i < nopac) ? kappav[i] : 1.0;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_dust = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_source(mcrt_ctx* c, double rstar, double tstar, double rmin) {
+  if (!c) return MCRT_E_ARG;
+  if (!(rstar > 0.0) || !(tstar > 0.0) || !(rmin > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_source: rstar, tstar, rmin must be > 0");
+  c->dm.rstar = rstar; c->dm.tstar = tstar; c->dm.rmin = rmin;
+  c->has_source = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_observer(mcrt_ctx* c, int32_t ipeel, int32_t npeel, const double* thete, const double* phie, int32_t nx,
+                      double rmaxi, int32_t nband, int32_t nwmax, const int32_t* band_nw, const double* band_freq,
+                      const double* band_phi, int32_t nfreq, int32_t nmu, int32_t nph, int32_t nap,
+                      const double* aperture2, const double* u_mu, double numin, double lnurat) {
+  if (!c) return MCRT_E_ARG;
+  if (npeel < 0 || (ipeel == 1 && npeel < 1) || nx < 1 || nband < 0 || nfreq < 1 || nmu < 1 || nph < 1 || nap < 1)
+    return fail(c, MCRT_E_ARG, "mcrt_set_observer: bad sizes");
+  if (npeel > 0 && (!thete || !phie)) return fail(c, MCRT_E_ARG, "mcrt_set_observer: null peel angles");
+  if (nband > 0 && (!band_nw || !band_freq || !band_phi || nwmax < 3)) return fail(c, MCRT_E_ARG, "mcrt_set_observer: bad band tables");
+  for (int b = 0; b < nband; ++b)
+    if (band_nw[b] < 3 || band_nw[b] > nwmax) return fail(c, MCRT_E_ARG, "mcrt_set_observer: band_nw must be 3..nwmax");
+  if (!aperture2 || !u_mu || !(numin > 0.0) || !(lnurat > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_observer: bad SED binning");
+  HIPCHECK(c, hipSetDevice(c->device));
+  DeviceModel& m = c->dm;
+  std::vector<double> pe((size_t)5 * (npeel > 0 ? npeel : 1), 0.0);
+  for (int k = 0; k < npeel; ++k) {
+    pe[5 * k + 0] = cos(thete[k]);
+    pe[5 * k + 1] = sin(thete[k]);
+    pe[5 * k + 2] = cos(phie[k]);
+    pe[5 * k + 3] = sin(phie[k]);
+    pe[5 * k + 4] = phie[k];
+  }
+  std::vector<double> sini(nmu);
+  for (int i = 0; i < nmu; ++i) sini[i] = sqrt(1.0 - u_mu[i] * u_mu[i]);   // newdisktherm_5_fuv.f90:114-116
+  m.ipeel = ipeel; m.npeel = npeel; m.nx = nx; m.rmaxi = rmaxi;
+  m.nband = nband; m.nwmax = nwmax;
+  m.nfreq = nfreq; m.nmu = nmu; m.nph = nph; m.nap = nap;
+  m.numin = numin; m.lnurat = lnurat;
+  int rc = 0;
+  rc |= upload(c, "peel", pe.data(), pe.size(), &m.peel);
+  if (nband > 0) {
+    rc |= upload(c, "band_nw", (const int*)band_nw, nband, &m.band_nw);
+    rc |= upload(c, "band_freq", band_freq, (size_t)nband * nwmax, &m.band_freq);
+    rc |= upload(c, "band_phi", band_phi, (size_t)nband * nwmax, &m.band_phi);
+  }
+  rc |= upload(c, "aperture2", aperture2, nap, &m.aperture2);
+  rc |= upload(c, "u_mu", u_mu, nmu, &m.u_mu);
+  rc |= upload(c, "sini_mu", sini.data(), nmu, &m.sini_mu);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_observer = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_tdust(mcrt_ctx* c, const double* tdust) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->has_grid) return fail(c, MCRT_E_STATE, "mcrt_set_tdust: set the grid first");
+  if (!tdust) return fail(c, MCRT_E_ARG, "mcrt_set_tdust: null array");
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = upload(c, "tdust", tdust, c->ncell, &c->dm.tdust);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_tdust = true;
+  return MCRT_OK;
+}
+
+int mcrt_zero_tallies(mcrt_ctx* c) {
+  if (!c) return MCRT_E_ARG;
+  HIPCHECK(c, hipSetDevice(c->device));
+  return zero_tallies(c);
+}
+
+int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64_t seed, int32_t precision) {
+  if (!c) return MCRT_E_ARG;
+  if (!ready(c)) return fail(c, MCRT_E_STATE, "mcrt_run_packets: grid, medium, dust, source, observer and tdust must be set");
+  if (npackets < 0 || first_packet < 0) return fail(c, MCRT_E_ARG, "mcrt_run_packets: negative packet range");
+  if (precision != MCRT_FP64 && precision != MCRT_MIXED) return fail(c, MCRT_E_ARG, "mcrt_run_packets: unknown precision");
+  if (c->trace_n > 0 && npackets > c->trace_n) return fail(c, MCRT_E_ARG, "mcrt_run_packets: more packets than the trace buffer holds");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (!c->dt.dtauabs) {
+    int rc = zero_tallies(c);
+    if (rc) return rc;
+  }
+  if (npackets == 0) return MCRT_OK;
+  RunParams rp;
+  rp.first_packet = (uint64_t)first_packet;
+  rp.npackets = (uint64_t)npackets;
+  rp.seed = seed;
+  rp.inczeta = (const double*)c->buf["inczeta"];
+  DeviceTallies t = c->dt;
+  t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
+  int64_t blocks = (npackets + 255) / 256;
+  int64_t cap = (int64_t)c->cu_count * 64;
+  int grid = (int)(blocks < cap ? blocks : cap);
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  hipError_t e = (precision == MCRT_FP64) ? launch_packets_fp64(c->dm, t, rp, grid, c->stream)
+                                          : launch_packets_mixed(c->dm, t, rp, grid, c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "packet kernel launch");
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  c->ev_valid = true;
+  return MCRT_OK;
+}
+
+int mcrt_last_run_ms(mcrt_ctx* c, float* ms) {
+  if (!c || !ms) return MCRT_E_ARG;
+  if (!c->ev_valid) return fail(c, MCRT_E_STATE, "mcrt_last_run_ms: no run recorded");
+  HIPCHECK(c, hipEventSynchronize(c->ev1));
+  HIPCHECK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return MCRT_OK;
+}
+
+int mcrt_get_tallies(mcrt_ctx* c, double* dtauabs, int32_t* nabs, int32_t* dbig, int32_t* killed, float* sed, float* psed,
+                     float* image, double* counters) {
+  if (!c) return MCRT_E_ARG;
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (!c->dt.dtauabs) {
+    int rc = zero_tallies(c);
+    if (rc) return rc;
+  }
+  hipStream_t s = c->stream;
+  if (dtauabs) HIPCHECK(c, hipMemcpyAsync(dtauabs, c->dt.dtauabs, c->ncell * 8, hipMemcpyDeviceToHost, s));
+  if (nabs) HIPCHECK(c, hipMemcpyAsync(nabs, c->dt.nabs, c->ncell * 4, hipMemcpyDeviceToHost, s));
+  if (dbig) HIPCHECK(c, hipMemcpyAsync(dbig, c->dt.dbig, c->ncell * 4, hipMemcpyDeviceToHost, s));
+  if (killed) HIPCHECK(c, hipMemcpyAsync(killed, c->dt.killed, c->ncell * 4, hipMemcpyDeviceToHost, s));
+  if (sed) HIPCHECK(c, hipMemcpyAsync(sed, c->dt.sed, c->nsed * 4, hipMemcpyDeviceToHost, s));
+  if (psed && c->npsed) HIPCHECK(c, hipMemcpyAsync(psed, c->dt.psed, c->npsed * 4, hipMemcpyDeviceToHost, s));
+  if (image && c->nimage) HIPCHECK(c, hipMemcpyAsync(image, c->dt.image, c->nimage * 4, hipMemcpyDeviceToHost, s));
+  unsigned long long cnt[C_NCOUNTERS];
+  HIPCHECK(c, hipMemcpyAsync(cnt, c->dt.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (counters)
+    for (int i = 0; i < C_NCOUNTERS; ++i) counters[i] = (double)cnt[i];
+  return MCRT_OK;
+}
+
+int mcrt_dusttemp(mcrt_ctx* c, double nphot, int32_t mode, int32_t update_tdust, double* tdave) {
+  if (!c) return MCRT_E_ARG;
+  if (!(c->has_grid && c->has_medium && c->has_dust && c->has_source)) return fail(c, MCRT_E_STATE, "mcrt_dusttemp: model not set");
+  if (!(nphot > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_dusttemp: nphot must be > 0");
+  if (mode != MCRT_T_REFERENCE && mode != MCRT_T_PERCELL) return fail(c, MCRT_E_ARG, "mcrt_dusttemp: unknown mode");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (!c->dt.dtauabs) {
+    int rc = zero_tallies(c);
+    if (rc) return rc;
+  }
+  void* p;
+  int rc = dev_alloc(c, "tdave", c->ncell * 8, &p);
+  if (rc) return rc;
+  double* d_tdave = (double*)p;
+  HIPCHECK(c, launch_tfinal(c->dm, c->dt.dtauabs, nphot, mode, d_tdave, c->stream));
+  if (update_tdust) {
+    void* q;
+    if ((rc = dev_alloc(c, "tdust", c->ncell * 8, &q))) return rc;
+    HIPCHECK(c, hipMemcpyAsync(q, d_tdave, c->ncell * 8, hipMemcpyDeviceToDevice, c->stream));
+    c->dm.tdust = (const double*)q;
+    c->has_tdust = true;
+  }
+  if (tdave) HIPCHECK(c, hipMemcpyAsync(tdave, d_tdave, c->ncell * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+int mcrt_trace_rays(mcrt_ctx* c, int32_t precision, int32_t n, const double* pos, const double* u, const int32_t* ijk,
+                    const uint8_t* on_wall, const double* tau_stop, int32_t max_steps, int32_t* faces, int32_t* cells,
+                    double* pos_out, int32_t* ijk_out, double* tau_out, int32_t* nsteps, int32_t* status) {
+  if (!c) return MCRT_E_ARG;
+  if (!(c->has_grid && c->has_medium)) return fail(c, MCRT_E_STATE, "mcrt_trace_rays: grid and medium must be set");
+  if (n < 0 || max_steps < 1) return fail(c, MCRT_E_ARG, "mcrt_trace_rays: bad sizes");
+  if (n == 0) return MCRT_OK;
+  if (!pos || !u || !ijk || !on_wall || !tau_stop || !faces || !cells || !pos_out || !ijk_out || !tau_out || !nsteps || !status)
+    return fail(c, MCRT_E_ARG, "mcrt_trace_rays: null array");
+  for (int r = 0; r < n; ++r) {
+    const int32_t* q = ijk + 3 * r;
+    if (q[0] < 1 || q[0] > c->dm.nrg - 1 || q[1] < 1 || q[1] > c->dm.ntg - 1 || q[2] < 1 || q[2] > c->dm.npg - 1)
+      return fail(c, MCRT_E_ARG, "mcrt_trace_rays: start cell outside the grid");
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  TraceArgs a;
+  a.n = n; a.max_steps = max_steps;
+  int rc = 0;
+  rc |= upload(c, "tr.pos", pos, (size_t)3 * n, &a.pos);
+  rc |= upload(c, "tr.u", u, (size_t)3 * n, &a.u);
+  rc |= upload(c, "tr.tau", tau_stop, (size_t)n, &a.tau_stop);
+  rc |= upload(c, "tr.ijk", (const int*)ijk, (size_t)3 * n, &a.ijk);
+  rc |= upload(c, "tr.onw", on_wall, (size_t)n, &a.on_wall);
+  void* p;
+  size_t steps = (size_t)n * max_steps;
+  rc |= dev_alloc(c, "tr.faces", steps * 4, &p); a.faces = (int*)p;
+  rc |= dev_alloc(c, "tr.cells", steps * 12, &p); a.cells = (int*)p;
+  rc |= dev_alloc(c, "tr.ijko", (size_t)n * 12, &p); a.ijk_out = (int*)p;
+  rc |= dev_alloc(c, "tr.nst", (size_t)n * 4, &p); a.nsteps = (int*)p;
+  rc |= dev_alloc(c, "tr.st", (size_t)n * 4, &p); a.status = (int*)p;
+  rc |= dev_alloc(c, "tr.poso", (size_t)n * 24, &p); a.pos_out = (double*)p;
+  rc |= dev_alloc(c, "tr.tauo", (size_t)n * 8, &p); a.tau_out = (double*)p;
+  if (rc) return rc;
+  HIPCHECK(c, launch_fill_i32(a.faces, steps, 0, c->stream));
+  HIPCHECK(c, launch_fill_i32(a.cells, steps * 3, 0, c->stream));
+  hipError_t e = (precision == MCRT_FP64) ? launch_trace_fp64(c->dm, a, c->stream) : launch_trace_mixed(c->dm, a, c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "trace kernel launch");
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync(faces, a.faces, steps * 4, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(cells, a.cells, steps * 12, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(ijk_out, a.ijk_out, (size_t)n * 12, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(nsteps, a.nsteps, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(status, a.status, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(pos_out, a.pos_out, (size_t)n * 24, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(tau_out, a.tau_out, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+  return MCRT_OK;
+}
+
+int mcrt_mueller(mcrt_ctx* c, int32_t precision, int32_t n, int32_t kind, const double* par, double* dir, double* stokes) {
+  if (!c) return MCRT_E_ARG;
+  if (n < 0 || (kind != 0 && kind != 1)) return fail(c, MCRT_E_ARG, "mcrt_mueller: bad n or kind");
+  if (n == 0) return MCRT_OK;
+  if (!par || !stokes || (kind == 0 && !dir)) return fail(c, MCRT_E_ARG, "mcrt_mueller: null array");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const double* dpar;
+  void *pd, *ps;
+  int rc = upload(c, "mu.par", par, (size_t)9 * n, &dpar);
+  rc |= dev_alloc(c, "mu.dir", (size_t)5 * n * 8, &pd);
+  rc |= dev_alloc(c, "mu.s", (size_t)4 * n * 8, &ps);
+  if (rc) return rc;
+  if (kind == 0) HIPCHECK(c, hipMemcpyAsync(pd, dir, (size_t)5 * n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(ps, stokes, (size_t)4 * n * 8, hipMemcpyHostToDevice, c->stream));
+  hipError_t e = (precision == MCRT_FP64) ? launch_mueller_fp64(n, kind, dpar, (double*)pd, (double*)ps, c->stream)
+                                          : launch_mueller_mixed(n, kind, dpar, (double*)pd, (double*)ps, c->stream);
+  if (e != hipSuccess) return hipfail(c, e, "mueller kernel launch");
+  if (kind == 0) HIPCHECK(c, hipMemcpyAsync(dir, pd, (size_t)5 * n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(stokes, ps, (size_t)4 * n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+int mcrt_enable_trace(mcrt_ctx* c, int64_t n) {
+  if (!c || n < 0) return MCRT_E_ARG;
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (n == 0) { c->trace_n = 0; return MCRT_OK; }
+  void* p;
+  int rc = dev_alloc(c, "trace", (size_t)n * 8 * 8, &p);
+  if (rc) return rc;
+  HIPCHECK(c, launch_fill_f64((double*)p, (size_t)n * 8, 0.0, c->stream));
+  c->trace_n = n;
+  return MCRT_OK;
+}
+
+int mcrt_get_trace(mcrt_ctx* c, double* trace, int64_t n) {
+  if (!c || !trace) return MCRT_E_ARG;
+  if (n > c->trace_n) return fail(c, MCRT_E_ARG, "mcrt_get_trace: n exceeds the trace buffer");
+  HIPCHECK(c, hipMemcpyAsync(trace, c->buf["trace"], (size_t)n * 64, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,721 @@
+// mcrt_device.h -- device-side photon-packet transport for CDNA4 (gfx950).
+//
+// Templated on the transport real type T: T=double is the FP64 path
+// (compiled with -ffp-contract=off so it reproduces the reference arithmetic
+// bit for bit), T=float is the mixed-precision path (FP32 transport, sampling
+// and Stokes; FP64 radial-shell roots, FP64 absorbed-energy tally).
+//
+// Reference routines restated here (paths relative to src3_mpi_cfitsio/):
+//   closest_wall.f90, radface.f90, thetaface.f90, phiface.f90, testgrid.f90,
+//   tauint_3d_2_fuv.f90, propagate_5_fuv.f90, peeloff_3d_2_test.f90,
+//   stokes.f90, dustmat.f90, stokespeel.f90, findangle3.f90,
+//   initpspot.f90, initp_common.f90, initpabs.f90, reemit.f90,
+//   lucydustfreq_2_test.f90, random.f90 (bbfreq), dust_5.f90 (opacset),
+//   newdisktherm_5_fuv.f90 (stellar packet set-up and direct peel).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <float.h>
+
+namespace mcrt {
+
+constexpr int kMaxOpac = 4;     // opacity tables resident per packet
+constexpr int kNo = 8;          // SED components (tts_mod_5.f90:19)
+constexpr double kPi = 3.1415926535897932384626433;
+constexpr double kR2p = 2.0 * 3.1415926535897932384626433;
+constexpr double kRsol = 6.9598e10;
+constexpr double kTmaxK = 199999.0;
+
+// Counters accumulated per launch (atomic, 64-bit).
+enum Counter {
+  C_TOT = 0, C_FLUX, C_SFLUX, C_AFLUX, C_ABFLUX, C_SCOUNT, C_NSCAT, C_HIT, C_HTOT,
+  C_ESCAPED, C_KILLED, C_SEGMENTS, C_PEELSEG, C_PEELEVT, C_RESETS, C_ERRORS, C_NCOUNTERS
+};
+
+struct DeviceModel {
+  // grid walls, 1-based reference semantics stored 0-based (wall k at [k-1])
+  int nrg, ntg, npg;
+  const double *rarr, *r2arr, *thetarr, *costarr, *tan2arr, *phiarr, *aarr, *barr;
+  const double* vcell;           // cm^3 per cell
+  const float *costarr_f, *tan2arr_f, *phiarr_f, *aarr_f, *barr_f, *r2arr_f;
+  // medium: one thermal species per cell
+  const double* rho64;
+  const float* rho32;
+  const uint8_t* species;
+  int species_opac[11];
+  // dust
+  int nopac, nlam;
+  const double *nudust, *kappad, *adust, *gdust, *pldust;
+  double kappav[kMaxOpac];
+  int nT, nnu;
+  const double *lTint, *Tint, *nuint, *kapint, *kappap;
+  const double* tdust;
+  // source
+  double rstar, tstar, rmin;
+  // observer
+  int ipeel, npeel;
+  const double* peel;            // [npeel][5]: coste, sinte, cospe, sinpe, phie
+  int nx;
+  double rmaxi;
+  int nband, nwmax;
+  const int* band_nw;
+  const double *band_freq, *band_phi;
+  int nfreq, nmu, nph, nap;
+  const double *aperture2, *u_mu, *sini_mu;
+  double numin, lnurat;
+  int iintmax;
+  double taumax;
+  long long max_segments;        // per walk; guards against zero-length cycling
+};
+
+struct DeviceTallies {
+  double* dtauabs;
+  int *nabs, *dbig, *killed;
+  float *sed, *psed, *image;
+  unsigned long long* counters;  // C_NCOUNTERS
+  double* trace;                 // optional: 8 doubles per packet
+};
+
+__device__ __forceinline__ size_t cell_index(const DeviceModel& m, int ir, int it, int ip) {
+  return (size_t)(ir - 1) + (size_t)(m.nrg - 1) * ((size_t)(it - 1) + (size_t)(m.ntg - 1) * (size_t)(ip - 1));
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG: Philox4x32-10, key = run seed, counter = (n/4, packet).
+// ---------------------------------------------------------------------------
+struct Rng {
+  uint32_t k0, k1, p0, p1, n;
+  uint32_t buf[4];
+};
+
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet) {
+  r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32);
+  r.p0 = (uint32_t)packet; r.p1 = (uint32_t)(packet >> 32);
+  r.n = 0;
+}
+
+__device__ __forceinline__ uint32_t rng_word(Rng& r) {
+  if ((r.n & 3u) == 0u) philox(r.n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, r.buf);
+  uint32_t w = r.buf[r.n & 3u];
+  r.n++;
+  return w;
+}
+
+template <class T> __device__ __forceinline__ T uniform(Rng& r);
+template <> __device__ __forceinline__ double uniform<double>(Rng& r) {
+  return ((double)rng_word(r) + 0.5) * 2.3283064365386963e-10;
+}
+template <> __device__ __forceinline__ float uniform<float>(Rng& r) {
+  return ((float)(rng_word(r) >> 8) + 0.5f) * 5.9604644775390625e-8f;
+}
+
+// ---------------------------------------------------------------------------
+// Real-type helpers
+// ---------------------------------------------------------------------------
+template <class T> struct RT;
+template <> struct RT<double> {
+  static constexpr double huge = DBL_MAX;
+  __device__ static double sqrt_(double x) { return sqrt(x); }
+  __device__ static double sin_(double x) { return sin(x); }
+  __device__ static double cos_(double x) { return cos(x); }
+  __device__ static double acos_(double x) { return acos(x); }
+  __device__ static double exp_(double x) { return exp(x); }
+  __device__ static double log_(double x) { return log(x); }
+  __device__ static double atan2_(double y, double x) { return atan2(y, x); }
+  __device__ static double pow_(double x, double y) { return pow(x, y); }
+  __device__ static double fabs_(double x) { return fabs(x); }
+};
+template <> struct RT<float> {
+  static constexpr float huge = FLT_MAX;
+  __device__ static float sqrt_(float x) { return sqrtf(x); }
+  __device__ static float sin_(float x) { return sinf(x); }
+  __device__ static float cos_(float x) { return cosf(x); }
+  __device__ static float acos_(float x) { return acosf(x); }
+  __device__ static float exp_(float x) { return expf(x); }
+  __device__ static float log_(float x) { return logf(x); }
+  __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
+  __device__ static float pow_(float x, float y) { return powf(x, y); }
+  __device__ static float fabs_(float x) { return fabsf(x); }
+};
+
+template <class T> __device__ __forceinline__ const T* walls_cos(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_cos<double>(const DeviceModel& m) { return m.costarr; }
+template <> __device__ __forceinline__ const float* walls_cos<float>(const DeviceModel& m) { return m.costarr_f; }
+template <class T> __device__ __forceinline__ const T* walls_tan2(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_tan2<double>(const DeviceModel& m) { return m.tan2arr; }
+template <> __device__ __forceinline__ const float* walls_tan2<float>(const DeviceModel& m) { return m.tan2arr_f; }
+template <class T> __device__ __forceinline__ const T* walls_phi(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_phi<double>(const DeviceModel& m) { return m.phiarr; }
+template <> __device__ __forceinline__ const float* walls_phi<float>(const DeviceModel& m) { return m.phiarr_f; }
+template <class T> __device__ __forceinline__ const T* walls_a(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_a<double>(const DeviceModel& m) { return m.aarr; }
+template <> __device__ __forceinline__ const float* walls_a<float>(const DeviceModel& m) { return m.aarr_f; }
+template <class T> __device__ __forceinline__ const T* walls_b(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_b<double>(const DeviceModel& m) { return m.barr; }
+template <> __device__ __forceinline__ const float* walls_b<float>(const DeviceModel& m) { return m.barr_f; }
+template <class T> __device__ __forceinline__ const T* walls_r2(const DeviceModel& m);
+template <> __device__ __forceinline__ const double* walls_r2<double>(const DeviceModel& m) { return m.r2arr; }
+template <> __device__ __forceinline__ const float* walls_r2<float>(const DeviceModel& m) { return m.r2arr_f; }
+template <class T> __device__ __forceinline__ T cell_rho(const DeviceModel& m, size_t c);
+template <> __device__ __forceinline__ double cell_rho<double>(const DeviceModel& m, size_t c) { return m.rho64[c]; }
+template <> __device__ __forceinline__ float cell_rho<float>(const DeviceModel& m, size_t c) { return m.rho32[c]; }
+
+// ---------------------------------------------------------------------------
+// closest_wall.f90:13-66
+// ---------------------------------------------------------------------------
+template <class T> struct Walls {
+  T tn1, tp1, tp2;
+  int in1, ip1, ip2;
+  __device__ __forceinline__ void reset() {
+    tn1 = -RT<T>::huge; tp1 = RT<T>::huge; tp2 = RT<T>::huge;
+    in1 = 0; ip1 = 0; ip2 = 0;
+  }
+  __device__ __forceinline__ void insert(T t, int i) {
+    if (t < (T)0) {
+      if (t > tn1) { tn1 = t; in1 = i; }
+    } else {
+      if (t < tp1) { tp2 = tp1; ip2 = ip1; tp1 = t; ip1 = i; }
+      else if (t < tp2) { tp2 = t; ip2 = i; }
+    }
+  }
+  // returns false when no wall exists ("ERROR - no valid solution")
+  __device__ __forceinline__ bool next(bool on_wall, T& t, int& i) const {
+    if (on_wall) {
+      if (RT<T>::fabs_(tn1) < RT<T>::fabs_(tp1)) { t = tp1; i = ip1; }
+      else { t = tp2; i = ip2; }
+    } else { t = tp1; i = ip1; }
+    return !(t == RT<T>::huge);
+  }
+};
+
+// radface.f90:44-62.  The radial quadratic is always solved in FP64: r spans
+// 1..1e5 R* so |x|^2 - R^2 cancels catastrophically in FP32 on a shell.
+template <class T>
+__device__ __forceinline__ void radface(Walls<T>& w, T ux_, T uy_, T uz_, T x_, T y_, T z_, double R2_1, double R2_2) {
+  double ux = ux_, uy = uy_, uz = uz_, x1 = x_, y1 = y_, z1 = z_;
+  double bb = 2.0 * (x1 * ux + y1 * uy + uz * z1);
+  double cc = x1 * x1 + y1 * y1 + z1 * z1 - R2_1;
+  double descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    w.insert((T)((-bb + descr) * 0.5), 1);
+    w.insert((T)((-bb - descr) * 0.5), 1);
+  }
+  cc = x1 * x1 + y1 * y1 + z1 * z1 - R2_2;
+  descr = bb * bb - 4.0 * cc;
+  if (descr > 0.0) {
+    descr = sqrt(descr);
+    w.insert((T)((-bb + descr) * 0.5), 2);
+    w.insert((T)((-bb - descr) * 0.5), 2);
+  }
+}
+
+// thetaface.f90:59-115 (equator flagged by tan2 = -1, poles by tan2 = 0)
+template <class T>
+__device__ __forceinline__ void thetaface(Walls<T>& w, T ux, T uy, T uz, T x1, T y1, T z1, T t1, T t2) {
+  const T zero = (T)0, two = (T)2, four = (T)4, half = (T)0.5, eps = (T)1.0e-10;
+  T tmp1 = x1 * ux + y1 * uy, tmp2 = ux * ux + uy * uy, tmp3 = x1 * x1 + y1 * y1;
+  T ta, tb, aa, bb, cc, descr;
+  int i1, i2;
+  if (t2 < t1) { ta = t2; tb = t1; i1 = 4; i2 = 3; }
+  else { ta = t1; tb = t2; i1 = 3; i2 = 4; }
+  if (ta < zero) {
+    if (uz != zero) w.insert(-z1 / uz, i1);
+  } else {
+    bb = two * (tmp1 - ta * uz * z1);
+    aa = tmp2 - ta * uz * uz;
+    cc = tmp3 - z1 * z1 * ta;
+    if (RT<T>::fabs_(aa) > eps) {
+      descr = bb * bb - four * aa * cc;
+      if (descr > zero && ta != zero) {
+        descr = RT<T>::sqrt_(descr);
+        w.insert((-bb + descr) * half / aa, i1);
+        w.insert((-bb - descr) * half / aa, i1);
+      }
+    } else {
+      if (RT<T>::fabs_(bb) > eps) w.insert(-cc / bb, i1);
+    }
+  }
+  bb = two * (tmp1 - tb * uz * z1);
+  aa = tmp2 - tb * uz * uz;
+  cc = tmp3 - z1 * z1 * tb;
+  if (RT<T>::fabs_(aa) > eps) {
+    descr = bb * bb - four * aa * cc;
+    if (descr > zero && tb != zero) {
+      descr = RT<T>::sqrt_(descr);
+      w.insert((-bb + descr) * half / aa, i2);
+      w.insert((-bb - descr) * half / aa, i2);
+    }
+  } else {
+    if (RT<T>::fabs_(bb) > eps) w.insert(-cc / bb, i2);
+  }
+}
+
+// phiface.f90:48-56 with C = D = 0 (tauint_3d_2_fuv.f90:47)
+template <class T>
+__device__ __forceinline__ void phiface(Walls<T>& w, T A1, T B1, T A2, T B2, T ux, T uy, T uz, T x1, T y1, T z1) {
+  const T zero = (T)0;
+  T denom1 = (A1 * ux + B1 * uy + zero * uz);
+  if (denom1 != zero) w.insert(-(A1 * x1 + B1 * y1 + zero * z1 + zero) / denom1, 5);
+  T denom2 = (A2 * ux + B2 * uy + zero * uz);
+  if (denom2 != zero) w.insert(-(A2 * x1 + B2 * y1 + zero * z1 + zero) / denom2, 6);
+}
+
+// locate, nr_2.f:131-154
+template <class T>
+__device__ __forceinline__ int locate(const T* xx, int n, T x) {
+  int jl = 0, ju = n + 1, jm;
+  bool ascnd = xx[n - 1] > xx[0];
+  while (ju - jl > 1) {
+    jm = (ju + jl) / 2;
+    if (ascnd == (x > xx[jm - 1])) jl = jm; else ju = jm;
+  }
+  if (jl == 0 && x == xx[0]) jl = 1;
+  return jl;
+}
+
+template <class T> __device__ __forceinline__ T check1(T x) {  // check.f90
+  if (x > (T)1) return (T)1;
+  if (x < (T)-1) return (T)-1;
+  return x;
+}
+
+// ---------------------------------------------------------------------------
+// Packet state
+// ---------------------------------------------------------------------------
+template <class T> struct Opac { T kappa, albedo, g, pl; };
+
+template <class T> struct Pkt {
+  T xp, yp, zp, rsq, rtot;
+  T ux, uy, uz;
+  T cost, sint, phi, cosp, sinp;
+  T sip, sqp, sup, svp;
+  T nub, wave;
+  T cosb, sinb, lp;
+  int ir, it, ip;
+  bool on_wall;
+  int iflag, i_orig, exitflag;
+  T kapd[kMaxOpac];
+  Opac<T> op[kMaxOpac];
+};
+
+// Per-thread accumulators flushed once per packet batch.
+struct LocalCounts {
+  unsigned long long c[C_NCOUNTERS];
+  __device__ void zero() {
+#pragma unroll
+    for (int i = 0; i < C_NCOUNTERS; ++i) c[i] = 0ull;
+  }
+};
+
+template <class T>
+__device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int order) {
+  for (int id = 0; id < m.nopac; ++id) {
+    if (order == 0) p.kapd[id] = (T)(p.op[id].kappa * (T)(m.rstar * kRsol) * (T)m.kappav[id]);
+    else if (order == 1) p.kapd[id] = (T)(p.op[id].kappa * (T)m.rstar * (T)kRsol * (T)m.kappav[id]);
+    else p.kapd[id] = (T)(p.op[id].kappa * (T)kRsol * (T)m.rstar * (T)m.kappav[id]);
+  }
+}
+
+// dust_5.f90:9-109 (dust species, no gas mixing)
+template <class T>
+__device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T>* o) {
+  for (int id = 0; id < m.nopac; ++id) {
+    const double* nd = m.nudust + (size_t)id * m.nlam;
+    const double *kd = m.kappad + (size_t)id * m.nlam, *ad = m.adust + (size_t)id * m.nlam;
+    const double *gd = m.gdust + (size_t)id * m.nlam, *pd = m.pldust + (size_t)id * m.nlam;
+    int nl = m.nlam;
+    if (nu >= (T)nd[0]) { o[id].g = (T)gd[0]; o[id].pl = (T)pd[0]; o[id].kappa = (T)kd[0]; o[id].albedo = (T)ad[0]; }
+    else if (nu <= (T)nd[nl - 1]) { o[id].g = (T)gd[nl - 1]; o[id].pl = (T)pd[nl - 1]; o[id].kappa = (T)kd[nl - 1]; o[id].albedo = (T)ad[nl - 1]; }
+    else {
+      int i0 = 1, i1 = nl, i;
+      while (i1 - i0 > 1) {
+        i = (i1 + i0) / 2;
+        if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
+      }
+      T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
+      T fm1 = (T)1 - f;
+      o[id].g = fm1 * (T)gd[i1 - 1] + f * (T)gd[i0 - 1];
+      o[id].pl = fm1 * (T)pd[i1 - 1] + f * (T)pd[i0 - 1];
+      o[id].kappa = fm1 * (T)kd[i1 - 1] + f * (T)kd[i0 - 1];
+      o[id].albedo = fm1 * (T)ad[i1 - 1] + f * (T)ad[i0 - 1];
+    }
+  }
+}
+
+// random.f90:44-92 bbfreq: the zeta(4) partial sums are computed once on the
+// host with the reference's summation order and passed in `inczeta`.
+template <class T>
+__device__ __forceinline__ T bbfreq(Rng& r, T teff, const double* inczeta) {
+  T z0 = (T)inczeta[144] * uniform<T>(r);
+  T z1 = uniform<T>(r), z2 = uniform<T>(r), z3 = uniform<T>(r), z4 = uniform<T>(r);
+  int i = 1;
+  while (z0 > (T)inczeta[i - 1]) i++;
+  return -RT<T>::log_(z1 * z2 * z3 * z4) * teff / (T)i;
+}
+
+// lucydustfreq_2_test.f90:19-160 with xi supplied
+template <class T>
+__device__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop, T xi) {
+  const int nT = m.nT, nnu = m.nnu;
+  const double* lT = m.lTint;
+  const double* nui = m.nuint;
+  const double* K = m.kapint + (size_t)iop * nT * nnu;
+#define KAP(i, j) ((T)K[(size_t)((j) - 1) * nT + ((i) - 1)])
+  T lT0 = RT<T>::log_(T0);
+  int j = (int)((T)(nT - 1) * (lT0 - (T)lT[0]) / ((T)lT[nT - 1] - (T)lT[0])) + 1;
+  if (j < 1) j = 1;
+  if (j > nT - 1) j = nT - 1;
+  T f = (lT0 - (T)lT[j - 1]) / ((T)lT[j] - (T)lT[j - 1]);
+  if (f < (T)0) f = (T)0;
+  if (f > (T)1) f = (T)1;
+  int i0 = 1, i1 = nnu, i;
+  T P0 = ((T)1 - f) * KAP(j, i0) + f * KAP(j + 1, i0), P1, P = (T)0;
+  if (xi < P0) {
+    if (P0 > (T)0) return T0 * xi * (T)nui[0] / P0;
+    return T0 * (T)nui[0];
+  }
+  P1 = ((T)1 - f) * KAP(j, i1) + f * KAP(j + 1, i1);
+  if (P1 > (T)1) P1 = (T)1;
+  if (xi >= P1) {
+    if (P1 < (T)1) {
+      if (xi < (T)1) return T0 * (T)nui[nnu - 1] * ((T)1 - P1) / ((T)1 - xi);
+      return T0 * (T)nui[nnu - 1];
+    }
+    return T0 * (T)nui[nnu - 1];
+  }
+  while (i1 - i0 > 1) {
+    i = (i0 + i1) / 2;
+    P = ((T)1 - f) * KAP(j, i) + f * KAP(j + 1, i);
+    if (P > (T)1) P = (T)1;
+    if (P < P0) { P0 = P; i0 = i; }
+    if (P > P1) { P1 = P; i1 = 1; }
+    if (P > xi) { i1 = i; P1 = P; } else { i0 = i; P0 = P; }
+  }
+  if (xi < P0) f = (T)0;
+  else if (xi > P1) f = (T)1;
+  else if (P0 <= (T)0) f = (T)0.5;
+  else if (P1 <= (T)0) { }
+  else if (xi <= (T)0) { }
+  else f = RT<T>::log_(xi / P0) / RT<T>::log_(P1 / P0);
+  return T0 * (T)nui[i0 - 1] * RT<T>::pow_((T)nui[i1 - 1] / (T)nui[i0 - 1], f);
+#undef KAP
+}
+
+// dustmat.f90:42-67
+template <class T>
+__device__ __forceinline__ void dustmat(T g, T pl, T cost, T cost2, T p[4]) {
+  T g2 = g * g, g2p1 = g2 + (T)1, onemg2 = (T)1 - g2, twog = (T)2 * g;
+  T temp = g2p1 - twog * cost;
+  T p1 = (T)1.5 * onemg2 / ((T)2 + g2) * ((T)1 + cost2) / (temp * RT<T>::sqrt_(temp));
+  p[0] = p1;
+  p[1] = -pl * p1 * ((T)1 - cost2) / ((T)1 + cost2);
+  p[2] = p1 * (T)2 * cost / ((T)1 + cost2);
+  p[3] = (T)0;
+}
+
+template <class T>
+__device__ __forceinline__ T cs_peak(T g) {  // stokes.f90:54-70
+  T g2 = g * g, g2p1 = g2 + (T)1, onemg2 = (T)1 - g2, twog = (T)2 * g, temp, peak;
+  if (g > (T)0) { temp = g2p1 - twog; peak = (T)1.5 * (onemg2) / ((T)2 + g2) * (T)2 / temp / RT<T>::sqrt_(temp); }
+  else { temp = g2p1 + twog; peak = (T)1.5 * (onemg2) / ((T)2 + g2) * (T)2 / temp / RT<T>::sqrt_(temp); }
+  return peak * (T)1.01;
+}
+
+// stokes.f90:117-336 for an accepted (bmu, ri1).  dir = cost,sint,phi,cosp,sinp.
+// Returns false (no change) for bmu == 1 as the reference does.
+template <class T>
+__device__ void stokes_apply(T g, T pl, T bmu, T ri1, T dir[5], T s[4]) {
+  const T one = (T)1, two = (T)2, zero = (T)0;
+  const T PI = (T)kPi, R2P = (T)kR2p;
+  T costp = dir[0], sintp = dir[1], phip = dir[2];
+  T cost = dir[0], sint = dir[1], phi = dir[2];
+  T sip = s[0], sqp = s[1], sup = s[2], svp = s[3];
+  T p[4], cosb2 = bmu * bmu, sinbt;
+  T a11, a12, a13, a21, a22, a23, a24, a31, a32, a33, a34, a42, a43, a44, a;
+  T sini2 = zero, cosi2 = zero, bott, cosdph, sin2i2, cos2i2, sin2, cos2, sin2cos1, cos2sin1;
+  dustmat(g, pl, bmu, cosb2, p);
+  sinbt = RT<T>::sqrt_(one - cosb2);
+  if (ri1 > PI) {
+    T ri3 = R2P - ri1, cosi3 = RT<T>::cos_(ri3), sini3 = RT<T>::sin_(ri3);
+    T sin2i3 = two * sini3 * cosi3, cos2i3 = two * (cosi3 * cosi3) - one;
+    a11 = p[0]; a12 = p[1] * cos2i3; a13 = p[1] * sin2i3;
+    a = (a11 * sip + a12 * sqp + a13 * sup) / sip;
+    if (bmu == one) return;
+    else if (bmu == -one) { sini3 = one; sini2 = one; cosi3 = zero; cosi2 = zero; cost = -costp; sint = sintp; }
+    else {
+      cost = costp * bmu + sintp * sinbt * cosi3;
+      if (RT<T>::fabs_(cost) < one) {
+        sint = RT<T>::fabs_(RT<T>::sqrt_(one - cost * cost));
+        sini2 = sini3 * sintp / sint;
+        bott = sint * sinbt;
+        cosi2 = costp / bott - cost * bmu / bott;
+      } else {
+        sint = zero; sini2 = zero;
+        if (cost >= one) cosi2 = -one;
+        if (cost <= -one) cosi2 = one;
+      }
+    }
+    cosdph = -cosi2 * cosi3 + sini2 * sini3 * bmu;
+    if (RT<T>::fabs_(cosdph) > one) cosdph = (cosdph > one) ? one : -one;
+    phi = phip + RT<T>::acos_(cosdph);
+    if (phi > R2P) phi = phi - R2P;
+    if (phi < zero) phi = phi + R2P;
+    sin2i2 = two * sini2 * cosi2; cos2i2 = two * (cosi2 * cosi2) - one;
+    sin2 = sin2i2 * sin2i3; cos2 = cos2i2 * cos2i3;
+    sin2cos1 = sin2i2 * cos2i3; cos2sin1 = cos2i2 * sin2i3;
+    a21 = p[1] * cos2i2; a22 = p[0] * cos2 - p[2] * sin2; a23 = p[0] * cos2sin1 + p[2] * sin2cos1; a24 = -p[3] * sin2i2;
+    a31 = -p[1] * sin2i2; a32 = -p[0] * sin2cos1 - p[2] * cos2sin1; a33 = -p[0] * sin2 + p[2] * cos2; a34 = -p[3] * cos2i2;
+    a42 = -p[3] * sin2i3; a43 = p[3] * cos2i3; a44 = p[2];
+  } else {
+    T cosi1 = RT<T>::cos_(ri1), sini1 = RT<T>::sin_(ri1);
+    T sin2i1 = two * sini1 * cosi1, cos2i1 = two * (cosi1 * cosi1) - one;
+    a11 = p[0]; a12 = p[1] * cos2i1; a13 = -p[1] * sin2i1;
+    a = (a11 * sip + a12 * sqp + a13 * sup) / sip;
+    if (bmu == one) return;
+    else if (bmu == -one) { sini1 = one; sini2 = one; cosi1 = zero; cosi2 = zero; cost = -costp; sint = sintp; }
+    else {
+      cost = costp * bmu + sintp * sinbt * cosi1;
+      if (RT<T>::fabs_(cost) < one) {
+        sint = RT<T>::fabs_(RT<T>::sqrt_(one - cost * cost));
+        sini2 = sini1 * sintp / sint;
+        bott = sint * sinbt;
+        cosi2 = costp / bott - cost * bmu / bott;
+      } else {
+        sint = zero; sini2 = zero;
+        if (cost >= one) cosi2 = -one;
+        if (cost <= -one) cosi2 = one;
+      }
+    }
+    cosdph = -cosi1 * cosi2 + sini1 * sini2 * bmu;
+    if (RT<T>::fabs_(cosdph) > one) cosdph = (cosdph > one) ? one : -one;
+    phi = phip - RT<T>::acos_(cosdph);
+    if (phi > R2P) phi = phi - R2P;
+    if (phi < zero) phi = phi + R2P;
+    sin2i2 = two * sini2 * cosi2; cos2i2 = two * (cosi2 * cosi2) - one;
+    sin2 = sin2i2 * sin2i1; cos2 = cos2i2 * cos2i1;
+    sin2cos1 = sin2i2 * cos2i1; cos2sin1 = cos2i2 * sin2i1;
+    a21 = p[1] * cos2i2; a22 = p[0] * cos2 - p[2] * sin2; a23 = -p[0] * cos2sin1 - p[2] * sin2cos1; a24 = p[3] * sin2i2;
+    a31 = p[1] * sin2i2; a32 = p[0] * sin2cos1 + p[2] * cos2sin1; a33 = -p[0] * sin2 + p[2] * cos2; a34 = -p[3] * cos2i2;
+    a42 = p[3] * sin2i1; a43 = p[3] * cos2i1; a44 = p[2];
+  }
+  s[0] = (a11 * sip + a12 * sqp + a13 * sup) / a;
+  s[1] = (a21 * sip + a22 * sqp + a23 * sup + a24 * svp) / a;
+  s[2] = (a31 * sip + a32 * sqp + a33 * sup + a34 * svp) / a;
+  s[3] = (a42 * sqp + a43 * sup + a44 * svp) / a;
+  dir[0] = cost; dir[1] = sint; dir[2] = phi; dir[3] = RT<T>::cos_(phi); dir[4] = RT<T>::sin_(phi);
+}
+
+// stokespeel.f90:77-322: fixed (bmu, ri1), no division by the phase function.
+template <class T>
+__device__ void stokespeel(T g, T pl, T s[4], T cost, T sint, T ri1, T bmu, T costp, T sintp) {
+  const T one = (T)1, two = (T)2, zero = (T)0;
+  const T PI = (T)kPi, R2P = (T)kR2p;
+  T sip = s[0], sqp = s[1], sup = s[2], svp = s[3];
+  T p[4], cosb2 = bmu * bmu, sinbt;
+  T a11, a12, a13, a21, a22, a23, a24, a31, a32, a33, a34, a42, a43, a44;
+  T sini2 = zero, cosi2 = zero, bott, sin2i2, cos2i2, sin2, cos2, sin2cos1, cos2sin1;
+  dustmat(g, pl, bmu, cosb2, p);
+  sinbt = RT<T>::sqrt_(one - cosb2);
+  if (ri1 > PI) {
+    T ri3 = R2P - ri1, cosi3 = RT<T>::cos_(ri3), sini3 = RT<T>::sin_(ri3);
+    T sin2i3 = two * sini3 * cosi3, cos2i3 = two * (cosi3 * cosi3) - one;
+    a11 = p[0]; a12 = p[1] * cos2i3; a13 = p[1] * sin2i3;
+    if (bmu == one) return;
+    else if (bmu == -one) { sini3 = one; sini2 = one; cosi3 = zero; cosi2 = zero; }
+    else {
+      if (RT<T>::fabs_(cost) < one) {
+        sini2 = sini3 * sintp / sint;
+        bott = sint * sinbt;
+        cosi2 = costp / bott - cost * bmu / bott;
+      } else {
+        sini2 = zero;
+        if (cost >= one) cosi2 = -one;
+        if (cost <= -one) cosi2 = one;
+      }
+    }
+    sin2i2 = two * sini2 * cosi2; cos2i2 = two * (cosi2 * cosi2) - one;
+    sin2 = sin2i2 * sin2i3; cos2 = cos2i2 * cos2i3;
+    sin2cos1 = sin2i2 * cos2i3; cos2sin1 = cos2i2 * sin2i3;
+    a21 = p[1] * cos2i2; a22 = p[0] * cos2 - p[2] * sin2; a23 = p[0] * cos2sin1 + p[2] * sin2cos1; a24 = -p[3] * sin2i2;
+    a31 = -p[1] * sin2i2; a32 = -p[0] * sin2cos1 - p[2] * cos2sin1; a33 = -p[0] * sin2 + p[2] * cos2; a34 = -p[3] * cos2i2;
+    a42 = -p[3] * sin2i3; a43 = p[3] * cos2i3; a44 = p[2];
+  } else {
+    T cosi1 = RT<T>::cos_(ri1), sini1 = RT<T>::sin_(ri1);
+    T sin2i1 = two * sini1 * cosi1, cos2i1 = two * (cosi1 * cosi1) - one;
+    a11 = p[0]; a12 = p[1] * cos2i1; a13 = -p[1] * sin2i1;
+    if (bmu == one) return;
+    else if (bmu == -one) { sini1 = one; sini2 = one; cosi1 = zero; cosi2 = zero; }
+    else {
+      if (RT<T>::fabs_(cost) < one) {
+        sini2 = sini1 * sintp / sint;
+        bott = sint * sinbt;
+        cosi2 = costp / bott - cost * bmu / bott;
+      } else {
+        sini2 = zero;
+        if (cost >= one) cosi2 = -one;
+        if (cost <= -one) cosi2 = one;
+      }
+    }
+    sin2i2 = two * sini2 * cosi2; cos2i2 = two * (cosi2 * cosi2) - one;
+    sin2 = sin2i2 * sin2i1; cos2 = cos2i2 * cos2i1;
+    sin2cos1 = sin2i2 * cos2i1; cos2sin1 = cos2i2 * sin2i1;
+    a21 = p[1] * cos2i2; a22 = p[0] * cos2 - p[2] * sin2; a23 = -p[0] * cos2sin1 - p[2] * sin2cos1; a24 = p[3] * sin2i2;
+    a31 = p[1] * sin2i2; a32 = p[0] * sin2cos1 + p[2] * cos2sin1; a33 = -p[0] * sin2 + p[2] * cos2; a34 = -p[3] * cos2i2;
+    a42 = p[3] * sin2i1; a43 = p[3] * cos2i1; a44 = p[2];
+  }
+  s[0] = (a11 * sip + a12 * sqp + a13 * sup);
+  s[1] = (a21 * sip + a22 * sqp + a23 * sup + a24 * svp);
+  s[2] = (a31 * sip + a32 * sqp + a33 * sup + a34 * svp);
+  s[3] = (a42 * sqp + a43 * sup + a44 * svp);
+}
+
+// findangle3.f90:31-67 (ri1 = 0 where the reference leaves it unset)
+template <class T>
+__device__ __forceinline__ void findangle3(T cost1, T sint1, T phi1, T cosp1, T sinp1,
+                                           T cost2, T sint2, T phi2, T cosp2, T sinp2,
+                                           T& costnew, T& sintnew, T& phinew, T& ri1) {
+  const T PI = (T)kPi, R2P = (T)kR2p;
+  T cn = cost1 * cost2 + (sint1 * sint2 * (cosp1 * cosp2 + sinp1 * sinp2));
+  cn = check1(cn);
+  T sn = RT<T>::sqrt_((T)1 - cn * cn);
+  costnew = cn; sintnew = sn; ri1 = (T)0;
+  if (sint1 > (T)1.0e-8) {
+    if (sn > (T)1.0e-8) {
+      T cosx = (cost2 - (cost1 * cn)) / (sint1 * sn);
+      cosx = check1(cosx);
+      T x = RT<T>::acos_(cosx);
+      if (phi2 > phi1) {
+        T diff = phi2 - phi1;
+        if (diff > PI) { ri1 = x; phinew = PI + x; }
+        else { phinew = PI - x; ri1 = R2P - x; }
+      } else {
+        T diff = phi1 - phi2;
+        if (diff > PI) { phinew = PI - x; ri1 = R2P - x; }
+        else { phinew = PI + x; ri1 = x; }
+      }
+    } else {
+      phinew = (T)0;
+    }
+  } else {
+    phinew = phi2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// testgrid.f90:40-124.  Returns true when state changed.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& rng) {
+  const T tol = (T)1.0e-4, one = (T)1, zero = (T)0;
+  const T PI = (T)kPi, R2P = (T)kR2p;
+  const int nrg = m.nrg, ntg = m.ntg, npg = m.npg;
+  const T* r2 = walls_r2<T>(m);
+  const T* ca = walls_cos<T>(m);
+  const T* pa = walls_phi<T>(m);
+  bool reset = false, changed = false;
+  T cosb = check1(p.zp / p.rtot);
+  T phi = zero;
+  if (npg > 2) { phi = RT<T>::atan2_(p.yp, p.xp); if (phi < zero) phi = phi + R2P; }
+  if ((p.rsq - r2[0]) / (r2[1] - r2[0]) < -tol || (p.rsq - r2[nrg - 2]) / (r2[nrg - 1] - r2[nrg - 2]) > one + tol) {
+    T ra = (T)m.rarr[nrg - 1], rb = (T)m.rarr[0];
+    p.rtot = RT<T>::pow_(uniform<T>(rng) * (RT<T>::pow_(ra, (T)3) - RT<T>::pow_(rb, (T)3)) + RT<T>::pow_(rb, (T)3), (T)(1.0f / 3.0f));
+    reset = true;
+  }
+  if ((cosb - ca[0]) / (ca[1] - ca[0]) < -tol || (cosb - ca[ntg - 2]) / (ca[ntg - 1] - ca[ntg - 2]) > one + tol) {
+    cosb = (uniform<T>(rng) * (ca[ntg - 1] - ca[0]) + ca[0]);
+    reset = true;
+  }
+  if (npg > 2 && ((phi - pa[0]) / (pa[1] - pa[0]) < -tol || (phi - pa[npg - 2]) / (pa[npg - 1] - pa[npg - 2]) > one + tol)) {
+    phi = uniform<T>(rng) * (pa[npg - 1] - pa[0]) + pa[0];
+    reset = true;
+  }
+  if (reset) {
+    T s = RT<T>::sqrt_(one - RT<T>::pow_(cosb, (T)2));
+    p.xp = p.rtot * s * RT<T>::cos_(phi);
+    p.yp = p.rtot * s * RT<T>::sin_(phi);
+    p.zp = p.rtot * cosb;
+    p.rsq = p.rtot * p.rtot;
+    p.on_wall = false;
+    changed = true;
+  }
+  if (p.ir < 1 || p.ir > nrg) { p.ir = locate<T>(r2, nrg, p.rsq); changed = true; }
+  if (p.it < 1 || p.it > ntg) { p.it = locate<T>(ca, ntg, cosb); changed = true; }
+  if (p.ip < 1 || p.ip > npg) { p.ip = locate<T>(pa, npg, phi); changed = true; }
+  T rfrac = (p.rsq - r2[p.ir - 1]) / (r2[p.ir] - r2[p.ir - 1]);
+  T tfrac = (cosb - ca[p.it - 1]) / (ca[p.it] - ca[p.it - 1]);
+  bool phierr = false;
+  if (npg > 2) {
+    T pfrac = (phi - pa[p.ip - 1]) / (pa[p.ip] - pa[p.ip - 1]);
+    if (pfrac < -tol || pfrac > one + tol) {
+      T d1 = phi - pa[p.ip - 1], d2 = phi - pa[p.ip];
+      if (d1 > PI) d1 -= R2P;
+      if (d1 < -PI) d1 += R2P;
+      if (d2 > PI) d2 -= R2P;
+      if (d2 < -PI) d2 += R2P;
+      phierr = RT<T>::fabs_(d1) > tol && RT<T>::fabs_(d2) > tol;
+    }
+  }
+  if (rfrac < -tol || rfrac > one + tol) { p.ir = locate<T>(r2, nrg, p.rsq); changed = true; }
+  if (tfrac < -tol || tfrac > one + tol) { p.it = locate<T>(ca, ntg, cosb); changed = true; }
+  if (phierr) { p.ip = locate<T>(pa, npg, phi); changed = true; }
+  return changed;
+}
+
+// one face-selection step shared by the transport and peel-off walks
+template <class T>
+__device__ __forceinline__ bool next_face(const DeviceModel& m, const Pkt<T>& p, T ux, T uy, T uz, T& dt, int& iface) {
+  Walls<T> w;
+  w.reset();
+  radface<T>(w, ux, uy, uz, p.xp, p.yp, p.zp, m.r2arr[p.ir - 1], m.r2arr[p.ir]);
+  if (m.ntg > 2) {
+    const T* t2 = walls_tan2<T>(m);
+    thetaface<T>(w, ux, uy, uz, p.xp, p.yp, p.zp, t2[p.it - 1], t2[p.it]);
+  }
+  if (m.npg > 2 && m.ntg > 2) {
+    const T* A = walls_a<T>(m);
+    const T* B = walls_b<T>(m);
+    phiface<T>(w, A[p.ip - 1], B[p.ip - 1], A[p.ip], B[p.ip], ux, uy, uz, p.xp, p.yp, p.zp);
+  }
+  if (!w.next(p.on_wall, dt, iface)) return false;
+  if (dt < (T)0) return false;
+  return true;
+}
+
+__device__ __forceinline__ void cross_face(const DeviceModel& m, int& ir, int& it, int& ip, int iface) {
+  switch (iface) {
+    case 1: ir--; break;
+    case 2: ir++; break;
+    case 3: it--; break;
+    case 4: it++; break;
+    case 5: ip--; break;
+    default: ip++; break;
+  }
+  if (ip == 0) ip = m.npg - 1;
+  if (ip == m.npg) ip = 1;
+}
+
+template <class T>
+__device__ __forceinline__ T cell_opac(const DeviceModel& m, const Pkt<T>& p, size_t c, int& iop, T& rho) {
+  int sp = m.species[c];
+  rho = cell_rho<T>(m, c);
+  iop = -1;
+  if (sp > 0 && rho > (T)0) { iop = m.species_opac[sp]; return (T)0 + p.kapd[iop] * rho; }
+  return (T)0;
+}
+
+}  // namespace mcrt

@@ -1,0 +1,194 @@
+// mcrt_kernels.h -- kernels around the packet transport: deterministic
+// fixed-direction ray walks, deterministic Mueller updates, the Lucy
+// temperature inversion, and fill kernels.  Instantiated per precision by
+// kernels_fp64.hip and kernels_mixed.hip.
+#pragma once
+#include "mcrt_transport.h"
+
+namespace mcrt {
+
+struct TraceArgs {
+  int n, max_steps;
+  const double *pos, *u, *tau_stop;
+  const int* ijk;
+  const uint8_t* on_wall;
+  int *faces, *cells, *ijk_out, *nsteps, *status;
+  double *pos_out, *tau_out;
+};
+
+// Fixed-direction walk (tauint's face loop with unit kapd, no RNG tallies).
+// status: 0 stopped inside, 1 escaped, 2 hit the star, 3 max steps, -1 no wall.
+template <class T>
+__global__ void __launch_bounds__(256) k_trace(DeviceModel m, TraceArgs a) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= a.n) return;
+  Pkt<T> p;
+  p.xp = (T)a.pos[3 * r]; p.yp = (T)a.pos[3 * r + 1]; p.zp = (T)a.pos[3 * r + 2];
+  p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+  p.rtot = RT<T>::sqrt_(p.rsq);
+  p.ux = (T)a.u[3 * r]; p.uy = (T)a.u[3 * r + 1]; p.uz = (T)a.u[3 * r + 2];
+  p.ir = a.ijk[3 * r]; p.it = a.ijk[3 * r + 1]; p.ip = a.ijk[3 * r + 2];
+  p.on_wall = a.on_wall[r] != 0;
+#pragma unroll
+  for (int i = 0; i < kMaxOpac; ++i) p.kapd[i] = (T)1;
+  Rng rng;
+  rng_init(rng, 0, (uint64_t)r);
+  T tau_stop = (T)a.tau_stop[r], tsum = (T)0, dt;
+  int iface, n = 0, st = 3;
+  while (n < a.max_steps) {
+    testgrid<T>(m, p, rng);
+    if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) { st = -1; break; }
+    size_t c = cell_index(m, p.ir, p.it, p.ip);
+    int iop;
+    T rho;
+    T opac = cell_opac<T>(m, p, c, iop, rho);
+    T dtau = opac * dt;
+    if ((tsum + dtau) > tau_stop) {
+      T ds = (tau_stop - tsum) / opac;
+      p.xp = p.xp + ds * p.ux; p.yp = p.yp + ds * p.uy; p.zp = p.zp + ds * p.uz;
+      tsum = tau_stop;
+      st = 0;
+      break;
+    }
+    cross_face(m, p.ir, p.it, p.ip, iface);
+    p.xp = p.xp + dt * p.ux; p.yp = p.yp + dt * p.uy; p.zp = p.zp + dt * p.uz;
+    p.on_wall = true;
+    p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+    p.rtot = RT<T>::sqrt_(p.rsq);
+    tsum = tsum + dtau;
+    size_t o = (size_t)r * a.max_steps + n;
+    a.faces[o] = iface;
+    a.cells[3 * o] = p.ir; a.cells[3 * o + 1] = p.it; a.cells[3 * o + 2] = p.ip;
+    n++;
+    if (p.ir == m.nrg) { st = 1; break; }
+    if (p.ir == 0) { st = 2; break; }
+  }
+  a.pos_out[3 * r] = (double)p.xp; a.pos_out[3 * r + 1] = (double)p.yp; a.pos_out[3 * r + 2] = (double)p.zp;
+  a.ijk_out[3 * r] = p.ir; a.ijk_out[3 * r + 1] = p.it; a.ijk_out[3 * r + 2] = p.ip;
+  a.tau_out[r] = (double)tsum;
+  a.nsteps[r] = n;
+  a.status[r] = st;
+}
+
+// Deterministic Mueller updates.  kind 0: stokes.f90 update for accepted
+// (bmu, ri1); kind 1: stokespeel.f90.  par per item (9 doubles):
+// g, pl, bmu, ri1, cost, sint, phi, costp, sintp.  dir per item (5): cost,
+// sint, phi, cosp, sinp (kind 0 in/out).  s per item (4) in/out.
+template <class T>
+__global__ void __launch_bounds__(256) k_mueller(int n, int kind, const double* par, double* dir, double* s) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double* q = par + 9 * r;
+  T st[4] = {(T)s[4 * r], (T)s[4 * r + 1], (T)s[4 * r + 2], (T)s[4 * r + 3]};
+  if (kind == 0) {
+    T d[5] = {(T)dir[5 * r], (T)dir[5 * r + 1], (T)dir[5 * r + 2], (T)dir[5 * r + 3], (T)dir[5 * r + 4]};
+    stokes_apply<T>((T)q[0], (T)q[1], (T)q[2], (T)q[3], d, st);
+    for (int i = 0; i < 5; ++i) dir[5 * r + i] = (double)d[i];
+  } else {
+    stokespeel<T>((T)q[0], (T)q[1], st, (T)q[4], (T)q[5], (T)q[3], (T)q[2], (T)q[7], (T)q[8]);
+  }
+  for (int i = 0; i < 4; ++i) s[4 * r + i] = (double)st[i];
+}
+
+#ifdef MCRT_FP64_ONLY_KERNELS
+// ---------------------------------------------------------------------------
+// Lucy temperature (FP64): dusttemp_5.f90:8-152 with zero velocities,
+// tfinal_5_test1.f90:206-281 (mode 0) or every cell independently (mode 1).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double pow4d(double x) { double x2 = x * x; return x2 * x2; }
+
+__device__ double dusttemp_dev(const DeviceModel& m, double na, double nphot, int iop, double mcell) {
+  const double Tfloor = 0.1 / 11605.0, Tcap = kTmaxK / 11605.0;
+  const int nT = m.nT;
+  const double* kp = m.kappap + (size_t)iop * nT;
+  const double* Ti = m.Tint;
+  double Td = Tfloor;
+  if (!(mcell > 0.0)) return Tfloor;
+  double absv = pow4d(m.tstar) * (na * kPi * m.rstar * m.rstar) / (nphot * m.kappav[iop]);
+  absv = absv * (kRsol * kRsol) / pow4d(mcell);
+  if (na == 0.0) return Tfloor;
+  double minkap = pow4d(Ti[0]) * kp[0], maxkap = minkap;
+  int mini = 1, maxi = 1;
+  for (int i = 1; i <= nT; ++i) {
+    double kt = pow4d(Ti[i - 1]) * kp[i - 1];
+    if (minkap > kt) { minkap = kt; mini = i; }
+    if (maxkap < kt) { maxkap = kt; maxi = i; }
+  }
+  if (maxkap < absv) { Td = Ti[maxi - 1]; if (Td > Tcap) Td = Tcap; if (Td < Tfloor) Td = Tfloor; }
+  if (minkap > absv) { Td = Ti[mini - 1]; if (Td > Tcap) Td = Tcap; if (Td < Tfloor) Td = Tfloor; }
+  int lo = (mini < maxi) ? mini : maxi, hi = (mini < maxi) ? maxi : mini;
+  bool up = mini < maxi;
+  for (int i = lo; i <= hi - 1; ++i) {
+    double kl = pow4d(Ti[i - 1]) * kp[i - 1], ku = pow4d(Ti[i]) * kp[i];
+    bool hitv = up ? (kl < absv && ku >= absv) : (kl > absv && ku <= absv);
+    if (hitv) {
+      double Tl = Ti[i - 1], Tu = Ti[i];
+      Td = Tl + (Tu - Tl) / (ku - kl) * (absv - kl);
+      if (Td > Tcap) Td = Tcap;
+      if (Td < Tfloor) Td = Tfloor;
+      break;
+    }
+  }
+  return Td;
+}
+
+// mode 0, step 1: upper hemisphere at ip = 1, averaged with the mirror cell
+__global__ void __launch_bounds__(256) k_tfinal_half(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
+  const int nr = m.nrg - 1, nh = (m.ntg - 1) / 2;
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * nh) return;
+  int ir = idx % nr + 1, it = idx / nr + 1;
+  size_t c = cell_index(m, ir, it, 1), cm = cell_index(m, ir, m.ntg - it, 1);
+  double t = 0.1 / 11605.0;
+  int sp = m.species[c];
+  if (sp > 0 && m.rho64[c] > 0.0) {
+    int iop = m.species_opac[sp];
+    double rho2 = (m.species[cm] == sp) ? m.rho64[cm] : 0.0;
+    double mcell1 = pow(m.rho64[c] * m.vcell[c], 0.25);
+    double mcell2 = pow(rho2 * m.vcell[c], 0.25);
+    double t1 = dusttemp_dev(m, dtauabs[c], nphot, iop, mcell1);
+    double t2 = dusttemp_dev(m, dtauabs[cm], nphot, iop, mcell2);
+    t = (t1 + t2) / 2.0;
+    if (t * 11605.0 >= kTmaxK) t = kTmaxK / 11605.0;
+  }
+  tdave[c] = t;
+}
+
+// mode 0, step 2: mirror to the lower hemisphere and replicate in phi
+__global__ void __launch_bounds__(256) k_tfinal_fill(DeviceModel m, double* tdave) {
+  const int nr = m.nrg - 1, nt = m.ntg - 1, np = m.npg - 1, nh = (m.ntg - 1) / 2;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nr * nt * np) return;
+  int ir = (int)(idx % nr) + 1;
+  int it = (int)((idx / nr) % nt) + 1;
+  int ip = (int)(idx / ((size_t)nr * nt)) + 1;
+  if (ip == 1 && it <= nh) return;                 // source cells
+  int its = (it <= nh) ? it : m.ntg - it;          // tfinal_5_test1.f90:267-271
+  tdave[idx] = tdave[cell_index(m, ir, its, 1)];
+}
+
+// mode 1: every cell from its own absorbed energy (no mirror, no phi copy)
+__global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
+  size_t ncell = (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1);
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  double t = 0.1 / 11605.0;
+  int sp = m.species[c];
+  if (sp > 0 && m.rho64[c] > 0.0) {
+    int iop = m.species_opac[sp];
+    t = dusttemp_dev(m, dtauabs[c], nphot, iop, pow(m.rho64[c] * m.vcell[c], 0.25));
+    if (t * 11605.0 >= kTmaxK) t = kTmaxK / 11605.0;
+  }
+  tdave[c] = t;
+}
+
+template <class V>
+__global__ void __launch_bounds__(256) k_fill(V* p, size_t n, V v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+#endif  // MCRT_FP64_ONLY_KERNELS
+
+}  // namespace mcrt

@@ -1,0 +1,23 @@
+// mcrt_launch.h -- host-side launchers exported by the per-precision kernel
+// translation units (kernels_fp64.hip: -ffp-contract=off; kernels_mixed.hip).
+#pragma once
+#include "mcrt_device.h"
+
+namespace mcrt {
+
+struct RunParams;
+struct TraceArgs;
+
+hipError_t launch_packets_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+hipError_t launch_packets_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
+hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
+hipError_t launch_mueller_fp64(int n, int kind, const double* par, double* dir, double* st, hipStream_t s);
+hipError_t launch_mueller_mixed(int n, int kind, const double* par, double* dir, double* st, hipStream_t s);
+hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s);
+hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);
+hipError_t launch_fill_i32(int* p, size_t n, int v, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s);
+
+}  // namespace mcrt

@@ -1,0 +1,463 @@
+// mcrt_transport.h -- whole-packet transport on the device: stellar emission,
+// direct peel, propagate (tauint / scatter / absorb + re-emit), peel-off
+// imaging and SED binning.  Restates newdisktherm_5_fuv.f90:294-491,
+// propagate_5_fuv.f90, tauint_3d_2_fuv.f90, peeloff_3d_2_test.f90.
+#pragma once
+#include "mcrt_device.h"
+
+namespace mcrt {
+
+struct RunParams {
+  uint64_t first_packet;
+  uint64_t npackets;
+  uint64_t seed;
+  const double* inczeta;   // 145 partial sums of zeta(4), random.f90:74-92
+};
+
+template <class T>
+struct Ctx {
+  const DeviceModel& m;
+  const DeviceTallies& t;
+  Rng rng;
+  LocalCounts cnt;
+  const double* inczeta;
+};
+
+template <class T> __device__ __forceinline__ void atomic_add_dtau(double* a, T v) { atomicAdd(a, (double)v); }
+
+// initp_common.f90:1-31 (limb = 0)
+template <class T>
+__device__ __forceinline__ void initp_sample(Ctx<T>& C, Pkt<T>& p) {
+  T xran = uniform<T>(C.rng);
+  p.cost = RT<T>::sqrt_(xran);
+  p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
+  xran = uniform<T>(C.rng);
+  p.phi = (T)kR2p * xran;
+  p.cosp = RT<T>::cos_(p.phi);
+  p.sinp = RT<T>::sin_(p.phi);
+}
+
+// initp_common.f90:31-92
+template <class T>
+__device__ __forceinline__ void initp_transform(Pkt<T>& p) {
+  const T PI = (T)kPi, R2P = (T)kR2p, one = (T)1;
+  T costi, sinti, cospn, phinew;
+  if (RT<T>::fabs_(p.cosb) < (T)0.9999999) {
+    costi = p.cost * p.cosb - p.sint * p.sinb * p.cosp;
+    if (RT<T>::fabs_(costi) > one) { if (costi > one) costi = one; if (p.cost < -one) costi = -one; }
+    sinti = RT<T>::sqrt_(one - costi * costi);
+    if (sinti > (T)0.0000001) {
+      cospn = (p.cost - costi * p.cosb) / (sinti * p.sinb);
+      if (RT<T>::fabs_(cospn) > one) cospn = (cospn < (T)0) ? -one : one;
+      if (p.phi < PI) { phinew = RT<T>::acos_(cospn); if (phinew > PI) phinew = PI; }
+      else { phinew = R2P - RT<T>::acos_(cospn); if (phinew < PI) phinew = PI; }
+    } else phinew = (T)0;
+    p.phi = phinew; p.cost = costi; p.sint = sinti;
+  } else {
+    if (p.cosb < (T)0) p.cost = -p.cost;
+  }
+  p.phi = p.phi + p.lp;
+  if (p.phi > R2P) p.phi = p.phi - R2P;
+  if (p.phi < (T)0) p.phi = p.phi + R2P;
+  p.sinp = RT<T>::sin_(p.phi);
+  p.cosp = RT<T>::cos_(p.phi);
+}
+
+// initpabs.f90 (blackbody star: frequency from bbfreq)
+template <class T>
+__device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
+  p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+  p.cosb = p.zp / p.rtot;
+  p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
+  p.lp = RT<T>::atan2_(p.yp, p.xp);
+  initp_sample(C, p);
+  initp_transform(p);
+  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  p.nub = bbfreq<T>(C.rng, (T)C.m.tstar, C.inczeta);
+}
+
+// tauint_3d_2_fuv.f90:5-210.  Returns false on a walk error.
+template <class T>
+__device__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
+  const DeviceModel& m = C.m;
+  T tsum = (T)0, dt;
+  int iface;
+  long long nseg = 0;
+  for (;;) {
+    if (testgrid<T>(m, p, C.rng)) C.cnt.c[C_RESETS]++;
+    if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) return false;
+    if (++nseg > m.max_segments) return false;
+    C.cnt.c[C_SEGMENTS]++;
+    size_t c = cell_index(m, p.ir, p.it, p.ip);
+    int iop;
+    T rho;
+    T opac = cell_opac<T>(m, p, c, iop, rho);
+    T dtau = opac * dt;
+    if ((tsum + dtau) > tau) {
+      T ds = (tau - tsum) / opac;
+      if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)p.kapd[iop] * (double)rho * (1.0 - (double)p.op[iop].albedo));
+      p.xp = p.xp + ds * p.ux;
+      p.yp = p.yp + ds * p.uy;
+      p.zp = p.zp + ds * p.uz;
+      p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+      p.rtot = RT<T>::sqrt_(p.rsq);
+      p.on_wall = false;
+      return true;
+    }
+    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)p.kapd[iop] * (double)rho * (1.0 - (double)p.op[iop].albedo));
+    cross_face(m, p.ir, p.it, p.ip, iface);
+    p.xp = p.xp + dt * p.ux;
+    p.yp = p.yp + dt * p.uy;
+    p.zp = p.zp + dt * p.uz;
+    p.on_wall = true;
+    p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+    p.rtot = RT<T>::sqrt_(p.rsq);
+    tsum = tsum + dtau;
+    if (p.ir == m.nrg) { p.exitflag = 1; return true; }
+    if (p.ir == 0) {
+      C.cnt.c[C_AFLUX]++;
+      p.rtot = RT<T>::sqrt_(p.rsq);
+      initpabs(C, p);
+      opacset<T>(m, p.nub, p.op);
+      p.wave = (T)1.2398 / p.nub;
+      p.i_orig = 1;
+      p.iflag = 0;
+      set_kapd(p, m, 2);
+      p.ir = 1;
+      tsum = (T)0;
+    }
+  }
+}
+
+__device__ __forceinline__ int i_orig_of(int sp) {  // propagate_5_fuv.f90:123-132
+  if (sp == 1 || sp == 2 || sp == 5 || sp == 6 || sp == 9) return 2;
+  if (sp == 3 || sp == 7) return 3;
+  return 4;
+}
+
+template <class T>
+__device__ __forceinline__ int freq_bin(const DeviceModel& m, T nub) {
+  int inub = (int)((T)m.nfreq * RT<T>::log_(nub / (T)m.numin) / (T)m.lnurat) + 1;
+  if (inub < 1) inub = 1;
+  if (inub > m.nfreq) inub = m.nfreq;
+  return inub;
+}
+
+// peeloff_3d_2_test.f90:7-347.  `q` is the caller's packet (copied).
+template <class T>
+__device__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, int idust_sp) {
+  const DeviceModel& m = C.m;
+  const DeviceTallies& t = C.t;
+  Pkt<T> p = q;
+  T s[4] = {sipnew, q.sqp, q.sup, q.svp};
+  T tsum = (T)0, dt;
+  int iface;
+  const int k = peelid - 1;
+  const double* pe = m.peel + 5 * k;
+  const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
+  const T ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
+  const T* r2 = walls_r2<T>(m);
+  if (p.on_wall) {
+    bool inner = RT<T>::fabs_(p.rsq - r2[p.ir - 1]) < RT<T>::fabs_(p.rsq - r2[p.ir]);
+    bool outwards = ux1 * p.xp + uy1 * p.yp + uz1 * p.zp > (T)0;
+    if (inner && !outwards) { if (p.ir == 1) return; p.ir = p.ir - 1; }
+    if (!inner && outwards) p.ir = p.ir + 1;
+  }
+  long long nseg = 0;
+  for (;;) {
+    if (testgrid<T>(m, p, C.rng)) C.cnt.c[C_RESETS]++;
+    if (!next_face<T>(m, p, ux1, uy1, uz1, dt, iface) || ++nseg > m.max_segments) { C.cnt.c[C_ERRORS]++; return; }
+    C.cnt.c[C_PEELSEG]++;
+    size_t c = cell_index(m, p.ir, p.it, p.ip);
+    int iop;
+    T rho;
+    T opac = cell_opac<T>(m, p, c, iop, rho);
+    T dtau = opac * dt;
+    tsum = tsum + dtau;
+    if (tsum > (T)m.taumax) return;
+    cross_face(m, p.ir, p.it, p.ip, iface);
+    p.xp = p.xp + dt * ux1;
+    p.yp = p.yp + dt * uy1;
+    p.zp = p.zp + dt * uz1;
+    p.on_wall = true;
+    p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+    p.rtot = RT<T>::sqrt_(p.rsq);
+    if (p.ir == m.nrg) break;
+    if (p.ir == 0) return;
+  }
+  C.cnt.c[C_PEELEVT]++;
+  T factor = RT<T>::exp_(-tsum);
+  s[0] = s[0] * factor; s[1] = s[1] * factor; s[2] = s[2] * factor; s[3] = s[3] * factor;
+  if (q.iflag == 1) {
+    T bmu, sinbm, phinew, ri1;
+    findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
+    int iop = m.species_opac[idust_sp];
+    stokespeel<T>(p.op[iop].g, p.op[iop].pl, s, coste, sinte, ri1, bmu, q.cost, q.sint);
+  }
+  T yimage = p.zp * sinte - p.yp * coste * sinpe - p.xp * coste * cospe;
+  T ximage = p.yp * cospe - p.xp * sinpe;
+  T rmaxi = (T)m.rmaxi;
+  T fractxh = (T)2 * rmaxi / (T)m.nx;
+  int ix = (int)((ximage + rmaxi) / fractxh) + 1;
+  int iy = (int)((yimage + rmaxi) / fractxh) + 1;
+  T freq = (T)2.9979e14 / q.wave;
+  if (m.nband > 0 && ix <= m.nx && iy <= m.nx && ix > 0 && iy > 0) {
+    size_t npix = (size_t)m.npeel * m.nx * m.nx;
+    size_t plane = (size_t)m.nband * npix;
+    for (int b = 0; b < m.nband; ++b) {
+      const double* fw = m.band_freq + (size_t)b * m.nwmax;
+      const double* fp = m.band_phi + (size_t)b * m.nwmax;
+      int nf = m.band_nw[b] - 1;
+      if (freq > (T)fw[0] && freq < (T)fw[nf - 1]) {
+        int iw = 1;
+        while (freq > (T)fw[iw - 1]) iw++;
+        iw = iw - 1;
+        T wt = (T)fp[iw - 1];
+        size_t o = (size_t)b * npix + (size_t)k + (size_t)m.npeel * ((size_t)(ix - 1) + (size_t)m.nx * (size_t)(iy - 1));
+        atomicAdd(&t.image[0 * plane + o], (float)(s[0] * wt));
+        atomicAdd(&t.image[1 * plane + o], (float)(s[1] * wt));
+        atomicAdd(&t.image[2 * plane + o], (float)(s[2] * wt));
+        atomicAdd(&t.image[3 * plane + o], (float)(s[3] * wt));
+        int pl = (q.i_orig == 1) ? 5 : (q.i_orig == 2) ? 4 : (q.i_orig == 3) ? 6 : (q.i_orig == 4) ? 7 : -1;
+        if (pl >= 0) atomicAdd(&t.image[(size_t)pl * plane + o], (float)(s[0] * wt));
+      }
+    }
+  }
+  T rho2 = ximage * ximage + yimage * yimage;
+  int inub = freq_bin<T>(m, q.nub);
+  int i_inter = (q.iflag == 1) ? 2 : ((q.i_orig == 1) ? 1 : 3);
+  int indx[3] = {1, 1 + q.i_orig, 5 + i_inter};
+  size_t nb = (size_t)m.nfreq * m.npeel * m.nap * kNo;
+  for (int ia = 1; ia <= m.nap; ++ia) {
+    if (rho2 < (T)m.aperture2[ia - 1]) {
+      for (int j = 0; j < 3; ++j) {
+        size_t o = (size_t)(inub - 1) + (size_t)m.nfreq * ((size_t)k + (size_t)m.npeel * ((size_t)(ia - 1) + (size_t)m.nap * (size_t)(indx[j] - 1)));
+        atomicAdd(&t.psed[0 * nb + o], (float)s[0]);
+        atomicAdd(&t.psed[1 * nb + o], (float)s[1]);
+        atomicAdd(&t.psed[2 * nb + o], (float)s[2]);
+        atomicAdd(&t.psed[3 * nb + o], (float)s[3]);
+        atomicAdd(&t.psed[4 * nb + o], (float)(s[0] * s[0]));
+        atomicAdd(&t.psed[5 * nb + o], (float)(s[1] * s[1]));
+        atomicAdd(&t.psed[6 * nb + o], (float)(s[2] * s[2]));
+        atomicAdd(&t.psed[7 * nb + o], (float)(s[3] * s[3]));
+        atomicAdd(&t.psed[8 * nb + o], 1.0f);
+      }
+    }
+  }
+}
+
+// stokes.f90:46-137 rejection sampling, then the Mueller update
+template <class T>
+__device__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
+  const T PI = (T)kPi, R2P = (T)kR2p, one = (T)1, two = (T)2;
+  T g = p.op[iop].g, pl = p.op[iop].pl;
+  T peak = cs_peak<T>(g), bmu, ri1, rprob, pp[4];
+  for (;;) {
+    C.cnt.c[C_HTOT]++;
+    bmu = one - two * uniform<T>(C.rng);
+    T cosb2 = bmu * bmu;
+    dustmat<T>(g, pl, bmu, cosb2, pp);
+    ri1 = R2P * uniform<T>(C.rng);
+    if (ri1 > PI) {
+      T ri3 = R2P - ri1, cosi3 = RT<T>::cos_(ri3), sini3 = RT<T>::sin_(ri3);
+      T sin2i3 = two * sini3 * cosi3, cos2i3 = two * (cosi3 * cosi3) - one;
+      rprob = (pp[0] * p.sip + pp[1] * cos2i3 * p.sqp + pp[1] * sin2i3 * p.sup) / p.sip;
+    } else {
+      T cosi1 = RT<T>::cos_(ri1), sini1 = RT<T>::sin_(ri1);
+      T sin2i1 = two * sini1 * cosi1, cos2i1 = two * (cosi1 * cosi1) - one;
+      rprob = (pp[0] * p.sip + pp[1] * cos2i1 * p.sqp + (-pp[1] * sin2i1) * p.sup) / p.sip;
+    }
+    if (rprob > peak) peak = rprob;
+    T xran = uniform<T>(C.rng);
+    if (peak * xran > rprob) continue;
+    C.cnt.c[C_HIT]++;
+    break;
+  }
+  T dir[5] = {p.cost, p.sint, p.phi, p.cosp, p.sinp};
+  T s[4] = {p.sip, p.sqp, p.sup, p.svp};
+  stokes_apply<T>(g, pl, bmu, ri1, dir, s);
+  p.cost = dir[0]; p.sint = dir[1]; p.phi = dir[2]; p.cosp = dir[3]; p.sinp = dir[4];
+  p.sip = s[0]; p.sqp = s[1]; p.sup = s[2]; p.svp = s[3];
+}
+
+// propagate_5_fuv.f90:1-326.  Returns 1 escaped, 2 killed, -1 error.
+template <class T>
+__device__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
+  const DeviceModel& m = C.m;
+  const DeviceTallies& t = C.t;
+  const T R2P = (T)kR2p;
+  T xpold = p.xp, ypold = p.yp, zpold = p.zp;
+  int iscat = 0, iint = 0, idust2;
+  nint = 0;
+  p.iflag = 0; p.exitflag = 0;
+  T tau = -RT<T>::log_(uniform<T>(C.rng));
+  if (!tauint(C, p, tau)) return -1;
+  if (!p.exitflag) {
+    (void)uniform<T>(C.rng);   // select_dust_2.f90:134 draw
+    idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
+    C.cnt.c[C_TOT]++;
+    for (;;) {
+      size_t c = cell_index(m, p.ir, p.it, p.ip);
+      int iop = m.species_opac[idust2];
+      nint++;
+      if (uniform<T>(C.rng) <= p.op[iop].albedo) {
+        p.iflag = 1; iscat++; iint++;
+        if (m.ipeel == 1)
+          for (int pid = 1; pid <= m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, idust2);
+        stokes<T>(C, p, iop);
+      } else {
+        atomicAdd(&t.dbig[c], 1);
+        p.i_orig = i_orig_of(idust2);
+        atomicAdd(&t.nabs[c], 1);
+        T tave = (T)m.tdust[c];
+        p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+        p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
+        p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
+        p.phi = R2P * uniform<T>(C.rng);
+        p.cosp = RT<T>::cos_(p.phi);
+        p.sinp = RT<T>::sin_(p.phi);
+        if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
+        else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
+        p.uy = p.sint * p.sinp; p.ux = p.sint * p.cosp; p.uz = p.cost;
+        p.iflag = 0;
+        opacset<T>(m, p.nub, p.op);
+        p.wave = (T)1.2398 / p.nub;
+        set_kapd(p, m, 1);
+        if (m.ipeel == 1)
+          for (int pid = 1; pid <= m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, idust2);
+        iint++;
+      }
+      tau = -RT<T>::log_(uniform<T>(C.rng));
+      xpold = p.xp; ypold = p.yp; zpold = p.zp;
+      p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+      int pir = p.ir, pit = p.it, pip = p.ip;
+      if (!tauint(C, p, tau)) return -1;
+      if (p.exitflag) break;
+      if (p.ir != pir || p.it != pit || p.ip != pip) iint = 0;
+      if (iint > m.iintmax) {
+        atomicAdd(&t.killed[cell_index(m, p.ir, p.it, p.ip)], 1);
+        C.cnt.c[C_ABFLUX]++;
+        return 2;
+      }
+      (void)uniform<T>(C.rng);
+      idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
+    }
+  }
+  // SED binning, propagate_5_fuv.f90:231-320
+  int itb = (int)((T)m.nmu * ((T)1 - p.cost) / (T)2) + 1;
+  if (p.phi < (T)0) p.phi = p.phi + R2P;
+  int ipb = (int)((T)m.nph * p.phi / R2P) + 1;
+  if (itb < 1) itb = 1;
+  if (itb > m.nmu) itb = m.nmu;
+  if (ipb < 1) ipb = 1;
+  if (ipb > m.nph) ipb = m.nph;
+  T x = ypold * p.cosp - xpold * p.sinp;
+  T y = zpold * (T)m.sini_mu[itb - 1] - ypold * (T)m.u_mu[itb - 1] * p.sinp - xpold * (T)m.u_mu[itb - 1] * p.cosp;
+  T rho2 = (x * x + y * y);
+  if (rho2 < (T)m.aperture2[m.nap - 1] * (T)1.0001) {
+    C.cnt.c[C_FLUX]++;
+    if (p.iflag == 1) { C.cnt.c[C_SFLUX]++; C.cnt.c[C_NSCAT] += (unsigned long long)iscat; }
+  }
+  int inub = freq_bin<T>(m, p.nub);
+  int i_inter = (p.iflag == 1) ? 2 : ((p.i_orig == 1) ? 1 : 3);
+  int indx[3] = {1, 1 + p.i_orig, 5 + i_inter};
+  size_t nb = (size_t)m.nfreq * m.nmu * m.nph * m.nap * kNo;
+  for (int ia = 1; ia <= m.nap; ++ia) {
+    if (rho2 < (T)m.aperture2[ia - 1]) {
+      for (int j = 0; j < 3; ++j) {
+        size_t o = (size_t)(inub - 1) + (size_t)m.nfreq * ((size_t)(itb - 1) + (size_t)m.nmu * ((size_t)(ipb - 1) + (size_t)m.nph * ((size_t)(ia - 1) + (size_t)m.nap * (size_t)(indx[j] - 1))));
+        atomicAdd(&t.sed[0 * nb + o], (float)p.sip);
+        atomicAdd(&t.sed[1 * nb + o], (float)p.sqp);
+        atomicAdd(&t.sed[2 * nb + o], (float)p.sup);
+        atomicAdd(&t.sed[3 * nb + o], (float)p.svp);
+        atomicAdd(&t.sed[4 * nb + o], (float)(p.sip * p.sip));
+        atomicAdd(&t.sed[5 * nb + o], (float)(p.sqp * p.sqp));
+        atomicAdd(&t.sed[6 * nb + o], (float)(p.sup * p.sup));
+        atomicAdd(&t.sed[7 * nb + o], (float)(p.svp * p.svp));
+        atomicAdd(&t.sed[8 * nb + o], 1.0f);
+        atomicAdd(&t.sed[9 * nb + o], (float)RT<T>::fabs_(p.cost));
+      }
+    }
+  }
+  if (p.iflag == 1) C.cnt.c[C_SCOUNT]++;
+  return 1;
+}
+
+// newdisktherm_5_fuv.f90:294-413 one stellar photon (ips = 0, no spots)
+template <class T>
+__device__ void one_packet(Ctx<T>& C, uint64_t pid) {
+  const DeviceModel& m = C.m;
+  Pkt<T> p;
+  p.on_wall = false;
+  p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+  for (;;) {
+    p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
+    p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
+    if (p.cosb != (T)0 && RT<T>::fabs_(p.cosb) < (T)1) break;
+  }
+  p.lp = (T)kR2p * uniform<T>(C.rng);
+  initp_sample(C, p);
+  initp_transform(p);
+  p.nub = bbfreq<T>(C.rng, (T)m.tstar, C.inczeta);
+  p.i_orig = 1;
+  p.iflag = 0;
+  p.exitflag = 0;
+  opacset<T>(m, p.nub, p.op);
+  p.wave = (T)1.2398 / p.nub;
+  set_kapd(p, m, 0);
+  T coslp = RT<T>::cos_(p.lp), sinlp = RT<T>::sin_(p.lp);
+  p.zp = p.cosb * (T)m.rmin;
+  T rp = p.sinb * (T)m.rmin;
+  p.xp = rp * coslp;
+  p.yp = rp * sinlp;
+  p.rsq = p.zp * p.zp + rp * rp;
+  p.rtot = RT<T>::sqrt_(p.rsq);
+  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  p.ir = 1;
+  p.on_wall = true;
+  p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
+  p.ip = locate<T>(walls_phi<T>(m), m.npg, p.lp);
+  if (m.ipeel == 1) {
+    for (int pk = 1; pk <= m.npeel; ++pk) {
+      const double* pe = m.peel + 5 * (pk - 1);
+      T cosnorm = p.cosb * (T)pe[0] + (p.sinb * (T)pe[1] * ((T)pe[2] * coslp + (T)pe[3] * sinlp));
+      T sipnew = (T)4 * p.sip * cosnorm;
+      if (cosnorm > (T)0) peeloff<T>(C, p, sipnew, pk, 0);
+    }
+  }
+  int nint = 0;
+  int code = propagate<T>(C, p, nint);
+  if (code == 1) C.cnt.c[C_ESCAPED]++;
+  else if (code == 2) C.cnt.c[C_KILLED]++;
+  else C.cnt.c[C_ERRORS]++;
+  if (C.t.trace) {
+    double* tr = C.t.trace + 8 * pid;
+    tr[0] = (double)p.nub; tr[1] = (double)p.cost; tr[2] = (double)p.phi; tr[3] = (double)p.sip;
+    tr[4] = (double)p.sqp; tr[5] = (double)p.sup; tr[6] = (double)nint; tr[7] = (double)code;
+  }
+}
+
+__device__ __forceinline__ void flush_counts(const DeviceTallies& t, LocalCounts& cnt) {
+  // wavefront reduction then one atomic per counter per wavefront
+#pragma unroll
+  for (int i = 0; i < C_NCOUNTERS; ++i) {
+    unsigned long long v = cnt.c[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&t.counters[i], v);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_packets(DeviceModel m, DeviceTallies t, RunParams rp) {
+  Ctx<T> C{m, t, {}, {}, rp.inczeta};
+  C.cnt.zero();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.npackets; i += stride) {
+    rng_init(C.rng, rp.seed, rp.first_packet + i);
+    one_packet<T>(C, i);
+  }
+  flush_counts(t, C.cnt);
+}
+
+}  // namespace mcrt

@@ -1,0 +1,429 @@
+"""Synthetic young-stellar-object models for the packet engine.
+
+Builds, on the host, exactly the module state the reference's packet loop
+reads (grid walls and face tables, per-cell density and species, dust and
+Planck-emission tables, star, observer set-up), following the reference's
+conventions (paths relative to src3_mpi_cfitsio/):
+
+* walls: 1-based rarr(1..nrg) with rarr(1) = stellar surface, r2arr = rarr**2
+  (gridset_5_model13new.f90:516); theta walls with the equator flagged by
+  tan2arr = -1 and the poles by tan2arr = 0, ntg odd (:706-731); phi walls
+  as half-planes aarr = sin(phi), barr = -cos(phi) (:766-772); cell volumes
+  -(r2^3-r1^3)(cos t2-cos t1)(p2-p1)/3 (R* rsol)^3 (:1108-1152);
+* dust tables as dustinit builds them (dust_5.f90:384-455): nudust = 12398/A
+  in eV (descending), kappa normalized by its 5500 A value kappav;
+* emission tables as setemiss builds them (dust_5.f90:507-699): nT = 500
+  log-spaced temperatures 0.1 K..2e5 K (in K/11605), nnu = 513 linear
+  x = h nu/kT in [1/32, 32], kapint = normalized cumulative integral of
+  kappa_abs(nu) B_x(x) with kappa linear across each x interval
+  (kappaB, dust_5.f90:784-796), kappap = the unnormalized total;
+* SED bins (setup.f90:36-41, initarr_2_new.f90:21-25) and peel-off
+  directions (tts_mod_5.f90, gridset check :695-701).
+
+The densities themselves (flared disk, Ulrich infall envelope, log-normal
+fractal clumping) are this project's synthetic workloads: the reference
+builds its models from hydrodynamic tables that are not part of the path.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Dict, Optional
+
+import numpy as np
+
+RSUN = 6.9598e10          # constants.f90
+MSUN = 1.989e33
+AU = 1.49598e13
+G = 6.67428e-8
+SIGT = 5.670400e-5
+LSUN = 3.839e33
+YEAR = 3.15576e7
+TK = 11605.0              # K per eV (the reference's temperature unit)
+
+
+@dataclasses.dataclass
+class Model:
+    name: str
+    # grid
+    nrg: int
+    ntg: int
+    npg: int
+    rarr: np.ndarray
+    r2arr: np.ndarray
+    thetarr: np.ndarray
+    costarr: np.ndarray
+    tan2arr: np.ndarray
+    phiarr: np.ndarray
+    aarr: np.ndarray
+    barr: np.ndarray
+    vcell: np.ndarray            # (np-1, nt-1, nr-1) C order == reference cell order
+    # medium
+    rho: np.ndarray              # same shape, g/cm^3
+    species: np.ndarray          # uint8, 0 = empty, 1 disk, 3 envelope
+    species_opac: np.ndarray     # int32[11]
+    # dust
+    nopac: int
+    nlam: int
+    nudust: np.ndarray           # (nopac, nlam) eV, descending
+    kappad: np.ndarray
+    adust: np.ndarray
+    gdust: np.ndarray
+    pldust: np.ndarray
+    kappav: np.ndarray           # (nopac,) cm^2/g
+    nT: int
+    nnu: int
+    lTint: np.ndarray
+    Tint: np.ndarray
+    nuint: np.ndarray
+    kapint: np.ndarray           # (nopac, nnu, nT)
+    kappap: np.ndarray           # (nopac, nT)
+    # source
+    rstar: float                 # R_sun
+    tstar: float                 # K/11605
+    rmin: float
+    # observer
+    ipeel: int
+    npeel: int
+    thete: np.ndarray
+    phie: np.ndarray
+    nx: int
+    rmaxi: float
+    nband: int
+    nwmax: int
+    band_nw: np.ndarray
+    band_freq: np.ndarray        # (nband, nwmax) Hz
+    band_phi: np.ndarray
+    nfreq: int
+    nmu: int
+    nph: int
+    nap: int
+    aperture2: np.ndarray
+    u_mu: np.ndarray
+    numin: float
+    lnurat: float
+    # run set-up
+    npackets: int = 0
+    niter: int = 1
+    tdust0: Optional[np.ndarray] = None
+
+    @property
+    def ncell(self) -> int:
+        return (self.nrg - 1) * (self.ntg - 1) * (self.npg - 1)
+
+    @property
+    def cell_shape(self):
+        return (self.npg - 1, self.ntg - 1, self.nrg - 1)
+
+    @property
+    def sini_mu(self) -> np.ndarray:
+        return np.sqrt(1.0 - self.u_mu ** 2)
+
+    @property
+    def lstar(self) -> float:
+        """4 pi R^2 sigma T^4 (newdisktherm_5_fuv.f90:195), erg/s."""
+        return 4.0 * math.pi * SIGT * (self.tstar * TK) ** 4 * (self.rstar * RSUN) ** 2
+
+    def tdust_initial(self) -> np.ndarray:
+        if self.tdust0 is not None:
+            return self.tdust0
+        return np.full(self.cell_shape, 0.1 / TK)
+
+    def with_packets(self, n: int) -> "Model":
+        return dataclasses.replace(self, npackets=int(n))
+
+
+# ---------------------------------------------------------------------------
+# walls
+# ---------------------------------------------------------------------------
+def radial_walls(ncell_r: int, rin_au: float, rout_au: float, rstar_rsun: float) -> np.ndarray:
+    """Wall 1 = stellar surface, wall 2 = Rin, log-spaced walls to Rout (R*)."""
+    autors = AU / (rstar_rsun * RSUN)
+    r = np.empty(ncell_r + 1)
+    r[0] = 1.0
+    r[1:] = np.logspace(math.log10(rin_au * autors), math.log10(rout_au * autors), ncell_r)
+    return r
+
+
+def theta_walls(ncell_t: int, midplane_refine: float = 0.9) -> np.ndarray:
+    """ncell_t even; walls 0..pi with an equator wall, finer toward the midplane."""
+    if ncell_t % 2:
+        raise ValueError("the theta cell count must be even (ntg odd, equator wall)")
+    nh = ncell_t // 2
+    s = np.linspace(0.0, 1.0, nh + 1)
+    w = midplane_refine
+    upper = 0.5 * math.pi * ((1.0 - w) * s + w * (1.0 - (1.0 - s) ** 3))
+    th = np.concatenate([upper, math.pi - upper[::-1][1:]])
+    th[0], th[nh], th[-1] = 0.0, 0.5 * math.pi, math.pi
+    return th
+
+
+def theta_tables(thetarr: np.ndarray):
+    """costarr / tan2arr with the reference's pole and equator flags."""
+    ntg = len(thetarr)
+    nttmp = (ntg + 1) // 2
+    cost = np.cos(thetarr)
+    tan2 = np.tan(thetarr) ** 2
+    cost[0], tan2[0] = 1.0, 0.0
+    cost[-1], tan2[-1] = -1.0, 0.0
+    if ntg > 2:
+        cost[nttmp - 1], tan2[nttmp - 1] = 0.0, -1.0
+    return cost, tan2
+
+
+def phi_walls(ncell_p: int):
+    dp = 2.0 * math.pi / ncell_p
+    phi = dp * np.arange(ncell_p + 1, dtype=np.float64)
+    return phi, np.sin(phi), -np.cos(phi)
+
+
+def cell_volumes(rarr, thetarr, phiarr, rstar_rsun):
+    dr3 = rarr[1:] ** 3 - rarr[:-1] ** 3
+    dcost = np.cos(thetarr[1:]) - np.cos(thetarr[:-1])
+    dp = phiarr[1:] - phiarr[:-1]
+    rmincgs = rstar_rsun * RSUN
+    return (-(dr3[None, None, :] * dcost[None, :, None] * dp[:, None, None]) / 3.0 * rmincgs ** 3)
+
+
+# ---------------------------------------------------------------------------
+# densities
+# ---------------------------------------------------------------------------
+def ulrich_mu0(mu: np.ndarray, rx: np.ndarray) -> np.ndarray:
+    """Streamline origin: root in [0,1] of mu0^3 + mu0 (rx-1) - mu rx = 0
+    (the cubic zerod.f90 solves), by safeguarded Newton iterations."""
+    mu = np.abs(mu)
+    m0 = np.where(rx < 1.0, np.maximum(mu, 0.5), mu)
+    lo = np.zeros_like(mu)
+    hi = np.ones_like(mu)
+    for _ in range(60):
+        f = m0 ** 3 + m0 * (rx - 1.0) - mu * rx
+        lo = np.where(f < 0, m0, lo)
+        hi = np.where(f >= 0, m0, hi)
+        df = 3 * m0 ** 2 + rx - 1.0
+        step = np.where(np.abs(df) > 1e-30, f / np.where(df == 0, 1.0, df), 0.0)
+        nm = m0 - step
+        bad = (nm <= lo) | (nm >= hi) | ~np.isfinite(nm)
+        m0 = np.where(bad, 0.5 * (lo + hi), nm)
+    return np.clip(m0, 1e-12, 1.0)
+
+
+def fractal_field(shape, sigma_ln: float, dimension: float, seed: int) -> np.ndarray:
+    """Log-normal multiplicative clumping exp(sigma g - sigma^2/2), where g is
+    a unit-variance Gaussian random field with power spectrum k^-(2(3-D)+3)
+    (fractional Brownian field of isosurface dimension D) on the cell index grid."""
+    rng = np.random.default_rng(seed)
+    beta = 2.0 * (3.0 - dimension) + 3.0
+    kz = np.fft.fftfreq(shape[0])[:, None, None]
+    ky = np.fft.fftfreq(shape[1])[None, :, None]
+    kx = np.fft.rfftfreq(shape[2])[None, None, :]
+    k = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    k[0, 0, 0] = 1.0
+    amp = k ** (-beta / 2.0)
+    amp[0, 0, 0] = 0.0
+    noise = rng.standard_normal(shape)
+    g = np.fft.irfftn(np.fft.rfftn(noise) * amp, s=shape, axes=(0, 1, 2))
+    g = (g - g.mean()) / g.std()
+    return np.exp(sigma_ln * g - 0.5 * sigma_ln ** 2)
+
+
+def yso_density(rarr, thetarr, phiarr, vcell, rstar_rsun, mdisk_msun, rin_au, rout_au,
+                h0_au=0.03, flare=1.25, alpha=2.25, mdot_msun_yr=1e-6, rc_au=100.0,
+                mstar_msun=0.5, clump_sigma=0.0, clump_dim=2.6, clump_seed=0):
+    """Flared disk + Ulrich envelope at cell centres; one species per cell
+    (1 = disk where the disk dominates, 3 = envelope), as the reference
+    zeroes envelope species inside the disk (gridset_5_model13new.f90:1233-1239)."""
+    autors = AU / (rstar_rsun * RSUN)
+    rc = 0.5 * (rarr[1:] + rarr[:-1])[None, None, :]
+    tc = 0.5 * (thetarr[1:] + thetarr[:-1])[None, :, None]
+    r_au = rc / autors
+    varpi = r_au * np.sin(tc)
+    z = r_au * np.cos(tc)
+    h = h0_au * np.maximum(varpi, 1e-30) ** flare
+    inside = (r_au >= rin_au) & (varpi >= rin_au) & (r_au <= rout_au)
+    rho_d = np.where(inside, np.maximum(varpi, 1e-30) ** (-alpha) * np.exp(-0.5 * (z / h) ** 2), 0.0)
+    rho_d = np.broadcast_to(rho_d, vcell.shape).copy()
+    mass = float(np.sum(rho_d * vcell))
+    if mass > 0:
+        rho_d *= mdisk_msun * MSUN / mass
+    # Ulrich (1976) infall: rho = rho0 (r/Rc)^-3/2 (1+mu/mu0)^-1/2 (mu/mu0 + 2 mu0^2 Rc/r)^-1
+    rx = r_au / rc_au
+    mu = np.abs(np.cos(tc)) * np.ones_like(rx)
+    rxb = rx * np.ones_like(mu)
+    mu0 = ulrich_mu0(mu, rxb)
+    rho0 = (mdot_msun_yr * MSUN / YEAR) / (4.0 * math.pi * math.sqrt(G * mstar_msun * MSUN * (rc_au * AU) ** 3))
+    rho_e = rho0 * rxb ** -1.5 / np.sqrt(1.0 + mu / mu0) / (mu / mu0 + 2.0 * mu0 ** 2 / rxb)
+    rho_e = np.where(r_au >= rin_au, rho_e, 0.0)
+    rho_e = np.broadcast_to(rho_e, vcell.shape).copy()
+    if clump_sigma > 0:
+        f = fractal_field(vcell.shape, clump_sigma, clump_dim, clump_seed)
+        rho_d *= f
+        rho_e *= f
+    species = np.where(rho_d + rho_e > 0, np.where(rho_d >= rho_e, 1, 3), 0).astype(np.uint8)
+    return rho_d + rho_e, species
+
+
+# ---------------------------------------------------------------------------
+# dust and emission tables
+# ---------------------------------------------------------------------------
+def synthetic_dust(nlam=100, lam_min=0.1, lam_max=1000.0, kappa0=200.0, slope=1.5,
+                   albedo=(0.5, 0.01), g=(0.6, 0.0), pl=0.0):
+    """kappa = kappa0 (lambda/um)^-slope; albedo and g linear in log lambda."""
+    lam = np.logspace(math.log10(lam_min), math.log10(lam_max), nlam)     # um, ascending
+    s = np.linspace(0.0, 1.0, nlam)
+    kap = kappa0 * lam ** (-slope)
+    alb = albedo[0] + (albedo[1] - albedo[0]) * s
+    gg = g[0] + (g[1] - g[0]) * s
+    pp = np.full(nlam, pl)
+    lamA = 10000.0 * lam
+    nu = 12398.0 / lamA                                                    # eV, descending
+    iv = int(np.nonzero(lam <= 0.55)[0][-1])                               # dust_5.f90:426,459-460
+    f = (5500.0 - lamA[iv]) / (lamA[iv + 1] - lamA[iv])
+    kappav1 = (1.0 - f) * kap[iv] + f * kap[iv + 1]
+    return dict(nudust=nu, kappad=kap / kappav1, adust=alb, gdust=gg, pldust=pp, kappav=kappav1)
+
+
+_AS_EXPM1 = (0.9999999995, 0.4999999206, 0.1666653019, 0.0416573475,
+             0.0083013598, 0.0013298820, 0.0001413161)
+
+
+def expm1_as(x):
+    """exp(x)-1 as mathsubs.f90:79-113 computes it: the Abramowitz & Stegun
+    4.2.45 polynomial for |x| <= 0.6931472, exp(x)-1 beyond."""
+    x = np.asarray(x, dtype=np.float64)
+    s = np.zeros_like(x)
+    for c in _AS_EXPM1[::-1]:
+        s = x * (c + s)
+    return np.where(np.abs(x) > 0.6931472, np.exp(x) - 1.0, s)
+
+
+def planck_bx(x):
+    """Bnu of mathsubs.f90:4-22: (15/pi^4) x^3 / expm1(x), 0 at x = 0."""
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(x == 0.0, 0.0, 0.1539897 * x * x * x / expm1_as(x))
+
+
+def _opacset_vec(nu, nudust, tab):
+    """dust_5.f90:21-55 linear interpolation in nu, clamped at the table ends."""
+    return np.interp(nu, nudust[::-1], tab[::-1])
+
+
+def emission_tables(dust, nT=500, nnu=513, Tmin=0.1, Tmax=200000.0, nodes=8):
+    """setemiss (dust_5.f90:507-699) by Gauss-Legendre quadrature per x interval."""
+    lTint = math.log(Tmin / TK) + np.arange(nT) * math.log(Tmax / Tmin) / (nT - 1)
+    Tint = np.exp(lTint)
+    numax, numin = 32.0, 32.0 / 1024.0
+    nuint = numin + np.arange(nnu) * (numax - numin) / (nnu - 1)
+    xg, wg = np.polynomial.legendre.leggauss(nodes)
+    fg = 0.5 * (xg + 1.0)
+    wg = 0.5 * wg
+    x0 = np.concatenate([[0.0], nuint[:-1]])
+    x1 = nuint
+    dx = x1 - x0
+    kabs = lambda nu: ((1.0 - _opacset_vec(nu, dust["nudust"], dust["adust"]))
+                       * _opacset_vec(nu, dust["nudust"], dust["kappad"]))
+    kap0 = kabs(x0[None, :] * Tint[:, None])                               # (nT, nnu)
+    kap1 = kabs(x1[None, :] * Tint[:, None])
+    xs = x0[:, None] + fg[None, :] * dx[:, None]                           # (nnu, nodes)
+    B = planck_bx(xs)                                                      # mathsubs.f90:4-22
+    kf = (1.0 - fg)[None, None, :] * kap0[:, :, None] + fg[None, None, :] * kap1[:, :, None]
+    seg = dx[None, :] * np.sum(kf * B[None, :, :] * wg[None, None, :], axis=2)
+    cum = np.cumsum(seg, axis=1)
+    kappap = cum[:, -1].copy()
+    kapint = cum / kappap[:, None]
+    kapint[:, -1] = 1.0
+    return dict(nT=nT, nnu=nnu, lTint=lTint, Tint=Tint, nuint=nuint,
+                kapint=np.ascontiguousarray(kapint.T), kappap=kappap)
+
+
+# ---------------------------------------------------------------------------
+# configurations
+# ---------------------------------------------------------------------------
+def tophat_bands(lam_ranges):
+    """Filter tables for top-hat bands (lambda ranges in um): three lines per
+    band so that filt_nu's lookup (peeloff_3d_2_test.f90:250-263) returns
+    weight 1 strictly inside (f_lo, f_hi)."""
+    nb = len(lam_ranges)
+    freq = np.zeros((max(nb, 1), 3))
+    phi = np.zeros((max(nb, 1), 3))
+    for b, (l0, l1) in enumerate(lam_ranges):
+        f_lo, f_hi = 2.9979e14 / l1, 2.9979e14 / l0
+        freq[b] = [f_lo, f_hi, f_hi]
+        phi[b] = [1.0, 1.0, 1.0]
+    return np.full(nb, 3, dtype=np.int32), freq, phi
+
+
+def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
+          rstar=2.0, tstar_k=4000.0, mdisk=0.01, mdot=1e-6, rc_au=100.0,
+          clump_sigma=0.0, clump_dim=2.6, clump_seed=1001, pl=0.0,
+          npeel=10, nx=101, bands=((0.01, 5000.0),), nfreq=100, nmu=10,
+          npackets=1_000_000, niter=5, nlam=100, nT=500, nnu=513,
+          thete_deg=None, phie_deg=0.0) -> Model:
+    rarr = radial_walls(nr, rin_au, rout_au, rstar)
+    thetarr = theta_walls(nt) if nt > 1 else np.array([0.0, math.pi])
+    costarr, tan2arr = theta_tables(thetarr)
+    phiarr, aarr, barr = phi_walls(nphi)
+    vcell = cell_volumes(rarr, thetarr, phiarr, rstar)
+    rho, species = yso_density(rarr, thetarr, phiarr, vcell, rstar, mdisk, rin_au, rout_au,
+                               mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
+                               clump_dim=clump_dim, clump_seed=clump_seed)
+    dust = synthetic_dust(nlam=nlam, pl=pl)
+    em = emission_tables(dust, nT=nT, nnu=nnu)
+    if thete_deg is None:
+        thete_deg = (np.arange(npeel) + 0.5) * (90.0 / npeel)
+    thete = np.radians(np.asarray(thete_deg, dtype=np.float64))
+    if np.any(np.min(np.abs(thete[:, None] - thetarr[None, :]), axis=1) == 0.0):
+        raise ValueError("a peel-off angle equals a wall angle (gridset_5_model13new.f90:695-701)")
+    band_nw, band_freq, band_phi = tophat_bands(bands)
+    rmax = float(rarr[-1])
+    numin, numax = 1.2398 / 5000.0, 1.2398 / 0.01                 # setup.f90:36-41
+    species_opac = np.zeros(11, dtype=np.int32)
+    return Model(
+        name=name, nrg=len(rarr), ntg=len(thetarr), npg=len(phiarr),
+        rarr=rarr, r2arr=rarr ** 2, thetarr=thetarr, costarr=costarr, tan2arr=tan2arr,
+        phiarr=phiarr, aarr=aarr, barr=barr, vcell=vcell, rho=rho, species=species,
+        species_opac=species_opac, nopac=1, nlam=nlam,
+        nudust=dust["nudust"][None, :].copy(), kappad=dust["kappad"][None, :].copy(),
+        adust=dust["adust"][None, :].copy(), gdust=dust["gdust"][None, :].copy(),
+        pldust=dust["pldust"][None, :].copy(), kappav=np.array([dust["kappav"]]),
+        nT=em["nT"], nnu=em["nnu"], lTint=em["lTint"], Tint=em["Tint"], nuint=em["nuint"],
+        kapint=em["kapint"][None, :, :].copy(), kappap=em["kappap"][None, :].copy(),
+        rstar=rstar, tstar=tstar_k / TK, rmin=1.0,
+        ipeel=1 if npeel > 0 else 0, npeel=npeel, thete=thete,
+        phie=np.full(npeel, math.radians(phie_deg)), nx=nx, rmaxi=1.1 * rmax,
+        nband=len(bands), nwmax=3, band_nw=band_nw, band_freq=band_freq, band_phi=band_phi,
+        nfreq=nfreq, nmu=nmu, nph=1, nap=1, aperture2=np.array([(2.0 * rmax) ** 2]),
+        u_mu=1.0 - (2.0 / nmu) * (np.arange(nmu) + 0.5), numin=numin,
+        lnurat=math.log(numax / numin), npackets=npackets, niter=niter)
+
+
+CONFIGS: Dict[str, dict] = {
+    # 2-D axisymmetric, FP64 plain method (CPU-runnable case)
+    "c0": dict(nr=100, nt=90, nphi=1, npeel=10, nx=101, bands=((0.01, 5000.0),),
+               npackets=1_000_000, niter=5),
+    # 3-D polarized clumpy; FP32 transport with FP64 face roots and tallies
+    "c1": dict(nr=200, nt=180, nphi=64, clump_sigma=0.5, clump_seed=1001, pl=0.5,
+               npeel=10, nx=256, bands=((1.1, 1.4), (2.0, 2.4), (3.2, 4.0)),
+               npackets=100_000_000, niter=5),
+    # adjoint model
+    "c3": dict(nr=150, nt=120, nphi=32, clump_sigma=0.3, clump_seed=1003, pl=0.5,
+               npeel=1, thete_deg=[60.0], nx=128, bands=((2.0, 2.4), (3.2, 4.0)),
+               npackets=10_000_000, niter=1),
+    # large optically thick Lucy run
+    "c4": dict(nr=400, nt=360, nphi=128, mdisk=0.1, pl=0.5, npeel=10, nx=512,
+               bands=((2.0, 2.4),), npackets=1_000_000_000, niter=10),
+}
+
+
+def config(name: str, **over) -> Model:
+    kw = dict(CONFIGS[name])
+    kw.update(over)
+    return build(name, **kw)
+
+
+def small(name="small", nr=24, nt=16, nphi=1, **over) -> Model:
+    """A few-thousand-cell model with the same physics, for parity tests."""
+    kw = dict(nr=nr, nt=nt, nphi=nphi, npeel=4, nx=33, nfreq=20, nmu=4, npackets=2000, niter=1,
+              nlam=40, nT=100, nnu=65)
+    kw.update(over)
+    return build(name, **kw)
