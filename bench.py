@@ -1,0 +1,204 @@
+"""Benchmark: photon packets/s of the mixed-precision engine on the 3-D
+polarized clumpy disk+envelope workload (200x180x64 cells), against the FP64
+engine on the same RNG counters, with the SED / temperature error of the
+mixed path stated, the roofline of the packet kernel, and the CPU oracle
+timed on the host cores.
+
+A step is one Lucy iteration: zero the tallies, run the step's packets
+(emission, cell walk, polarized scattering, 10-direction peel-off into
+256x256 images in 3 bands and SEDs, FP64 absorbed-energy tally), then the
+FP64 temperature inversion that the next step re-emits with.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "photon packets/s, mixed-prec vs FP64 GPU path at SED/T err ≤1% of FP64 oracle"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(c, npackets):
+    """Bytes the packet path must move per launch:
+    transport segment = density (4 B) + species (1 B) + FP64 dtauabs RMW (16 B);
+    peel segment = density + species (5 B); peel event reaching the observer =
+    peeled SED 3 components x 9 floats RMW (216 B) + image I,Q,U,V,origin RMW
+    (40 B); escaped packet = SED 3 components x 10 floats RMW (240 B)."""
+    return (c["segments"] * 21.0 + c["peel_segments"] * 5.0 + c["peel_events"] * 256.0
+            + c["escaped"] * 240.0)
+
+
+def sed_error(a, b):
+    """Flux-weighted relative L1 of the total SED over (frequency, inclination)
+    bins holding >= 100 packets in the reference run b."""
+    fa, fb = a["ti"][0, 0].astype(np.float64), b["ti"][0, 0].astype(np.float64)
+    nb = b["numt"][0, 0]
+    k = nb >= 100
+    return float(np.sum(np.abs(fa[k] - fb[k])) / max(np.sum(np.abs(fb[k])), 1e-300))
+
+
+def t_error(ta, tb, mass):
+    k = (tb > 0.1 / 11605.0 * 1.0001) & (mass > 0)
+    rel = np.abs(ta[k] - tb[k]) / tb[k]
+    w = mass[k] / mass[k].sum()
+    return float(np.sum(w * rel)), float(np.percentile(rel, 99))
+
+
+def cpu_baseline(m, tdust, seconds, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    probe = 2000
+    t0 = time.perf_counter()
+    oracle_py.run_packets(m, probe, seed=777, first_packet=10 ** 12, tdust=tdust, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n = int(max(probe, min(5_000_000, probe * seconds / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    oracle_py.run_packets(m, n, seed=778, first_packet=2 * 10 ** 12, tdust=tdust, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--packets", type=int, default=0, help="packets per step (default: auto)")
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the path shards by packet ranges with one tally all-reduce per Lucy iteration; "
+              "see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from src3_mpi_cfitsio_amd import model as M
+    from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine
+
+    t0 = time.perf_counter()
+    m = M.config(args.config)
+    log(f"model {args.config}: {m.nrg-1}x{m.ntg-1}x{m.npg-1} cells built in {time.perf_counter()-t0:.1f}s")
+    mass = m.rho * m.vcell
+    eng = Engine(m)
+
+    # temperature warm-up: short Lucy iterations so the timed steps transport
+    # packets through a heated disk (re-emission at the dust temperature)
+    nw = 2_000_000
+    for k in range(4):
+        eng.zero_tallies()
+        eng.run_packets(nw, seed=1000 + k, precision=MCRT_MIXED, first_packet=0)
+        eng.synchronize()
+        tw = eng.last_run_ms() / 1e3
+        eng.dusttemp(nw, update_tdust=True)
+        log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s)")
+    rate_guess = nw / tw
+    N = args.packets or int(min(m.npackets, max(1_000_000, rate_guess * 15.0)))
+    N = (N // 1000) * 1000
+    log(f"packets per step: {N}")
+
+    seed = 20250101
+    step_ms, kern_ms, counters = [], [], []
+    first = 0
+    for s in range(args.warmup + args.steps):
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.zero_tallies()
+        eng.run_packets(N, seed=seed, precision=MCRT_MIXED, first_packet=first)
+        tal = eng.tallies()                        # post-reduction tallies to the host
+        eng.dusttemp(N, update_tdust=True)
+        eng.synchronize()
+        dt = (time.perf_counter() - t0) * 1e3
+        km = eng.last_run_ms()
+        first += N
+        if s >= args.warmup:
+            step_ms.append(dt)
+            kern_ms.append(km)
+            counters.append(tal["counters"])
+        log(f"step {s} ({'warm-up' if s < args.warmup else 'timed'}): {dt:.0f} ms, kernel {km:.0f} ms, "
+            f"{N/(dt/1e3):.4g} packets/s, segments/packet {tal['counters']['segments']/N:.0f}, "
+            f"peel segments/packet {tal['counters']['peel_segments']/N:.0f}, errors {tal['counters']['errors']:.0f}")
+    ms = float(np.mean(step_ms))
+    value = N / (ms / 1e3)
+    kmean = float(np.mean(kern_ms))
+    c = {k: float(np.mean([cc[k] for cc in counters])) for k in counters[0]}
+    B = algorithmic_bytes(c, N)
+    achieved = B / (kmean / 1e3) / 1e9
+
+    # FP64 path and the mixed path's error against it on identical counters
+    tdust_state = None
+    extra = {}
+    if not args.no_fp64:
+        Nc = max(1_000_000, N // 4)
+        tfix = eng.dusttemp(N, update_tdust=False)
+        eng.set_tdust(tfix)
+        res = {}
+        for prec in (MCRT_FP64, MCRT_MIXED):
+            eng.set_tdust(tfix)
+            eng.zero_tallies()
+            eng.synchronize()
+            t0 = time.perf_counter()
+            eng.run_packets(Nc, seed=seed + 1, precision=prec, first_packet=10 ** 11)
+            tal = eng.tallies()
+            T = eng.dusttemp(Nc, mode=1, update_tdust=False)
+            eng.synchronize()
+            dt = time.perf_counter() - t0
+            res[prec] = (tal, T, dt, eng.last_run_ms())
+            log(f"{'fp64' if prec == MCRT_FP64 else 'mixed'} comparison run: {Nc} packets in {dt:.2f}s "
+                f"({Nc/dt:.4g} packets/s)")
+        tdust_state = tfix
+        sed_e = sed_error(res[MCRT_MIXED][0], res[MCRT_FP64][0])
+        t_mean, t_p99 = t_error(res[MCRT_MIXED][1], res[MCRT_FP64][1], mass)
+        fp64_rate = Nc / res[MCRT_FP64][2]
+        extra = dict(fp64_value=fp64_rate, mixed_over_fp64=(Nc / res[MCRT_MIXED][2]) / fp64_rate,
+                     sed_err_vs_fp64=sed_e, t_err_vs_fp64_mass_weighted=t_mean, t_err_vs_fp64_p99=t_p99,
+                     fp64_comparison_packets=Nc)
+    else:
+        tdust_state = eng.dusttemp(N, update_tdust=False)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, n, dt = cpu_baseline(m, tdust_state, args.cpu_seconds, threads)
+        cpu = {"value": rate, "unit": "packets/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 C oracle, same {args.config} model and dust temperature, {n} packets "
+                         f"in {dt:.1f}s on {threads} host threads (one packet range per thread)"}
+        log(f"cpu baseline: {rate:.4g} packets/s on {threads} threads")
+    eng.close()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: 3-D polarized clumpy disk+Ulrich envelope, "
+                               f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
+                               f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
+                   "packets_per_step": N, "precision": "mixed (FP32 transport, FP64 radial roots and tallies)",
+                   "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_packets<float>", "kernel_ms": kmean,
+                     "algorithmic_bytes_per_launch": B},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

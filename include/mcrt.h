@@ -103,6 +103,14 @@ int mcrt_zero_tallies(mcrt_ctx *ctx);
 int mcrt_run_packets(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
                      uint64_t seed, int32_t precision);
 
+/* Launch configuration of mcrt_run_packets (no counterpart in the reference,
+ * whose rank loop has none): packets per batch, peel-off queue slots per
+ * packet of a batch (-1 walks every peel-off inline in the transport kernel),
+ * and resident blocks per CU for the transport and peel kernels.  0 keeps
+ * the built-in default.  Results do not depend on these values. */
+int mcrt_set_launch(mcrt_ctx *ctx, int64_t batch_packets, int32_t queue_per_packet,
+                    int32_t transport_blocks_per_cu, int32_t peel_blocks_per_cu);
+
 /* Post-ALLREDUCE tallies (newdisktherm_5_fuv.f90:524-602): any pointer may be
  * NULL.  Layouts: dtauabs/nabs/dbig/killed per cell; sed
  * [10][8][nap][nph][nmu][nfreq] = si,sq,su,sv,si2,sq2,su2,sv2,nums,aveinc;

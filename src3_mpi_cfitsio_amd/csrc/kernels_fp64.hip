@@ -12,6 +12,13 @@ hipError_t launch_packets_fp64(const DeviceModel& m, const DeviceTallies& t, con
   return hipGetLastError();
 }
 
+hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel<double>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  return hipGetLastError();
+}
+
+size_t peel_record_bytes_fp64() { return sizeof(PeelRec<double>); }
+
 hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_trace<double>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
   return hipGetLastError();

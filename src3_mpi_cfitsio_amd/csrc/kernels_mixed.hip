@@ -11,6 +11,13 @@ hipError_t launch_packets_mixed(const DeviceModel& m, const DeviceTallies& t, co
   return hipGetLastError();
 }
 
+hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel<float>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  return hipGetLastError();
+}
+
+size_t peel_record_bytes_mixed() { return sizeof(PeelRec<float>); }
+
 hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_trace<float>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
   return hipGetLastError();

@@ -28,6 +28,12 @@ struct mcrt_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool ev_valid = false;
   int cu_count = 256;
+  // launch configuration (mcrt_set_launch); 0 = built-in default
+  int64_t batch_packets = 0;
+  int64_t queue_per_packet = 0;
+  bool inline_peel = false;
+  int transport_blocks_per_cu = 0;
+  int peel_blocks_per_cu = 0;
 };
 
 namespace {
@@ -370,22 +376,64 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     if (rc) return rc;
   }
   if (npackets == 0) return MCRT_OK;
-  RunParams rp;
-  rp.first_packet = (uint64_t)first_packet;
-  rp.npackets = (uint64_t)npackets;
-  rp.seed = seed;
-  rp.inczeta = (const double*)c->buf["inczeta"];
   DeviceTallies t = c->dt;
   t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
-  int64_t blocks = (npackets + 255) / 256;
-  int64_t cap = (int64_t)c->cu_count * 64;
-  int grid = (int)(blocks < cap ? blocks : cap);
+  const bool fp64 = (precision == MCRT_FP64);
+  // packets run in batches; each batch's peel-off events are queued by the
+  // transport kernel and walked by the peel kernel right after it
+  const int64_t batch = c->batch_packets > 0 ? c->batch_packets : (int64_t)1 << 20;
+  const int64_t per = c->queue_per_packet > 0 ? c->queue_per_packet : 32;
+  PeelQueue q{nullptr, nullptr, 0};
+  if (c->dm.ipeel == 1 && c->dm.npeel > 0 && per > 0 && !c->inline_peel) {
+    const size_t rb = fp64 ? peel_record_bytes_fp64() : peel_record_bytes_mixed();
+    int64_t nb = npackets < batch ? npackets : batch;
+    int64_t capacity = nb * per;
+    if (capacity > ((int64_t)1 << 31)) capacity = (int64_t)1 << 31;
+    void* p;
+    int rc = dev_alloc(c, fp64 ? "q.recs64" : "q.recs32", (size_t)capacity * rb, &p);
+    if (rc) return rc;
+    q.recs = p;
+    if ((rc = dev_alloc(c, "q.count", 64, &p))) return rc;
+    q.count = (unsigned*)p;
+    q.capacity = (unsigned)capacity;
+  }
+  const int tgrid_cap = c->cu_count * (c->transport_blocks_per_cu > 0 ? c->transport_blocks_per_cu : 8);
+  const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  hipError_t e = (precision == MCRT_FP64) ? launch_packets_fp64(c->dm, t, rp, grid, c->stream)
-                                          : launch_packets_mixed(c->dm, t, rp, grid, c->stream);
-  if (e != hipSuccess) return hipfail(c, e, "packet kernel launch");
+  for (int64_t done = 0; done < npackets; done += batch) {
+    const int64_t n = (npackets - done) < batch ? (npackets - done) : batch;
+    RunParams rp;
+    rp.first_packet = (uint64_t)(first_packet + done);
+    rp.npackets = (uint64_t)n;
+    rp.seed = seed;
+    rp.inczeta = (const double*)c->buf["inczeta"];
+    rp.q = q;
+    rp.trace_base = (uint64_t)done;
+    if (q.capacity) HIPCHECK(c, launch_fill_i32((int*)q.count, 1, 0, c->stream));
+    int64_t blocks = (n + 255) / 256;
+    int grid = (int)(blocks < tgrid_cap ? blocks : tgrid_cap);
+    hipError_t e = fp64 ? launch_packets_fp64(c->dm, t, rp, grid, c->stream) : launch_packets_mixed(c->dm, t, rp, grid, c->stream);
+    if (e != hipSuccess) return hipfail(c, e, "packet kernel launch");
+    if (q.capacity) {
+      e = fp64 ? launch_peel_fp64(c->dm, t, rp, pgrid, c->stream) : launch_peel_mixed(c->dm, t, rp, pgrid, c->stream);
+      if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
+    }
+  }
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->ev_valid = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_launch(mcrt_ctx* c, int64_t batch_packets, int32_t queue_per_packet, int32_t transport_blocks_per_cu,
+                    int32_t peel_blocks_per_cu) {
+  if (!c) return MCRT_E_ARG;
+  if (batch_packets < 0 || queue_per_packet < -1 || transport_blocks_per_cu < 0 || peel_blocks_per_cu < 0)
+    return fail(c, MCRT_E_ARG, "mcrt_set_launch: negative value");
+  c->batch_packets = batch_packets;
+  c->inline_peel = (queue_per_packet == -1);
+  c->queue_per_packet = queue_per_packet > 0 ? queue_per_packet : 0;
+  c->transport_blocks_per_cu = transport_blocks_per_cu;
+  c->peel_blocks_per_cu = peel_blocks_per_cu;
   return MCRT_OK;
 }
 

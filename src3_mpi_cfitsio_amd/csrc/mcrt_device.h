@@ -85,7 +85,7 @@ __device__ __forceinline__ size_t cell_index(const DeviceModel& m, int ir, int i
 // ---------------------------------------------------------------------------
 struct Rng {
   uint32_t k0, k1, p0, p1, n;
-  uint32_t buf[4];
+  uint32_t b0, b1, b2, b3;      // current Philox block (scalars: no indexed private memory)
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
@@ -109,8 +109,13 @@ __device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet)
 }
 
 __device__ __forceinline__ uint32_t rng_word(Rng& r) {
-  if ((r.n & 3u) == 0u) philox(r.n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, r.buf);
-  uint32_t w = r.buf[r.n & 3u];
+  const uint32_t j = r.n & 3u;
+  if (j == 0u) {
+    uint32_t o[4];
+    philox(r.n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
+    r.b0 = o[0]; r.b1 = o[1]; r.b2 = o[2]; r.b3 = o[3];
+  }
+  uint32_t w = (j == 0u) ? r.b0 : (j == 1u) ? r.b1 : (j == 2u) ? r.b2 : r.b3;
   r.n++;
   return w;
 }
@@ -313,6 +318,17 @@ template <class T> struct Pkt {
   Opac<T> op[kMaxOpac];
 };
 
+// Runtime-indexed reads of the per-packet opacity arrays through an unrolled
+// select chain, so the arrays stay in VGPRs instead of private memory.
+template <class V>
+__device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
+  V v = a[0];
+#pragma unroll
+  for (int k = 1; k < kMaxOpac; ++k)
+    if (i == k) v = a[k];
+  return v;
+}
+
 // Per-thread accumulators flushed once per packet batch.
 struct LocalCounts {
   unsigned long long c[C_NCOUNTERS];
@@ -324,7 +340,9 @@ struct LocalCounts {
 
 template <class T>
 __device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int order) {
-  for (int id = 0; id < m.nopac; ++id) {
+#pragma unroll
+  for (int id = 0; id < kMaxOpac; ++id) {
+    if (id >= m.nopac) break;
     if (order == 0) p.kapd[id] = (T)(p.op[id].kappa * (T)(m.rstar * kRsol) * (T)m.kappav[id]);
     else if (order == 1) p.kapd[id] = (T)(p.op[id].kappa * (T)m.rstar * (T)kRsol * (T)m.kappav[id]);
     else p.kapd[id] = (T)(p.op[id].kappa * (T)kRsol * (T)m.rstar * (T)m.kappav[id]);
@@ -333,8 +351,10 @@ __device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int or
 
 // dust_5.f90:9-109 (dust species, no gas mixing)
 template <class T>
-__device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T>* o) {
-  for (int id = 0; id < m.nopac; ++id) {
+__device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T> (&o)[kMaxOpac]) {
+#pragma unroll
+  for (int id = 0; id < kMaxOpac; ++id) {
+    if (id >= m.nopac) break;
     const double* nd = m.nudust + (size_t)id * m.nlam;
     const double *kd = m.kappad + (size_t)id * m.nlam, *ad = m.adust + (size_t)id * m.nlam;
     const double *gd = m.gdust + (size_t)id * m.nlam, *pd = m.pldust + (size_t)id * m.nlam;
@@ -370,7 +390,7 @@ __device__ __forceinline__ T bbfreq(Rng& r, T teff, const double* inczeta) {
 
 // lucydustfreq_2_test.f90:19-160 with xi supplied
 template <class T>
-__device__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop, T xi) {
+__device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop, T xi) {
   const int nT = m.nT, nnu = m.nnu;
   const double* lT = m.lTint;
   const double* nui = m.nuint;
@@ -439,7 +459,7 @@ __device__ __forceinline__ T cs_peak(T g) {  // stokes.f90:54-70
 // stokes.f90:117-336 for an accepted (bmu, ri1).  dir = cost,sint,phi,cosp,sinp.
 // Returns false (no change) for bmu == 1 as the reference does.
 template <class T>
-__device__ void stokes_apply(T g, T pl, T bmu, T ri1, T dir[5], T s[4]) {
+__device__ __forceinline__ void stokes_apply(T g, T pl, T bmu, T ri1, T dir[5], T s[4]) {
   const T one = (T)1, two = (T)2, zero = (T)0;
   const T PI = (T)kPi, R2P = (T)kR2p;
   T costp = dir[0], sintp = dir[1], phip = dir[2];
@@ -522,7 +542,7 @@ __device__ void stokes_apply(T g, T pl, T bmu, T ri1, T dir[5], T s[4]) {
 
 // stokespeel.f90:77-322: fixed (bmu, ri1), no division by the phase function.
 template <class T>
-__device__ void stokespeel(T g, T pl, T s[4], T cost, T sint, T ri1, T bmu, T costp, T sintp) {
+__device__ __forceinline__ void stokespeel(T g, T pl, T s[4], T cost, T sint, T ri1, T bmu, T costp, T sintp) {
   const T one = (T)1, two = (T)2, zero = (T)0;
   const T PI = (T)kPi, R2P = (T)kR2p;
   T sip = s[0], sqp = s[1], sup = s[2], svp = s[3];
@@ -620,7 +640,7 @@ __device__ __forceinline__ void findangle3(T cost1, T sint1, T phi1, T cosp1, T 
 // testgrid.f90:40-124.  Returns true when state changed.
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& rng) {
+__device__ __forceinline__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& rng) {
   const T tol = (T)1.0e-4, one = (T)1, zero = (T)0;
   const T PI = (T)kPi, R2P = (T)kR2p;
   const int nrg = m.nrg, ntg = m.ntg, npg = m.npg;
@@ -714,7 +734,7 @@ __device__ __forceinline__ T cell_opac(const DeviceModel& m, const Pkt<T>& p, si
   int sp = m.species[c];
   rho = cell_rho<T>(m, c);
   iop = -1;
-  if (sp > 0 && rho > (T)0) { iop = m.species_opac[sp]; return (T)0 + p.kapd[iop] * rho; }
+  if (sp > 0 && rho > (T)0) { iop = m.species_opac[sp]; return (T)0 + pick(p.kapd, iop) * rho; }
   return (T)0;
 }
 

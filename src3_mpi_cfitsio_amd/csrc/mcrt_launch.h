@@ -10,6 +10,10 @@ struct TraceArgs;
 
 hipError_t launch_packets_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
 hipError_t launch_packets_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+size_t peel_record_bytes_fp64();
+size_t peel_record_bytes_mixed();
 hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
 hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
 hipError_t launch_mueller_fp64(int n, int kind, const double* par, double* dir, double* st, hipStream_t s);

@@ -7,11 +7,35 @@
 
 namespace mcrt {
 
+// A peel-off event queued by the transport kernel: everything peeloff
+// (peeloff_3d_2_test.f90:65-89) copies in from the packet at the event.
+// One record serves all npeel directions.  For the direct stellar peel
+// (newdisktherm_5_fuv.f90:371-414) the incident-direction slots carry
+// cosb, sinb, cos(lp), sin(lp) and the weight is formed per direction.
+template <class T>
+struct alignas(16) PeelRec {
+  T xp, yp, zp, rsq, rtot;
+  T sip, sqp, sup, svp;
+  T cost, sint, phi, cosp, sinp;
+  T nub, wave, g, pl;
+  T kapd[kMaxOpac];
+  uint32_t cell;     // ir | it << 10 | ip << 20 | on_wall << 30
+  uint32_t flags;    // iflag | i_orig << 1 | direct << 4 | species << 8
+};
+
+struct PeelQueue {
+  void* recs;              // PeelRec<T>[capacity]
+  unsigned* count;         // slots reserved (may exceed capacity: overflow runs inline)
+  unsigned capacity;
+};
+
 struct RunParams {
   uint64_t first_packet;
   uint64_t npackets;
   uint64_t seed;
   const double* inczeta;   // 145 partial sums of zeta(4), random.f90:74-92
+  PeelQueue q;             // q.capacity == 0: every peel-off walk runs inline
+  uint64_t trace_base;     // index of this launch's first packet in the trace buffer
 };
 
 template <class T>
@@ -21,9 +45,22 @@ struct Ctx {
   Rng rng;
   LocalCounts cnt;
   const double* inczeta;
+  PeelQueue q;
 };
 
 template <class T> __device__ __forceinline__ void atomic_add_dtau(double* a, T v) { atomicAdd(a, (double)v); }
+
+// One atomic per wavefront: the active lanes reserve consecutive slots.
+__device__ __forceinline__ unsigned reserve_slot(unsigned* counter) {
+  unsigned long long mask = __ballot(1);
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+  unsigned base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  return base + rank;
+}
 
 // initp_common.f90:1-31 (limb = 0)
 template <class T>
@@ -78,7 +115,7 @@ __device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
 
 // tauint_3d_2_fuv.f90:5-210.  Returns false on a walk error.
 template <class T>
-__device__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
+__device__ __forceinline__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
   const DeviceModel& m = C.m;
   T tsum = (T)0, dt;
   int iface;
@@ -95,7 +132,7 @@ __device__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
     T dtau = opac * dt;
     if ((tsum + dtau) > tau) {
       T ds = (tau - tsum) / opac;
-      if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)p.kapd[iop] * (double)rho * (1.0 - (double)p.op[iop].albedo));
+      if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
       p.xp = p.xp + ds * p.ux;
       p.yp = p.yp + ds * p.uy;
       p.zp = p.zp + ds * p.uz;
@@ -104,7 +141,7 @@ __device__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
       p.on_wall = false;
       return true;
     }
-    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)p.kapd[iop] * (double)rho * (1.0 - (double)p.op[iop].albedo));
+    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
     cross_face(m, p.ir, p.it, p.ip, iface);
     p.xp = p.xp + dt * p.ux;
     p.yp = p.yp + dt * p.uy;
@@ -145,7 +182,7 @@ __device__ __forceinline__ int freq_bin(const DeviceModel& m, T nub) {
 
 // peeloff_3d_2_test.f90:7-347.  `q` is the caller's packet (copied).
 template <class T>
-__device__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, int idust_sp) {
+__device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, T gsc, T plsc) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
   Pkt<T> p = q;
@@ -191,8 +228,7 @@ __device__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, int id
   if (q.iflag == 1) {
     T bmu, sinbm, phinew, ri1;
     findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
-    int iop = m.species_opac[idust_sp];
-    stokespeel<T>(p.op[iop].g, p.op[iop].pl, s, coste, sinte, ri1, bmu, q.cost, q.sint);
+    stokespeel<T>(gsc, plsc, s, coste, sinte, ri1, bmu, q.cost, q.sint);
   }
   T yimage = p.zp * sinte - p.yp * coste * sinpe - p.xp * coste * cospe;
   T ximage = p.yp * cospe - p.xp * sinpe;
@@ -246,11 +282,59 @@ __device__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, int id
   }
 }
 
+// ---------------------------------------------------------------------------
+// Peel-off events: queued for the peel kernel, or walked inline when the
+// queue is full or disabled (same peeloff arithmetic either way).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned slot, int sp, bool direct,
+                                          T coslp, T sinlp) {
+  PeelRec<T>* r = (PeelRec<T>*)C.q.recs + slot;
+  r->xp = p.xp; r->yp = p.yp; r->zp = p.zp; r->rsq = p.rsq; r->rtot = p.rtot;
+  r->sip = p.sip; r->sqp = p.sqp; r->sup = p.sup; r->svp = p.svp;
+  if (direct) { r->cost = p.cosb; r->sint = p.sinb; r->phi = (T)0; r->cosp = coslp; r->sinp = sinlp; }
+  else { r->cost = p.cost; r->sint = p.sint; r->phi = p.phi; r->cosp = p.cosp; r->sinp = p.sinp; }
+  r->nub = p.nub; r->wave = p.wave;
+  const Opac<T> o = pick(p.op, C.m.species_opac[sp]);
+  r->g = o.g; r->pl = o.pl;
+#pragma unroll
+  for (int i = 0; i < kMaxOpac; ++i) r->kapd[i] = p.kapd[i];
+  r->cell = (uint32_t)p.ir | ((uint32_t)p.it << 10) | ((uint32_t)p.ip << 20) | ((uint32_t)(p.on_wall ? 1 : 0) << 30);
+  r->flags = (uint32_t)p.iflag | ((uint32_t)p.i_orig << 1) | ((uint32_t)(direct ? 1 : 0) << 4) | ((uint32_t)sp << 8);
+}
+
+// scattering / re-emission peel (propagate_5_fuv.f90:83-90, :162-170)
+template <class T>
+__device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp) {
+  if (C.q.capacity) {
+    unsigned slot = reserve_slot(C.q.count);
+    if (slot < C.q.capacity) { write_rec<T>(C, p, slot, sp, false, (T)0, (T)0); return; }
+  }
+  const Opac<T> o = pick(p.op, C.m.species_opac[sp]);
+  for (int pid = 1; pid <= C.m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, o.g, o.pl);
+}
+
+// direct stellar peel, weight 4 I cos(norm) (newdisktherm_5_fuv.f90:371-414)
+template <class T>
+__device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp, T sinlp) {
+  if (C.q.capacity) {
+    unsigned slot = reserve_slot(C.q.count);
+    if (slot < C.q.capacity) { write_rec<T>(C, p, slot, 0, true, coslp, sinlp); return; }
+  }
+  for (int pk = 1; pk <= C.m.npeel; ++pk) {
+    const double* pe = C.m.peel + 5 * (pk - 1);
+    T cosnorm = p.cosb * (T)pe[0] + (p.sinb * (T)pe[1] * ((T)pe[2] * coslp + (T)pe[3] * sinlp));
+    T sipnew = (T)4 * p.sip * cosnorm;
+    if (cosnorm > (T)0) peeloff<T>(C, p, sipnew, pk, (T)0, (T)0);
+  }
+}
+
 // stokes.f90:46-137 rejection sampling, then the Mueller update
 template <class T>
-__device__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
+__device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
   const T PI = (T)kPi, R2P = (T)kR2p, one = (T)1, two = (T)2;
-  T g = p.op[iop].g, pl = p.op[iop].pl;
+  const Opac<T> o = pick(p.op, iop);
+  T g = o.g, pl = o.pl;
   T peak = cs_peak<T>(g), bmu, ri1, rprob, pp[4];
   for (;;) {
     C.cnt.c[C_HTOT]++;
@@ -282,7 +366,7 @@ __device__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
 
 // propagate_5_fuv.f90:1-326.  Returns 1 escaped, 2 killed, -1 error.
 template <class T>
-__device__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
+__device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
   const T R2P = (T)kR2p;
@@ -300,10 +384,9 @@ __device__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
       size_t c = cell_index(m, p.ir, p.it, p.ip);
       int iop = m.species_opac[idust2];
       nint++;
-      if (uniform<T>(C.rng) <= p.op[iop].albedo) {
+      if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
         p.iflag = 1; iscat++; iint++;
-        if (m.ipeel == 1)
-          for (int pid = 1; pid <= m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, idust2);
+        if (m.ipeel == 1) peel_event<T>(C, p, idust2);
         stokes<T>(C, p, iop);
       } else {
         atomicAdd(&t.dbig[c], 1);
@@ -323,8 +406,7 @@ __device__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
         opacset<T>(m, p.nub, p.op);
         p.wave = (T)1.2398 / p.nub;
         set_kapd(p, m, 1);
-        if (m.ipeel == 1)
-          for (int pid = 1; pid <= m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, idust2);
+        if (m.ipeel == 1) peel_event<T>(C, p, idust2);
         iint++;
       }
       tau = -RT<T>::log_(uniform<T>(C.rng));
@@ -385,7 +467,7 @@ __device__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
 
 // newdisktherm_5_fuv.f90:294-413 one stellar photon (ips = 0, no spots)
 template <class T>
-__device__ void one_packet(Ctx<T>& C, uint64_t pid) {
+__device__ __forceinline__ void one_packet(Ctx<T>& C, uint64_t pid) {
   const DeviceModel& m = C.m;
   Pkt<T> p;
   p.on_wall = false;
@@ -417,14 +499,7 @@ __device__ void one_packet(Ctx<T>& C, uint64_t pid) {
   p.on_wall = true;
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
   p.ip = locate<T>(walls_phi<T>(m), m.npg, p.lp);
-  if (m.ipeel == 1) {
-    for (int pk = 1; pk <= m.npeel; ++pk) {
-      const double* pe = m.peel + 5 * (pk - 1);
-      T cosnorm = p.cosb * (T)pe[0] + (p.sinb * (T)pe[1] * ((T)pe[2] * coslp + (T)pe[3] * sinlp));
-      T sipnew = (T)4 * p.sip * cosnorm;
-      if (cosnorm > (T)0) peeloff<T>(C, p, sipnew, pk, 0);
-    }
-  }
+  if (m.ipeel == 1) peel_direct<T>(C, p, coslp, sinlp);
   int nint = 0;
   int code = propagate<T>(C, p, nint);
   if (code == 1) C.cnt.c[C_ESCAPED]++;
@@ -448,14 +523,60 @@ __device__ __forceinline__ void flush_counts(const DeviceTallies& t, LocalCounts
   }
 }
 
+// Transport kernel: one packet per thread (grid-stride).  Peel-off events go
+// to the queue; k_peel walks them.
 template <class T>
 __global__ void __launch_bounds__(256) k_packets(DeviceModel m, DeviceTallies t, RunParams rp) {
-  Ctx<T> C{m, t, {}, {}, rp.inczeta};
+  Ctx<T> C{m, t, {}, {}, rp.inczeta, rp.q};
   C.cnt.zero();
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.npackets; i += stride) {
     rng_init(C.rng, rp.seed, rp.first_packet + i);
-    one_packet<T>(C, i);
+    one_packet<T>(C, rp.trace_base + i);
+  }
+  flush_counts(t, C.cnt);
+}
+
+// Peel kernel: one (event, direction) pair per thread.  Lanes of a
+// wavefront run the same face-walk loop from queued events, so the walk that
+// dominates the path (npeel walks per interaction) runs without the
+// transport kernel's emission / sampling divergence and register load.
+template <class T>
+__global__ void __launch_bounds__(256) k_peel(DeviceModel m, DeviceTallies t, RunParams rp) {
+  Ctx<T> C{m, t, {}, {}, rp.inczeta, PeelQueue{nullptr, nullptr, 0}};
+  C.cnt.zero();
+  const unsigned nq = *rp.q.count;
+  const uint64_t nrec = nq < rp.q.capacity ? nq : rp.q.capacity;
+  const uint64_t total = nrec * (uint64_t)m.npeel;
+  const PeelRec<T>* recs = (const PeelRec<T>*)rp.q.recs;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint64_t ri = i / (uint64_t)m.npeel;
+    const int pk = (int)(i - ri * (uint64_t)m.npeel) + 1;
+    const PeelRec<T>& r = recs[ri];
+    Pkt<T> q;
+    q.xp = r.xp; q.yp = r.yp; q.zp = r.zp; q.rsq = r.rsq; q.rtot = r.rtot;
+    q.sip = r.sip; q.sqp = r.sqp; q.sup = r.sup; q.svp = r.svp;
+    q.cost = r.cost; q.sint = r.sint; q.phi = r.phi; q.cosp = r.cosp; q.sinp = r.sinp;
+    q.nub = r.nub; q.wave = r.wave;
+#pragma unroll
+    for (int k = 0; k < kMaxOpac; ++k) q.kapd[k] = r.kapd[k];
+    q.ir = (int)(r.cell & 1023u); q.it = (int)((r.cell >> 10) & 1023u); q.ip = (int)((r.cell >> 20) & 1023u);
+    q.on_wall = ((r.cell >> 30) & 1u) != 0;
+    q.iflag = (int)(r.flags & 1u);
+    q.i_orig = (int)((r.flags >> 1) & 7u);
+    const bool direct = ((r.flags >> 4) & 1u) != 0;
+    const int sp = (int)((r.flags >> 8) & 255u);
+    rng_init(C.rng, rp.seed ^ 0x5045454C00000000ull, rp.first_packet * 64u + i);   // testgrid resets only
+    T sipnew = q.sip;
+    if (direct) {
+      const double* pe = m.peel + 5 * (pk - 1);
+      T cosnorm = q.cost * (T)pe[0] + (q.sint * (T)pe[1] * ((T)pe[2] * q.cosp + (T)pe[3] * q.sinp));
+      if (!(cosnorm > (T)0)) continue;
+      sipnew = (T)4 * q.sip * cosnorm;
+      q.cosb = q.cost; q.sinb = q.sint;
+    }
+    peeloff<T>(C, q, sipnew, pk, r.g, r.pl);
   }
   flush_counts(t, C.cnt);
 }

@@ -117,6 +117,11 @@ class Engine:
         self._check(self.lib.mcrt_run_packets(self.h, int(first_packet), int(npackets),
                                               C.c_uint64(int(seed)), int(precision)))
 
+    def set_launch(self, batch_packets=0, queue_per_packet=0, transport_blocks_per_cu=0, peel_blocks_per_cu=0):
+        """Launch shape of run_packets (queue_per_packet=-1: peel-offs inline)."""
+        self._check(self.lib.mcrt_set_launch(self.h, int(batch_packets), int(queue_per_packet),
+                                             int(transport_blocks_per_cu), int(peel_blocks_per_cu)))
+
     def synchronize(self):
         self._check(self.lib.mcrt_synchronize(self.h))
 

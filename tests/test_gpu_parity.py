@@ -154,6 +154,30 @@ def test_packets_3d_polarized_fp64(eng3d, oracle_lib):
     assert same.mean() >= 0.99
 
 
+@pytest.mark.parametrize("which", ["eng2d", "eng3d"])
+def test_peel_queue_equals_inline_peeloff(which, request):
+    """Queued peel-offs (transport kernel -> k_peel) and inline peel-offs run
+    the same arithmetic: identical counters, tallies equal up to float
+    summation order.  A tiny queue forces the overflow fallback as well."""
+    e = request.getfixturevalue(which)
+    n = 4000
+    res = []
+    for q in (-1, 0, 1):           # inline, default queue, 1 slot/packet (overflows)
+        e.set_launch(batch_packets=1500, queue_per_packet=q)
+        e.zero_tallies()
+        e.run_packets(n, seed=42)
+        res.append(e.tallies())
+    e.set_launch()
+    a = res[0]
+    for b in res[1:]:
+        for k in ("escaped", "killed", "errors", "segments", "peel_segments", "peel_events", "tot"):
+            assert a["counters"][k] == b["counters"][k], k
+        np.testing.assert_array_equal(a["nabs"], b["nabs"])
+        np.testing.assert_allclose(a["dtauabs"], b["dtauabs"], rtol=1e-12, atol=1e-300)
+        for name in ("ti", "tq", "image_b_i", "image_b_q", "si"):
+            np.testing.assert_allclose(a[name], b[name], rtol=2e-4, atol=1e-6 * float(np.abs(a[name]).max() + 1e-30))
+
+
 def test_dusttemp_matches_oracle(eng2d, oracle_lib):
     e = eng2d
     m = e.model
