@@ -1245,14 +1245,14 @@ int32_t orc_run_packets(const orc_model *m, const double *tdust, orc_tallies *t,
 
 int32_t orc_trace_ray(const orc_model *m, double pos[3], const double u[3], int32_t ijk[3], int on_wall,
                       double tau_stop, int32_t max_steps, int32_t *faces, int32_t *cells,
-                      double *tau_out, int32_t *status)
+                      double *tau_out, int32_t *status, uint64_t stream)
 {
     /* deterministic walk: kapd from the first opacity entry at its current
      * table values (kapd = 1 per unit density when rho is an optical-depth
      * density); no RNG use, no tallies */
     orc_tallies dummy;
     memset(&dummy, 0, sizeof(dummy));
-    orc_run R = { m, NULL, &dummy, { 0, 0, 0, 0 } };
+    orc_run R = { m, NULL, &dummy, { 0, 0, stream, 0 } };
     orc_pkt p;
     memset(&p, 0, sizeof(p));
     p.xp = pos[0]; p.yp = pos[1]; p.zp = pos[2];

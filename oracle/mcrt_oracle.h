@@ -123,10 +123,11 @@ void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double 
  * walks from (x,y,z) in cell ijk along u until tau_stop is reached, the packet
  * escapes (ir==nrg) or hits the star (ir==0).  Records each crossed face code
  * (1..6) and cell after the crossing.  Returns the number of crossings, or -1
- * when no wall was found.  status: 0 stop inside, 1 escaped, 2 hit star, 3 max steps. */
+ * when no wall was found.  status: 0 stop inside, 1 escaped, 2 hit star, 3 max steps.
+ * testgrid re-placements draw from Philox stream (seed 0, `stream`). */
 int32_t orc_trace_ray(const orc_model *m, double pos[3], const double u[3], int32_t ijk[3], int on_wall,
                       double tau_stop, int32_t max_steps, int32_t *faces, int32_t *cells,
-                      double *tau_out, int32_t *status);
+                      double *tau_out, int32_t *status, uint64_t stream);
 
 /* Whole packets: newdisktherm_5_fuv.f90:220-491 (stellar photons) + propagate.
  * rng_mode 0 = counter-based per-packet stream (seed, first_packet+i);

@@ -99,7 +99,7 @@ def oracle():
         L.orc_dusttemp.restype = C.c_double
         L.orc_tfinal.argtypes = [C.POINTER(OrcModel), _dp, C.c_double, _dp]
         L.orc_trace_ray.argtypes = [C.POINTER(OrcModel), _dp, _dp, _ip, C.c_int, C.c_double, C.c_int32,
-                                    _ip, _ip, _dp, _ip]
+                                    _ip, _ip, _dp, _ip, C.c_uint64]
         L.orc_trace_ray.restype = C.c_int32
         L.orc_run_packets.argtypes = [C.POINTER(OrcModel), _dp, C.POINTER(OrcTallies), C.c_uint64,
                                       C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _dp]
@@ -229,7 +229,7 @@ def lucydustfreq(m, T0, iop, xi):
     return oracle().orc_lucydustfreq(C.byref(om.s), float(T0), int(iop), float(xi))
 
 
-def trace_ray(m, pos, u, ijk, on_wall, tau_stop, max_steps=4096, om=None):
+def trace_ray(m, pos, u, ijk, on_wall, tau_stop, max_steps=4096, om=None, stream=0):
     L = oracle()
     om = om or OracleModel(m)
     p = np.array(pos, dtype=np.float64)
@@ -241,6 +241,6 @@ def trace_ray(m, pos, u, ijk, on_wall, tau_stop, max_steps=4096, om=None):
     st = C.c_int32()
     n = L.orc_trace_ray(C.byref(om.s), _p(p, C.c_double), _p(uu, C.c_double), _p(ij, C.c_int32), int(on_wall),
                         float(tau_stop), int(max_steps), _p(faces, C.c_int32), _p(cells, C.c_int32),
-                        C.byref(tau), C.byref(st))
+                        C.byref(tau), C.byref(st), C.c_uint64(int(stream)))
     return dict(n=n, faces=faces[:max(n, 0)], cells=cells[:max(n, 0)], pos=p, ijk=ij, tau=tau.value,
                 status=st.value)

@@ -225,12 +225,6 @@ int mcrt_set_grid(mcrt_ctx* c, int32_t nrg, int32_t ntg, int32_t npg, const doub
   rc |= upload(c, "aarr", aarr, npg, &m.aarr);
   rc |= upload(c, "barr", barr, npg, &m.barr);
   rc |= upload(c, "vcell", vcell, ncell, &m.vcell);
-  rc |= upload_cast<float>(c, "r2arr_f", r2arr, nrg, &m.r2arr_f);
-  rc |= upload_cast<float>(c, "costarr_f", costarr, ntg, &m.costarr_f);
-  rc |= upload_cast<float>(c, "tan2arr_f", tan2arr, ntg, &m.tan2arr_f);
-  rc |= upload_cast<float>(c, "phiarr_f", phiarr, npg, &m.phiarr_f);
-  rc |= upload_cast<float>(c, "aarr_f", aarr, npg, &m.aarr_f);
-  rc |= upload_cast<float>(c, "barr_f", barr, npg, &m.barr_f);
   if (rc) return rc;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->has_grid = true;
@@ -250,6 +244,7 @@ int mcrt_set_medium(mcrt_ctx* c, const double* rho, const uint8_t* species, cons
     if (s > 10) return fail(c, MCRT_E_ARG, "mcrt_set_medium: species code above 10");
     if (s >= 5 && s <= 8 && rho[i] > 0.0) return fail(c, MCRT_E_ARG, "mcrt_set_medium: small-grain species (5-8) are not supported");
     if (!(rho[i] >= 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_medium: negative or NaN density");
+    if ((rho[i] > 0.0) != (s > 0)) return fail(c, MCRT_E_ARG, "mcrt_set_medium: a cell has density without species or species without density");
   }
   HIPCHECK(c, hipSetDevice(c->device));
   DeviceModel& m = c->dm;

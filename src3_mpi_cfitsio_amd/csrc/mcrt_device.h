@@ -37,7 +37,6 @@ struct DeviceModel {
   int nrg, ntg, npg;
   const double *rarr, *r2arr, *thetarr, *costarr, *tan2arr, *phiarr, *aarr, *barr;
   const double* vcell;           // cm^3 per cell
-  const float *costarr_f, *tan2arr_f, *phiarr_f, *aarr_f, *barr_f, *r2arr_f;
   // medium: one thermal species per cell
   const double* rho64;
   const float* rho32;
@@ -157,24 +156,6 @@ template <> struct RT<float> {
   __device__ static float fabs_(float x) { return fabsf(x); }
 };
 
-template <class T> __device__ __forceinline__ const T* walls_cos(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_cos<double>(const DeviceModel& m) { return m.costarr; }
-template <> __device__ __forceinline__ const float* walls_cos<float>(const DeviceModel& m) { return m.costarr_f; }
-template <class T> __device__ __forceinline__ const T* walls_tan2(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_tan2<double>(const DeviceModel& m) { return m.tan2arr; }
-template <> __device__ __forceinline__ const float* walls_tan2<float>(const DeviceModel& m) { return m.tan2arr_f; }
-template <class T> __device__ __forceinline__ const T* walls_phi(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_phi<double>(const DeviceModel& m) { return m.phiarr; }
-template <> __device__ __forceinline__ const float* walls_phi<float>(const DeviceModel& m) { return m.phiarr_f; }
-template <class T> __device__ __forceinline__ const T* walls_a(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_a<double>(const DeviceModel& m) { return m.aarr; }
-template <> __device__ __forceinline__ const float* walls_a<float>(const DeviceModel& m) { return m.aarr_f; }
-template <class T> __device__ __forceinline__ const T* walls_b(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_b<double>(const DeviceModel& m) { return m.barr; }
-template <> __device__ __forceinline__ const float* walls_b<float>(const DeviceModel& m) { return m.barr_f; }
-template <class T> __device__ __forceinline__ const T* walls_r2(const DeviceModel& m);
-template <> __device__ __forceinline__ const double* walls_r2<double>(const DeviceModel& m) { return m.r2arr; }
-template <> __device__ __forceinline__ const float* walls_r2<float>(const DeviceModel& m) { return m.r2arr_f; }
 template <class T> __device__ __forceinline__ T cell_rho(const DeviceModel& m, size_t c);
 template <> __device__ __forceinline__ double cell_rho<double>(const DeviceModel& m, size_t c) { return m.rho64[c]; }
 template <> __device__ __forceinline__ float cell_rho<float>(const DeviceModel& m, size_t c) { return m.rho32[c]; }
@@ -182,15 +163,15 @@ template <> __device__ __forceinline__ float cell_rho<float>(const DeviceModel& 
 // ---------------------------------------------------------------------------
 // closest_wall.f90:13-66
 // ---------------------------------------------------------------------------
-template <class T> struct Walls {
-  T tn1, tp1, tp2;
+struct Walls {
+  double tn1, tp1, tp2;
   int in1, ip1, ip2;
   __device__ __forceinline__ void reset() {
-    tn1 = -RT<T>::huge; tp1 = RT<T>::huge; tp2 = RT<T>::huge;
+    tn1 = -DBL_MAX; tp1 = DBL_MAX; tp2 = DBL_MAX;
     in1 = 0; ip1 = 0; ip2 = 0;
   }
-  __device__ __forceinline__ void insert(T t, int i) {
-    if (t < (T)0) {
+  __device__ __forceinline__ void insert(double t, int i) {
+    if (t < 0.0) {
       if (t > tn1) { tn1 = t; in1 = i; }
     } else {
       if (t < tp1) { tp2 = tp1; ip2 = ip1; tp1 = t; ip1 = i; }
@@ -198,86 +179,84 @@ template <class T> struct Walls {
     }
   }
   // returns false when no wall exists ("ERROR - no valid solution")
-  __device__ __forceinline__ bool next(bool on_wall, T& t, int& i) const {
+  __device__ __forceinline__ bool next(bool on_wall, double& t, int& i) const {
     if (on_wall) {
-      if (RT<T>::fabs_(tn1) < RT<T>::fabs_(tp1)) { t = tp1; i = ip1; }
+      if (fabs(tn1) < fabs(tp1)) { t = tp1; i = ip1; }
       else { t = tp2; i = ip2; }
     } else { t = tp1; i = ip1; }
-    return !(t == RT<T>::huge);
+    return !(t == DBL_MAX);
   }
 };
 
-// radface.f90:44-62.  The radial quadratic is always solved in FP64: r spans
-// 1..1e5 R* so |x|^2 - R^2 cancels catastrophically in FP32 on a shell.
-template <class T>
-__device__ __forceinline__ void radface(Walls<T>& w, T ux_, T uy_, T uz_, T x_, T y_, T z_, double R2_1, double R2_2) {
-  double ux = ux_, uy = uy_, uz = uz_, x1 = x_, y1 = y_, z1 = z_;
+// radface.f90:44-62.  r spans 1..1e5 R*, so |x|^2 - R^2 on a shell cancels
+// catastrophically below FP64.
+__device__ __forceinline__ void radface(Walls& w, double ux, double uy, double uz, double x1, double y1, double z1,
+                                        double R2_1, double R2_2) {
   double bb = 2.0 * (x1 * ux + y1 * uy + uz * z1);
   double cc = x1 * x1 + y1 * y1 + z1 * z1 - R2_1;
   double descr = bb * bb - 4.0 * cc;
   if (descr > 0.0) {
     descr = sqrt(descr);
-    w.insert((T)((-bb + descr) * 0.5), 1);
-    w.insert((T)((-bb - descr) * 0.5), 1);
+    w.insert((-bb + descr) * 0.5, 1);
+    w.insert((-bb - descr) * 0.5, 1);
   }
   cc = x1 * x1 + y1 * y1 + z1 * z1 - R2_2;
   descr = bb * bb - 4.0 * cc;
   if (descr > 0.0) {
     descr = sqrt(descr);
-    w.insert((T)((-bb + descr) * 0.5), 2);
-    w.insert((T)((-bb - descr) * 0.5), 2);
+    w.insert((-bb + descr) * 0.5, 2);
+    w.insert((-bb - descr) * 0.5, 2);
   }
 }
 
-// thetaface.f90:59-115 (equator flagged by tan2 = -1, poles by tan2 = 0)
-template <class T>
-__device__ __forceinline__ void thetaface(Walls<T>& w, T ux, T uy, T uz, T x1, T y1, T z1, T t1, T t2) {
-  const T zero = (T)0, two = (T)2, four = (T)4, half = (T)0.5, eps = (T)1.0e-10;
-  T tmp1 = x1 * ux + y1 * uy, tmp2 = ux * ux + uy * uy, tmp3 = x1 * x1 + y1 * y1;
-  T ta, tb, aa, bb, cc, descr;
+// thetaface.f90:59-115 (equator flagged by tan2 = -1, poles by tan2 = 0).
+// On a cone wall x^2+y^2 - tan^2 z^2 cancels like the radial quadratic.
+__device__ __forceinline__ void thetaface(Walls& w, double ux, double uy, double uz, double x1, double y1, double z1,
+                                          double t1, double t2) {
+  double tmp1 = x1 * ux + y1 * uy, tmp2 = ux * ux + uy * uy, tmp3 = x1 * x1 + y1 * y1;
+  double ta, tb, aa, bb, cc, descr;
   int i1, i2;
   if (t2 < t1) { ta = t2; tb = t1; i1 = 4; i2 = 3; }
   else { ta = t1; tb = t2; i1 = 3; i2 = 4; }
-  if (ta < zero) {
-    if (uz != zero) w.insert(-z1 / uz, i1);
+  if (ta < 0.0) {
+    if (uz != 0.0) w.insert(-z1 / uz, i1);
   } else {
-    bb = two * (tmp1 - ta * uz * z1);
+    bb = 2.0 * (tmp1 - ta * uz * z1);
     aa = tmp2 - ta * uz * uz;
     cc = tmp3 - z1 * z1 * ta;
-    if (RT<T>::fabs_(aa) > eps) {
-      descr = bb * bb - four * aa * cc;
-      if (descr > zero && ta != zero) {
-        descr = RT<T>::sqrt_(descr);
-        w.insert((-bb + descr) * half / aa, i1);
-        w.insert((-bb - descr) * half / aa, i1);
+    if (fabs(aa) > 1.0e-10) {
+      descr = bb * bb - 4.0 * aa * cc;
+      if (descr > 0.0 && ta != 0.0) {
+        descr = sqrt(descr);
+        w.insert((-bb + descr) * 0.5 / aa, i1);
+        w.insert((-bb - descr) * 0.5 / aa, i1);
       }
     } else {
-      if (RT<T>::fabs_(bb) > eps) w.insert(-cc / bb, i1);
+      if (fabs(bb) > 1.0e-10) w.insert(-cc / bb, i1);
     }
   }
-  bb = two * (tmp1 - tb * uz * z1);
+  bb = 2.0 * (tmp1 - tb * uz * z1);
   aa = tmp2 - tb * uz * uz;
   cc = tmp3 - z1 * z1 * tb;
-  if (RT<T>::fabs_(aa) > eps) {
-    descr = bb * bb - four * aa * cc;
-    if (descr > zero && tb != zero) {
-      descr = RT<T>::sqrt_(descr);
-      w.insert((-bb + descr) * half / aa, i2);
-      w.insert((-bb - descr) * half / aa, i2);
+  if (fabs(aa) > 1.0e-10) {
+    descr = bb * bb - 4.0 * aa * cc;
+    if (descr > 0.0 && tb != 0.0) {
+      descr = sqrt(descr);
+      w.insert((-bb + descr) * 0.5 / aa, i2);
+      w.insert((-bb - descr) * 0.5 / aa, i2);
     }
   } else {
-    if (RT<T>::fabs_(bb) > eps) w.insert(-cc / bb, i2);
+    if (fabs(bb) > 1.0e-10) w.insert(-cc / bb, i2);
   }
 }
 
 // phiface.f90:48-56 with C = D = 0 (tauint_3d_2_fuv.f90:47)
-template <class T>
-__device__ __forceinline__ void phiface(Walls<T>& w, T A1, T B1, T A2, T B2, T ux, T uy, T uz, T x1, T y1, T z1) {
-  const T zero = (T)0;
-  T denom1 = (A1 * ux + B1 * uy + zero * uz);
-  if (denom1 != zero) w.insert(-(A1 * x1 + B1 * y1 + zero * z1 + zero) / denom1, 5);
-  T denom2 = (A2 * ux + B2 * uy + zero * uz);
-  if (denom2 != zero) w.insert(-(A2 * x1 + B2 * y1 + zero * z1 + zero) / denom2, 6);
+__device__ __forceinline__ void phiface(Walls& w, double A1, double B1, double A2, double B2, double ux, double uy,
+                                        double uz, double x1, double y1, double z1) {
+  double denom1 = (A1 * ux + B1 * uy + 0.0 * uz);
+  if (denom1 != 0.0) w.insert(-(A1 * x1 + B1 * y1 + 0.0 * z1 + 0.0) / denom1, 5);
+  double denom2 = (A2 * ux + B2 * uy + 0.0 * uz);
+  if (denom2 != 0.0) w.insert(-(A2 * x1 + B2 * y1 + 0.0 * z1 + 0.0) / denom2, 6);
 }
 
 // locate, nr_2.f:131-154
@@ -304,9 +283,15 @@ template <class T> __device__ __forceinline__ T check1(T x) {  // check.f90
 // ---------------------------------------------------------------------------
 template <class T> struct Opac { T kappa, albedo, g, pl; };
 
+// Packet geometry (position, direction of flight, face distances) is FP64 in
+// both precisions: at r ~ 1e5 R* an FP32 position carries ~1e-2 R* of
+// rounding, enough for the on-wall tie rule of closest_wall.f90:52-59 to
+// pick the wrong root near cell corners.  T is the type of everything else.
+typedef double G;
+
 template <class T> struct Pkt {
-  T xp, yp, zp, rsq, rtot;
-  T ux, uy, uz;
+  G xp, yp, zp, rsq, rtot;
+  G ux, uy, uz;
   T cost, sint, phi, cosp, sinp;
   T sip, sqp, sup, svp;
   T nub, wave;
@@ -637,82 +622,93 @@ __device__ __forceinline__ void findangle3(T cost1, T sint1, T phi1, T cosp1, T 
 }
 
 // ---------------------------------------------------------------------------
-// testgrid.f90:40-124.  Returns true when state changed.
-// ---------------------------------------------------------------------------
+// testgrid.f90:40-124, on the FP64 geometry.  Returns true when state changed.
 template <class T>
 __device__ __forceinline__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& rng) {
-  const T tol = (T)1.0e-4, one = (T)1, zero = (T)0;
-  const T PI = (T)kPi, R2P = (T)kR2p;
+  const double tol = 1.0e-4;
+  const double PI = kPi, R2P = kR2p;
   const int nrg = m.nrg, ntg = m.ntg, npg = m.npg;
-  const T* r2 = walls_r2<T>(m);
-  const T* ca = walls_cos<T>(m);
-  const T* pa = walls_phi<T>(m);
+  const double* r2 = m.r2arr;
+  const double* ca = m.costarr;
+  const double* pa = m.phiarr;
   bool reset = false, changed = false;
-  T cosb = check1(p.zp / p.rtot);
-  T phi = zero;
-  if (npg > 2) { phi = RT<T>::atan2_(p.yp, p.xp); if (phi < zero) phi = phi + R2P; }
-  if ((p.rsq - r2[0]) / (r2[1] - r2[0]) < -tol || (p.rsq - r2[nrg - 2]) / (r2[nrg - 1] - r2[nrg - 2]) > one + tol) {
-    T ra = (T)m.rarr[nrg - 1], rb = (T)m.rarr[0];
-    p.rtot = RT<T>::pow_(uniform<T>(rng) * (RT<T>::pow_(ra, (T)3) - RT<T>::pow_(rb, (T)3)) + RT<T>::pow_(rb, (T)3), (T)(1.0f / 3.0f));
+  double cosb = check1(p.zp / p.rtot);
+  double phi = 0.0;
+  // The azimuth (atan2) only decides anything when the packet is not inside
+  // its phi wedge: phi = atan2(..) lies in [0, 2pi] so the whole-grid phi
+  // test never fires, and inside the wedge pfrac is in [0, 1].  When the
+  // plane signs sin(phi_k) x - cos(phi_k) y prove the packet inside (cells
+  // narrower than pi), atan2 is skipped; every decision is unchanged.
+  bool have_phi = true;
+  if (npg > 2) {
+    have_phi = false;
+    if (npg >= 4 && p.ip >= 1 && p.ip <= npg - 1) {
+      double s1 = m.aarr[p.ip - 1] * p.xp + m.barr[p.ip - 1] * p.yp;
+      double s2 = m.aarr[p.ip] * p.xp + m.barr[p.ip] * p.yp;
+      if (!(s1 < 0.0 && s2 > 0.0)) have_phi = true;
+    } else {
+      have_phi = true;
+    }
+    if (have_phi) { phi = atan2(p.yp, p.xp); if (phi < 0.0) phi = phi + R2P; }
+  }
+  if ((p.rsq - r2[0]) / (r2[1] - r2[0]) < -tol || (p.rsq - r2[nrg - 2]) / (r2[nrg - 1] - r2[nrg - 2]) > 1.0 + tol) {
+    double ra = m.rarr[nrg - 1], rb = m.rarr[0];
+    p.rtot = pow((double)uniform<T>(rng) * (pow(ra, 3.0) - pow(rb, 3.0)) + pow(rb, 3.0), (double)(1.0f / 3.0f));
     reset = true;
   }
-  if ((cosb - ca[0]) / (ca[1] - ca[0]) < -tol || (cosb - ca[ntg - 2]) / (ca[ntg - 1] - ca[ntg - 2]) > one + tol) {
-    cosb = (uniform<T>(rng) * (ca[ntg - 1] - ca[0]) + ca[0]);
+  if ((cosb - ca[0]) / (ca[1] - ca[0]) < -tol || (cosb - ca[ntg - 2]) / (ca[ntg - 1] - ca[ntg - 2]) > 1.0 + tol) {
+    cosb = ((double)uniform<T>(rng) * (ca[ntg - 1] - ca[0]) + ca[0]);
     reset = true;
   }
-  if (npg > 2 && ((phi - pa[0]) / (pa[1] - pa[0]) < -tol || (phi - pa[npg - 2]) / (pa[npg - 1] - pa[npg - 2]) > one + tol)) {
-    phi = uniform<T>(rng) * (pa[npg - 1] - pa[0]) + pa[0];
+  if (npg > 2 && have_phi && ((phi - pa[0]) / (pa[1] - pa[0]) < -tol || (phi - pa[npg - 2]) / (pa[npg - 1] - pa[npg - 2]) > 1.0 + tol)) {
+    phi = (double)uniform<T>(rng) * (pa[npg - 1] - pa[0]) + pa[0];
     reset = true;
   }
+  if (reset && !have_phi) { phi = atan2(p.yp, p.xp); if (phi < 0.0) phi = phi + R2P; have_phi = true; }
   if (reset) {
-    T s = RT<T>::sqrt_(one - RT<T>::pow_(cosb, (T)2));
-    p.xp = p.rtot * s * RT<T>::cos_(phi);
-    p.yp = p.rtot * s * RT<T>::sin_(phi);
+    double s = sqrt(1.0 - pow(cosb, 2.0));
+    p.xp = p.rtot * s * cos(phi);
+    p.yp = p.rtot * s * sin(phi);
     p.zp = p.rtot * cosb;
     p.rsq = p.rtot * p.rtot;
     p.on_wall = false;
     changed = true;
   }
-  if (p.ir < 1 || p.ir > nrg) { p.ir = locate<T>(r2, nrg, p.rsq); changed = true; }
-  if (p.it < 1 || p.it > ntg) { p.it = locate<T>(ca, ntg, cosb); changed = true; }
-  if (p.ip < 1 || p.ip > npg) { p.ip = locate<T>(pa, npg, phi); changed = true; }
-  T rfrac = (p.rsq - r2[p.ir - 1]) / (r2[p.ir] - r2[p.ir - 1]);
-  T tfrac = (cosb - ca[p.it - 1]) / (ca[p.it] - ca[p.it - 1]);
+  if (p.ir < 1 || p.ir > nrg) { p.ir = locate<double>(r2, nrg, p.rsq); changed = true; }
+  if (p.it < 1 || p.it > ntg) { p.it = locate<double>(ca, ntg, cosb); changed = true; }
+  if (p.ip < 1 || p.ip > npg) { p.ip = locate<double>(pa, npg, phi); changed = true; }
+  double rfrac = (p.rsq - r2[p.ir - 1]) / (r2[p.ir] - r2[p.ir - 1]);
+  double tfrac = (cosb - ca[p.it - 1]) / (ca[p.it] - ca[p.it - 1]);
   bool phierr = false;
-  if (npg > 2) {
-    T pfrac = (phi - pa[p.ip - 1]) / (pa[p.ip] - pa[p.ip - 1]);
-    if (pfrac < -tol || pfrac > one + tol) {
-      T d1 = phi - pa[p.ip - 1], d2 = phi - pa[p.ip];
+  if (npg > 2 && have_phi) {
+    double pfrac = (phi - pa[p.ip - 1]) / (pa[p.ip] - pa[p.ip - 1]);
+    if (pfrac < -tol || pfrac > 1.0 + tol) {
+      double d1 = phi - pa[p.ip - 1], d2 = phi - pa[p.ip];
       if (d1 > PI) d1 -= R2P;
       if (d1 < -PI) d1 += R2P;
       if (d2 > PI) d2 -= R2P;
       if (d2 < -PI) d2 += R2P;
-      phierr = RT<T>::fabs_(d1) > tol && RT<T>::fabs_(d2) > tol;
+      phierr = fabs(d1) > tol && fabs(d2) > tol;
     }
   }
-  if (rfrac < -tol || rfrac > one + tol) { p.ir = locate<T>(r2, nrg, p.rsq); changed = true; }
-  if (tfrac < -tol || tfrac > one + tol) { p.it = locate<T>(ca, ntg, cosb); changed = true; }
-  if (phierr) { p.ip = locate<T>(pa, npg, phi); changed = true; }
+  if (rfrac < -tol || rfrac > 1.0 + tol) { p.ir = locate<double>(r2, nrg, p.rsq); changed = true; }
+  if (tfrac < -tol || tfrac > 1.0 + tol) { p.it = locate<double>(ca, ntg, cosb); changed = true; }
+  if (phierr) { p.ip = locate<double>(pa, npg, phi); changed = true; }
   return changed;
 }
 
 // one face-selection step shared by the transport and peel-off walks
 template <class T>
-__device__ __forceinline__ bool next_face(const DeviceModel& m, const Pkt<T>& p, T ux, T uy, T uz, T& dt, int& iface) {
-  Walls<T> w;
+__device__ __forceinline__ bool next_face(const DeviceModel& m, const Pkt<T>& p, double ux, double uy, double uz,
+                                          double& dt, int& iface) {
+  Walls w;
   w.reset();
-  radface<T>(w, ux, uy, uz, p.xp, p.yp, p.zp, m.r2arr[p.ir - 1], m.r2arr[p.ir]);
-  if (m.ntg > 2) {
-    const T* t2 = walls_tan2<T>(m);
-    thetaface<T>(w, ux, uy, uz, p.xp, p.yp, p.zp, t2[p.it - 1], t2[p.it]);
-  }
-  if (m.npg > 2 && m.ntg > 2) {
-    const T* A = walls_a<T>(m);
-    const T* B = walls_b<T>(m);
-    phiface<T>(w, A[p.ip - 1], B[p.ip - 1], A[p.ip], B[p.ip], ux, uy, uz, p.xp, p.yp, p.zp);
-  }
+  radface(w, ux, uy, uz, p.xp, p.yp, p.zp, m.r2arr[p.ir - 1], m.r2arr[p.ir]);
+  if (m.ntg > 2) thetaface(w, ux, uy, uz, p.xp, p.yp, p.zp, m.tan2arr[p.it - 1], m.tan2arr[p.it]);
+  if (m.npg > 2 && m.ntg > 2)
+    phiface(w, m.aarr[p.ip - 1], m.barr[p.ip - 1], m.aarr[p.ip], m.barr[p.ip], ux, uy, uz, p.xp, p.yp, p.zp);
   if (!w.next(p.on_wall, dt, iface)) return false;
-  if (dt < (T)0) return false;
+  if (dt < 0.0) return false;
   return true;
 }
 
@@ -731,8 +727,12 @@ __device__ __forceinline__ void cross_face(const DeviceModel& m, int& ir, int& i
 
 template <class T>
 __device__ __forceinline__ T cell_opac(const DeviceModel& m, const Pkt<T>& p, size_t c, int& iop, T& rho) {
-  int sp = m.species[c];
   rho = cell_rho<T>(m, c);
+  if (m.nopac == 1) {   // one opacity table: rho > 0 exactly when a species is present
+    iop = rho > (T)0 ? 0 : -1;
+    return rho > (T)0 ? (T)0 + p.kapd[0] * rho : (T)0;
+  }
+  int sp = m.species[c];
   iop = -1;
   if (sp > 0 && rho > (T)0) { iop = m.species_opac[sp]; return (T)0 + pick(p.kapd, iop) * rho; }
   return (T)0;

@@ -23,17 +23,18 @@ __global__ void __launch_bounds__(256) k_trace(DeviceModel m, TraceArgs a) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= a.n) return;
   Pkt<T> p;
-  p.xp = (T)a.pos[3 * r]; p.yp = (T)a.pos[3 * r + 1]; p.zp = (T)a.pos[3 * r + 2];
+  p.xp = a.pos[3 * r]; p.yp = a.pos[3 * r + 1]; p.zp = a.pos[3 * r + 2];
   p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-  p.rtot = RT<T>::sqrt_(p.rsq);
-  p.ux = (T)a.u[3 * r]; p.uy = (T)a.u[3 * r + 1]; p.uz = (T)a.u[3 * r + 2];
+  p.rtot = sqrt(p.rsq);
+  p.ux = a.u[3 * r]; p.uy = a.u[3 * r + 1]; p.uz = a.u[3 * r + 2];
   p.ir = a.ijk[3 * r]; p.it = a.ijk[3 * r + 1]; p.ip = a.ijk[3 * r + 2];
   p.on_wall = a.on_wall[r] != 0;
 #pragma unroll
   for (int i = 0; i < kMaxOpac; ++i) p.kapd[i] = (T)1;
   Rng rng;
-  rng_init(rng, 0, (uint64_t)r);
-  T tau_stop = (T)a.tau_stop[r], tsum = (T)0, dt;
+  rng_init(rng, 0, (uint64_t)r);      // testgrid re-placements: stream = ray index
+  T tau_stop = (T)a.tau_stop[r], tsum = (T)0;
+  G dt;
   int iface, n = 0, st = 3;
   while (n < a.max_steps) {
     testgrid<T>(m, p, rng);
@@ -42,10 +43,10 @@ __global__ void __launch_bounds__(256) k_trace(DeviceModel m, TraceArgs a) {
     int iop;
     T rho;
     T opac = cell_opac<T>(m, p, c, iop, rho);
-    T dtau = opac * dt;
+    T dtau = opac * (T)dt;
     if ((tsum + dtau) > tau_stop) {
       T ds = (tau_stop - tsum) / opac;
-      p.xp = p.xp + ds * p.ux; p.yp = p.yp + ds * p.uy; p.zp = p.zp + ds * p.uz;
+      p.xp = p.xp + (G)ds * p.ux; p.yp = p.yp + (G)ds * p.uy; p.zp = p.zp + (G)ds * p.uz;
       tsum = tau_stop;
       st = 0;
       break;
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(256) k_trace(DeviceModel m, TraceArgs a) {
     p.xp = p.xp + dt * p.ux; p.yp = p.yp + dt * p.uy; p.zp = p.zp + dt * p.uz;
     p.on_wall = true;
     p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-    p.rtot = RT<T>::sqrt_(p.rsq);
+    p.rtot = sqrt(p.rsq);
     tsum = tsum + dtau;
     size_t o = (size_t)r * a.max_steps + n;
     a.faces[o] = iface;

@@ -14,7 +14,7 @@ namespace mcrt {
 // cosb, sinb, cos(lp), sin(lp) and the weight is formed per direction.
 template <class T>
 struct alignas(16) PeelRec {
-  T xp, yp, zp, rsq, rtot;
+  G xp, yp, zp, rsq, rtot;
   T sip, sqp, sup, svp;
   T cost, sint, phi, cosp, sinp;
   T nub, wave, g, pl;
@@ -104,9 +104,9 @@ __device__ __forceinline__ void initp_transform(Pkt<T>& p) {
 template <class T>
 __device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
   p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
-  p.cosb = p.zp / p.rtot;
+  p.cosb = (T)(p.zp / p.rtot);
   p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
-  p.lp = RT<T>::atan2_(p.yp, p.xp);
+  p.lp = RT<T>::atan2_((T)p.yp, (T)p.xp);
   initp_sample(C, p);
   initp_transform(p);
   p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
@@ -117,7 +117,8 @@ __device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
 template <class T>
 __device__ __forceinline__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
   const DeviceModel& m = C.m;
-  T tsum = (T)0, dt;
+  T tsum = (T)0;
+  G dt;
   int iface;
   long long nseg = 0;
   for (;;) {
@@ -129,15 +130,15 @@ __device__ __forceinline__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
     int iop;
     T rho;
     T opac = cell_opac<T>(m, p, c, iop, rho);
-    T dtau = opac * dt;
+    T dtau = opac * (T)dt;
     if ((tsum + dtau) > tau) {
       T ds = (tau - tsum) / opac;
       if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
-      p.xp = p.xp + ds * p.ux;
-      p.yp = p.yp + ds * p.uy;
-      p.zp = p.zp + ds * p.uz;
+      p.xp = p.xp + (G)ds * p.ux;
+      p.yp = p.yp + (G)ds * p.uy;
+      p.zp = p.zp + (G)ds * p.uz;
       p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-      p.rtot = RT<T>::sqrt_(p.rsq);
+      p.rtot = sqrt(p.rsq);
       p.on_wall = false;
       return true;
     }
@@ -148,12 +149,12 @@ __device__ __forceinline__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
     p.zp = p.zp + dt * p.uz;
     p.on_wall = true;
     p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-    p.rtot = RT<T>::sqrt_(p.rsq);
+    p.rtot = sqrt(p.rsq);
     tsum = tsum + dtau;
     if (p.ir == m.nrg) { p.exitflag = 1; return true; }
     if (p.ir == 0) {
       C.cnt.c[C_AFLUX]++;
-      p.rtot = RT<T>::sqrt_(p.rsq);
+      p.rtot = sqrt(p.rsq);
       initpabs(C, p);
       opacset<T>(m, p.nub, p.op);
       p.wave = (T)1.2398 / p.nub;
@@ -187,16 +188,17 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
   const DeviceTallies& t = C.t;
   Pkt<T> p = q;
   T s[4] = {sipnew, q.sqp, q.sup, q.svp};
-  T tsum = (T)0, dt;
+  T tsum = (T)0;
+  G dt;
   int iface;
   const int k = peelid - 1;
   const double* pe = m.peel + 5 * k;
   const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
-  const T ux1 = sinte * cospe, uy1 = sinte * sinpe, uz1 = coste;
-  const T* r2 = walls_r2<T>(m);
+  const G ux1 = pe[1] * pe[2], uy1 = pe[1] * pe[3], uz1 = pe[0];
+  const double* r2 = m.r2arr;
   if (p.on_wall) {
-    bool inner = RT<T>::fabs_(p.rsq - r2[p.ir - 1]) < RT<T>::fabs_(p.rsq - r2[p.ir]);
-    bool outwards = ux1 * p.xp + uy1 * p.yp + uz1 * p.zp > (T)0;
+    bool inner = fabs(p.rsq - r2[p.ir - 1]) < fabs(p.rsq - r2[p.ir]);
+    bool outwards = ux1 * p.xp + uy1 * p.yp + uz1 * p.zp > 0.0;
     if (inner && !outwards) { if (p.ir == 1) return; p.ir = p.ir - 1; }
     if (!inner && outwards) p.ir = p.ir + 1;
   }
@@ -209,7 +211,7 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
     int iop;
     T rho;
     T opac = cell_opac<T>(m, p, c, iop, rho);
-    T dtau = opac * dt;
+    T dtau = opac * (T)dt;
     tsum = tsum + dtau;
     if (tsum > (T)m.taumax) return;
     cross_face(m, p.ir, p.it, p.ip, iface);
@@ -218,7 +220,7 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
     p.zp = p.zp + dt * uz1;
     p.on_wall = true;
     p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-    p.rtot = RT<T>::sqrt_(p.rsq);
+    p.rtot = sqrt(p.rsq);
     if (p.ir == m.nrg) break;
     if (p.ir == 0) return;
   }
@@ -230,10 +232,10 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
     findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
     stokespeel<T>(gsc, plsc, s, coste, sinte, ri1, bmu, q.cost, q.sint);
   }
-  T yimage = p.zp * sinte - p.yp * coste * sinpe - p.xp * coste * cospe;
-  T ximage = p.yp * cospe - p.xp * sinpe;
-  T rmaxi = (T)m.rmaxi;
-  T fractxh = (T)2 * rmaxi / (T)m.nx;
+  G yimage = p.zp * pe[1] - p.yp * pe[0] * pe[3] - p.xp * pe[0] * pe[2];
+  G ximage = p.yp * pe[2] - p.xp * pe[3];
+  G rmaxi = m.rmaxi;
+  G fractxh = 2.0 * rmaxi / (G)m.nx;
   int ix = (int)((ximage + rmaxi) / fractxh) + 1;
   int iy = (int)((yimage + rmaxi) / fractxh) + 1;
   T freq = (T)2.9979e14 / q.wave;
@@ -259,13 +261,13 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
       }
     }
   }
-  T rho2 = ximage * ximage + yimage * yimage;
+  G rho2 = ximage * ximage + yimage * yimage;
   int inub = freq_bin<T>(m, q.nub);
   int i_inter = (q.iflag == 1) ? 2 : ((q.i_orig == 1) ? 1 : 3);
   int indx[3] = {1, 1 + q.i_orig, 5 + i_inter};
   size_t nb = (size_t)m.nfreq * m.npeel * m.nap * kNo;
   for (int ia = 1; ia <= m.nap; ++ia) {
-    if (rho2 < (T)m.aperture2[ia - 1]) {
+    if (rho2 < m.aperture2[ia - 1]) {
       for (int j = 0; j < 3; ++j) {
         size_t o = (size_t)(inub - 1) + (size_t)m.nfreq * ((size_t)k + (size_t)m.npeel * ((size_t)(ia - 1) + (size_t)m.nap * (size_t)(indx[j] - 1)));
         atomicAdd(&t.psed[0 * nb + o], (float)s[0]);
@@ -370,7 +372,7 @@ __device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
   const T R2P = (T)kR2p;
-  T xpold = p.xp, ypold = p.yp, zpold = p.zp;
+  G xpold = p.xp, ypold = p.yp, zpold = p.zp;
   int iscat = 0, iint = 0, idust2;
   nint = 0;
   p.iflag = 0; p.exitflag = 0;
@@ -433,10 +435,10 @@ __device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
   if (itb > m.nmu) itb = m.nmu;
   if (ipb < 1) ipb = 1;
   if (ipb > m.nph) ipb = m.nph;
-  T x = ypold * p.cosp - xpold * p.sinp;
-  T y = zpold * (T)m.sini_mu[itb - 1] - ypold * (T)m.u_mu[itb - 1] * p.sinp - xpold * (T)m.u_mu[itb - 1] * p.cosp;
-  T rho2 = (x * x + y * y);
-  if (rho2 < (T)m.aperture2[m.nap - 1] * (T)1.0001) {
+  G x = ypold * (G)p.cosp - xpold * (G)p.sinp;
+  G y = zpold * m.sini_mu[itb - 1] - ypold * m.u_mu[itb - 1] * (G)p.sinp - xpold * m.u_mu[itb - 1] * (G)p.cosp;
+  G rho2 = (x * x + y * y);
+  if (rho2 < m.aperture2[m.nap - 1] * 1.0001) {
     C.cnt.c[C_FLUX]++;
     if (p.iflag == 1) { C.cnt.c[C_SFLUX]++; C.cnt.c[C_NSCAT] += (unsigned long long)iscat; }
   }
@@ -445,7 +447,7 @@ __device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
   int indx[3] = {1, 1 + p.i_orig, 5 + i_inter};
   size_t nb = (size_t)m.nfreq * m.nmu * m.nph * m.nap * kNo;
   for (int ia = 1; ia <= m.nap; ++ia) {
-    if (rho2 < (T)m.aperture2[ia - 1]) {
+    if (rho2 < m.aperture2[ia - 1]) {
       for (int j = 0; j < 3; ++j) {
         size_t o = (size_t)(inub - 1) + (size_t)m.nfreq * ((size_t)(itb - 1) + (size_t)m.nmu * ((size_t)(ipb - 1) + (size_t)m.nph * ((size_t)(ia - 1) + (size_t)m.nap * (size_t)(indx[j] - 1))));
         atomicAdd(&t.sed[0 * nb + o], (float)p.sip);
@@ -488,17 +490,17 @@ __device__ __forceinline__ void one_packet(Ctx<T>& C, uint64_t pid) {
   p.wave = (T)1.2398 / p.nub;
   set_kapd(p, m, 0);
   T coslp = RT<T>::cos_(p.lp), sinlp = RT<T>::sin_(p.lp);
-  p.zp = p.cosb * (T)m.rmin;
-  T rp = p.sinb * (T)m.rmin;
-  p.xp = rp * coslp;
-  p.yp = rp * sinlp;
+  p.zp = (G)p.cosb * m.rmin;
+  G rp = (G)p.sinb * m.rmin;
+  p.xp = rp * (G)coslp;
+  p.yp = rp * (G)sinlp;
   p.rsq = p.zp * p.zp + rp * rp;
-  p.rtot = RT<T>::sqrt_(p.rsq);
+  p.rtot = sqrt(p.rsq);
   p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
   p.ir = 1;
   p.on_wall = true;
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
-  p.ip = locate<T>(walls_phi<T>(m), m.npg, p.lp);
+  p.ip = locate<double>(m.phiarr, m.npg, (double)p.lp);
   if (m.ipeel == 1) peel_direct<T>(C, p, coslp, sinlp);
   int nint = 0;
   int code = propagate<T>(C, p, nint);
