@@ -31,14 +31,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(c, npackets):
-    """Bytes the packet path must move per launch:
-    transport segment = density (4 B) + species (1 B) + FP64 dtauabs RMW (16 B);
-    peel segment = density + species (5 B); peel event reaching the observer =
-    peeled SED 3 components x 9 floats RMW (216 B) + image I,Q,U,V,origin RMW
-    (40 B); escaped packet = SED 3 components x 10 floats RMW (240 B)."""
-    return (c["segments"] * 21.0 + c["peel_segments"] * 5.0 + c["peel_events"] * 256.0
-            + c["escaped"] * 240.0)
+def kernel_bytes(c, st, rec_bytes, lane_bytes):
+    """Algorithmic HBM bytes of the two kernels of a run.
+
+    transport: per face crossing the FP32 density (4 B) + species (1 B) +
+    FP64 dtauabs read-modify-write (16 B); per escaped packet its SED row,
+    3 components x 10 floats read-modify-write (240 B); per queued peel-off
+    record its write (rec_bytes); per parked lane-round the lane state
+    written and read back (2 x lane_bytes).
+    peel: per face crossing density + species (5 B); per walk that reaches
+    the observer 3 peeled-SED components x 9 floats (216 B) + image I,Q,U,V,
+    origin (40 B) read-modify-write; per record one read (rec_bytes)."""
+    transport = (c["segments"] * 21.0 + c["escaped"] * 240.0 + st["records"] * rec_bytes
+                 + st["parked"] * 2.0 * lane_bytes)
+    peel = c["peel_segments"] * 5.0 + c["peel_events"] * 256.0 + st["records"] * rec_bytes
+    return transport, peel
 
 
 def sed_error(a, b):
@@ -113,7 +120,7 @@ def main():
     log(f"packets per step: {N}")
 
     seed = 20250101
-    step_ms, kern_ms, counters = [], [], []
+    step_ms, kern_ms, counters, stats = [], [], [], []
     first = 0
     for s in range(args.warmup + args.steps):
         eng.synchronize()
@@ -130,6 +137,7 @@ def main():
             step_ms.append(dt)
             kern_ms.append(km)
             counters.append(tal["counters"])
+            stats.append(eng.run_stats())
         log(f"step {s} ({'warm-up' if s < args.warmup else 'timed'}): {dt:.0f} ms, kernel {km:.0f} ms, "
             f"{N/(dt/1e3):.4g} packets/s, segments/packet {tal['counters']['segments']/N:.0f}, "
             f"peel segments/packet {tal['counters']['peel_segments']/N:.0f}, errors {tal['counters']['errors']:.0f}")
@@ -137,8 +145,17 @@ def main():
     value = N / (ms / 1e3)
     kmean = float(np.mean(kern_ms))
     c = {k: float(np.mean([cc[k] for cc in counters])) for k in counters[0]}
-    B = algorithmic_bytes(c, N)
-    achieved = B / (kmean / 1e3) / 1e9
+    st = {k: float(np.mean([ss[k] for ss in stats])) for k in stats[0]}
+    b_tr, b_pe = kernel_bytes(c, st, st["record_bytes"], st["lane_bytes"])
+    # the dominant kernel's roofline: algorithmic bytes over its summed
+    # HIP-event time (every launch of the step bracketed on the run stream)
+    if st["peel_ms"] >= st["transport_ms"]:
+        kname, kms, kB, nl = "k_peel<float>", st["peel_ms"], b_pe, st["peel_launches"]
+    else:
+        kname, kms, kB, nl = "k_transport<float>", st["transport_ms"], b_tr, st["rounds"]
+    achieved = kB / (kms / 1e3) / 1e9
+    log(f"kernels per step: transport {st['transport_ms']:.1f} ms ({b_tr/1e9:.2f} GB algorithmic), "
+        f"peel {st['peel_ms']:.1f} ms ({b_pe/1e9:.2f} GB), rounds {st['rounds']:.0f}")
 
     # FP64 path and the mixed path's error against it on identical counters
     tdust_state = None
@@ -188,12 +205,15 @@ def main():
         "config": {"workload": f"{args.config}: 3-D polarized clumpy disk+Ulrich envelope, "
                                f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
                                f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
-                   "packets_per_step": N, "precision": "mixed (FP32 transport, FP64 radial roots and tallies)",
+                   "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; FP64 geometry and dtauabs)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_packets<float>", "kernel_ms": kmean,
-                     "algorithmic_bytes_per_launch": B},
+                     "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
+                     "kernel_ms_per_launch": kms / max(nl, 1.0),
+                     "algorithmic_bytes_per_launch": kB / max(nl, 1.0),
+                     "transport_ms_per_step": st["transport_ms"], "peel_ms_per_step": st["peel_ms"],
+                     "run_ms_per_step": kmean},
         "cpu_baseline": cpu,
     }
     out.update(extra)

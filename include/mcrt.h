@@ -104,12 +104,17 @@ int mcrt_run_packets(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
                      uint64_t seed, int32_t precision);
 
 /* Launch configuration of mcrt_run_packets (no counterpart in the reference,
- * whose rank loop has none): packets per batch, peel-off queue slots per
- * packet of a batch (-1 walks every peel-off inline in the transport kernel),
- * and resident blocks per CU for the transport and peel kernels.  0 keeps
- * the built-in default.  Results do not depend on these values. */
-int mcrt_set_launch(mcrt_ctx *ctx, int64_t batch_packets, int32_t queue_per_packet,
-                    int32_t transport_blocks_per_cu, int32_t peel_blocks_per_cu);
+ * whose rank loop has none).  lanes: resident packet slots of the transport
+ * kernel (a multiple of 256).  queue_per_lane: peel-off queue records per
+ * lane (at least 8).  events_per_round: interaction events per lane between
+ * two peel-kernel passes (at most queue_per_lane - 7, so the queue cannot
+ * overflow).  peel_blocks_per_cu: peel-kernel blocks per CU.  regroup: lanes
+ * of a wavefront (1..64) that must be waiting for an interaction, an
+ * emission or a peel-off tally before that code runs while other lanes still
+ * walk.  0 keeps the built-in default.  Results do not depend on these
+ * values. */
+int mcrt_set_launch(mcrt_ctx *ctx, int64_t lanes, int32_t queue_per_lane,
+                    int32_t events_per_round, int32_t peel_blocks_per_cu, int32_t regroup);
 
 /* Post-ALLREDUCE tallies (newdisktherm_5_fuv.f90:524-602): any pointer may be
  * NULL.  Layouts: dtauabs/nabs/dbig/killed per cell; sed
@@ -155,6 +160,25 @@ int mcrt_get_trace(mcrt_ctx *ctx, double *trace, int64_t n);
 /* Duration in ms of the last mcrt_run_packets launch, from HIP events
  * recorded on the context stream (synchronizes). */
 int mcrt_last_run_ms(mcrt_ctx *ctx, float *ms);
+
+/* Measurement of the last mcrt_run_packets (no counterpart in the
+ * reference): out[MCRT_NSTATS], indexed by MCRT_STAT_*.  Kernel times are
+ * sums over rounds of HIP-event intervals bracketing each launch on the
+ * context stream. */
+enum {
+  MCRT_STAT_ROUNDS = 0,        /* transport rounds */
+  MCRT_STAT_TRANSPORT_MS = 1,  /* summed transport-kernel time */
+  MCRT_STAT_PEEL_MS = 2,       /* summed peel-kernel time */
+  MCRT_STAT_PEEL_LAUNCHES = 3, /* peel-kernel launches */
+  MCRT_STAT_RECORDS = 4,       /* peel-off records queued (each walked in npeel directions) */
+  MCRT_STAT_PARKED = 5,        /* lane-rounds ending with a parked packet */
+  MCRT_STAT_LANES = 6,         /* resident lanes */
+  MCRT_STAT_EVENTS0 = 7,       /* events per lane in the first round */
+  MCRT_STAT_RECORD_BYTES = 8,  /* bytes of one peel-off record */
+  MCRT_STAT_LANE_BYTES = 9,    /* bytes of one parked lane */
+  MCRT_NSTATS = 10
+};
+int mcrt_run_stats(mcrt_ctx *ctx, double *out);
 
 #ifdef __cplusplus
 }

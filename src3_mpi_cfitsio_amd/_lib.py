@@ -37,7 +37,7 @@ SIGNATURES = {
     "mcrt_set_tdust": (C.c_int, [_vp, _dp]),
     "mcrt_zero_tallies": (C.c_int, [_vp]),
     "mcrt_run_packets": (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int32]),
-    "mcrt_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    "mcrt_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mcrt_get_tallies": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _fp, _fp, _fp, _dp]),
     "mcrt_dusttemp": (C.c_int, [_vp, C.c_double, C.c_int32, C.c_int32, _dp]),
     "mcrt_trace_rays": (C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp, _ip, _u8p, _dp, C.c_int32,
@@ -46,6 +46,7 @@ SIGNATURES = {
     "mcrt_enable_trace": (C.c_int, [_vp, C.c_int64]),
     "mcrt_get_trace": (C.c_int, [_vp, _dp, C.c_int64]),
     "mcrt_last_run_ms": (C.c_int, [_vp, _fp]),
+    "mcrt_run_stats": (C.c_int, [_vp, _dp]),
 }
 
 _lib = None

@@ -1,15 +1,17 @@
-// kernels_mixed.hip -- mixed-precision instantiation: FP32 transport,
-// sampling and Stokes algebra; FP64 radial-shell roots (radface) and FP64
-// absorbed-energy atomics (dtauabs).
+// kernels_mixed.hip -- mixed-precision instantiation: FP32 opacities,
+// optical depths, sampling and Stokes algebra; FP64 packet geometry (face
+// roots, positions) and FP64 absorbed-energy atomics (dtauabs).
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
 
 namespace mcrt {
 
-hipError_t launch_packets_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_packets<float>, dim3(grid), dim3(256), 0, s, m, t, rp);
+hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
+  hipLaunchKernelGGL(k_transport<float>, dim3((unsigned)(rp.lanes / 256)), dim3(256), 0, s, m, t, rp);
   return hipGetLastError();
 }
+
+int lane_words_mixed() { return lane_words<float>(); }
 
 hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_peel<float>, dim3(grid), dim3(256), 0, s, m, t, rp);

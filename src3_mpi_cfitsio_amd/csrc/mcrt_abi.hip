@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -29,11 +30,14 @@ struct mcrt_ctx {
   bool ev_valid = false;
   int cu_count = 256;
   // launch configuration (mcrt_set_launch); 0 = built-in default
-  int64_t batch_packets = 0;
-  int64_t queue_per_packet = 0;
-  bool inline_peel = false;
-  int transport_blocks_per_cu = 0;
+  int64_t lanes = 0;
+  int64_t queue_per_lane = 0;
+  int events_per_round = 0;
   int peel_blocks_per_cu = 0;
+  int regroup = 0;
+  RunCtl* host_ctl = nullptr;             // pinned read-back of the run control block
+  hipEvent_t evr[3] = {nullptr, nullptr, nullptr};   // per-round kernel bracketing
+  double stats[MCRT_NSTATS] = {0};        // mcrt_run_stats of the last run
 };
 
 namespace {
@@ -164,6 +168,12 @@ int mcrt_create(int device, mcrt_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0) c->cu_count = prop.multiProcessorCount;
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
+  for (auto& e : c->evr) (void)hipEventCreate(&e);
+  if (hipHostMalloc((void**)&c->host_ctl, sizeof(RunCtl), hipHostMallocDefault) != hipSuccess) {
+    c->host_ctl = nullptr;
+    mcrt_destroy(c);
+    return MCRT_E_HIP;
+  }
   c->dm.max_segments = 1000000;
   c->dm.iintmax = 10000;
   c->dm.taumax = 25.0;
@@ -184,6 +194,8 @@ int mcrt_destroy(mcrt_ctx* c) {
   for (auto& kv : c->buf) (void)hipFree(kv.second);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
+  if (c->host_ctl) (void)hipHostFree(c->host_ctl);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return MCRT_OK;
@@ -211,9 +223,25 @@ int mcrt_set_grid(mcrt_ctx* c, int32_t nrg, int32_t ntg, int32_t npg, const doub
   if (nrg < 3 || ntg < 2 || npg < 2 || !rarr || !r2arr || !thetarr || !costarr || !tan2arr || !phiarr || !aarr || !barr || !vcell)
     return fail(c, MCRT_E_ARG, "mcrt_set_grid: need nrg>=3, ntg>=2, npg>=2 and all arrays");
   if (ntg > 2 && (ntg % 2) == 0) return fail(c, MCRT_E_ARG, "mcrt_set_grid: ntg must be odd (equator wall)");
+  if (nrg > 1023 || ntg > 1023 || npg > 1023) return fail(c, MCRT_E_ARG, "mcrt_set_grid: at most 1023 walls per axis");
+  double dphi_min = 2.0 * M_PI;
+  if (npg > 2) {
+    // testgrid takes phi = atan2 in [0, 2pi]: the walls must span exactly that
+    if (std::fabs(phiarr[0]) > 1e-12 || std::fabs(phiarr[npg - 1] - 2.0 * M_PI) > 1e-9)
+      return fail(c, MCRT_E_ARG, "mcrt_set_grid: phi walls must run from 0 to 2*pi");
+    for (int k = 1; k < npg; ++k) {
+      const double d = phiarr[k] - phiarr[k - 1];
+      if (!(d > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_grid: phi walls must increase");
+      if (d < dphi_min) dphi_min = d;
+    }
+  }
   HIPCHECK(c, hipSetDevice(c->device));
   DeviceModel& m = c->dm;
   m.nrg = nrg; m.ntg = ntg; m.npg = npg;
+  {
+    const double e = std::sin(0.5 * 1.0e-4 * dphi_min);
+    m.phi_tol2 = e * e;
+  }
   size_t ncell = (size_t)(nrg - 1) * (ntg - 1) * (npg - 1);
   int rc = 0;
   rc |= upload(c, "rarr", rarr, nrg, &m.rarr);
@@ -252,8 +280,8 @@ int mcrt_set_medium(mcrt_ctx* c, const double* rho, const uint8_t* species, cons
   rc |= upload(c, "rho64", rho, c->ncell, &m.rho64);
   rc |= upload_cast<float>(c, "rho32", rho, c->ncell, &m.rho32);
   rc |= upload(c, "species", species, c->ncell, &m.species);
+  rc |= upload(c, "species_opac", (const int*)species_opac, 11, &m.species_opac);
   if (rc) return rc;
-  for (int i = 0; i <= 10; ++i) m.species_opac[i] = species_opac[i];
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->has_medium = true;
   return MCRT_OK;
@@ -374,61 +402,132 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   DeviceTallies t = c->dt;
   t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
   const bool fp64 = (precision == MCRT_FP64);
-  // packets run in batches; each batch's peel-off events are queued by the
-  // transport kernel and walked by the peel kernel right after it
-  const int64_t batch = c->batch_packets > 0 ? c->batch_packets : (int64_t)1 << 20;
-  const int64_t per = c->queue_per_packet > 0 ? c->queue_per_packet : 32;
-  PeelQueue q{nullptr, nullptr, 0};
-  if (c->dm.ipeel == 1 && c->dm.npeel > 0 && per > 0 && !c->inline_peel) {
+  // Resident lanes: four waves per SIMD on every CU (never more than the
+  // packets need).  Each round, every lane runs up to `events` events and
+  // queues at most one peel-off record per event; the peel kernel then walks
+  // the round's records.  Packets alive at the end of a round stay parked in
+  // their lane.  Once no new packets are left, the few parked ones get the
+  // whole queue's worth of events per round, so the longest packet's chain of
+  // interactions runs in a handful of rounds.
+  int64_t lanes = c->lanes > 0 ? c->lanes : (int64_t)c->cu_count * 4 * 4 * 64;
+  const int64_t need = ((npackets + 255) / 256) * 256;
+  if (lanes > need) lanes = need;
+  lanes = ((lanes + 255) / 256) * 256;
+  const bool queued = c->dm.ipeel == 1 && c->dm.npeel > 0;
+  // a lane fills at most `events` records of the kSlotChunk-slot chunks it
+  // reserves, so events <= capacity / lanes - (kSlotChunk - 1) keeps every
+  // record of a round inside the queue
+  const int64_t slack = (int64_t)kSlotChunk - 1;
+  int64_t per = c->queue_per_lane > 0 ? c->queue_per_lane : 40;
+  if (per < slack + 1) per = slack + 1;
+  int64_t capacity = queued ? lanes * per : 0;
+  if (capacity > ((int64_t)1 << 30)) capacity = ((int64_t)1 << 30) / lanes * lanes;
+  const int64_t share = queued ? capacity / lanes - slack : ((int64_t)1 << 20);
+  int64_t events = c->events_per_round > 0 ? c->events_per_round : (queued ? share : 64);
+  if (events > share) events = share;
+  if (events < 1) events = 1;
+  const int words = fp64 ? lane_words_fp64() : lane_words_mixed();
+  void* p;
+  int rc;
+  if ((rc = dev_alloc(c, "run.ctl", sizeof(RunCtl), &p))) return rc;
+  RunCtl* ctl = (RunCtl*)p;
+  if ((rc = dev_alloc(c, fp64 ? "lanes.words64" : "lanes.words32", (size_t)lanes * words * 4, &p))) return rc;
+  uint32_t* lane_words = (uint32_t*)p;
+  if ((rc = dev_alloc(c, "lanes.live", (size_t)lanes, &p))) return rc;
+  uint8_t* lane_live = (uint8_t*)p;
+  PeelQueue q{nullptr, &ctl->q_count, 0};
+  if (queued) {
     const size_t rb = fp64 ? peel_record_bytes_fp64() : peel_record_bytes_mixed();
-    int64_t nb = npackets < batch ? npackets : batch;
-    int64_t capacity = nb * per;
-    if (capacity > ((int64_t)1 << 31)) capacity = (int64_t)1 << 31;
-    void* p;
-    int rc = dev_alloc(c, fp64 ? "q.recs64" : "q.recs32", (size_t)capacity * rb, &p);
-    if (rc) return rc;
+    if ((rc = dev_alloc(c, fp64 ? "q.recs64" : "q.recs32", (size_t)capacity * rb, &p))) return rc;
     q.recs = p;
-    if ((rc = dev_alloc(c, "q.count", 64, &p))) return rc;
-    q.count = (unsigned*)p;
     q.capacity = (unsigned)capacity;
   }
-  const int tgrid_cap = c->cu_count * (c->transport_blocks_per_cu > 0 ? c->transport_blocks_per_cu : 8);
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
+  double st[MCRT_NSTATS] = {0};
+  const char* venv = std::getenv("MCRT_VERBOSE");
+  const bool verbose = venv && venv[0] && venv[0] != '0';
+  st[MCRT_STAT_LANES] = (double)lanes;
+  st[MCRT_STAT_EVENTS0] = (double)events;
+  st[MCRT_STAT_RECORD_BYTES] = (double)(fp64 ? peel_record_bytes_fp64() : peel_record_bytes_mixed());
+  st[MCRT_STAT_LANE_BYTES] = 4.0 * words;
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  for (int64_t done = 0; done < npackets; done += batch) {
-    const int64_t n = (npackets - done) < batch ? (npackets - done) : batch;
+  HIPCHECK(c, launch_fill_i32((int*)ctl, sizeof(RunCtl) / 4, 0, c->stream));
+  int64_t round = 0;
+  for (;; ++round) {
+    if (round >= ((int64_t)1 << 24)) return fail(c, MCRT_E_HIP, "mcrt_run_packets: packets did not finish");
     RunParams rp;
-    rp.first_packet = (uint64_t)(first_packet + done);
-    rp.npackets = (uint64_t)n;
+    rp.first_packet = (uint64_t)first_packet;
+    rp.npackets = (uint64_t)npackets;
     rp.seed = seed;
     rp.inczeta = (const double*)c->buf["inczeta"];
     rp.q = q;
-    rp.trace_base = (uint64_t)done;
-    if (q.capacity) HIPCHECK(c, launch_fill_i32((int*)q.count, 1, 0, c->stream));
-    int64_t blocks = (n + 255) / 256;
-    int grid = (int)(blocks < tgrid_cap ? blocks : tgrid_cap);
-    hipError_t e = fp64 ? launch_packets_fp64(c->dm, t, rp, grid, c->stream) : launch_packets_mixed(c->dm, t, rp, grid, c->stream);
-    if (e != hipSuccess) return hipfail(c, e, "packet kernel launch");
-    if (q.capacity) {
+    rp.lane_words = lane_words;
+    rp.lane_live = lane_live;
+    rp.lanes = (uint64_t)lanes;
+    rp.events = (uint32_t)events;
+    rp.round = (uint32_t)(round < 0xffffffffll ? round : 0xffffffffll);
+    rp.ctl = ctl;
+    rp.regroup = (uint32_t)(c->regroup > 0 ? c->regroup : 32);
+    // q_count and live restart every round (a fill kernel, not a memset)
+    HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
+    HIPCHECK(c, hipEventRecord(c->evr[0], c->stream));
+    hipError_t e = fp64 ? launch_transport_fp64(c->dm, t, rp, c->stream) : launch_transport_mixed(c->dm, t, rp, c->stream);
+    if (e != hipSuccess) return hipfail(c, e, "transport kernel launch");
+    HIPCHECK(c, hipEventRecord(c->evr[1], c->stream));
+    if (queued) {
       e = fp64 ? launch_peel_fp64(c->dm, t, rp, pgrid, c->stream) : launch_peel_mixed(c->dm, t, rp, pgrid, c->stream);
       if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
     }
+    HIPCHECK(c, hipEventRecord(c->evr[2], c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->host_ctl, ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    const RunCtl h = *c->host_ctl;
+    float ta = 0.f, tb = 0.f;
+    HIPCHECK(c, hipEventElapsedTime(&ta, c->evr[0], c->evr[1]));
+    HIPCHECK(c, hipEventElapsedTime(&tb, c->evr[1], c->evr[2]));
+    st[MCRT_STAT_TRANSPORT_MS] += ta;
+    st[MCRT_STAT_PEEL_MS] += tb;
+    st[MCRT_STAT_RECORDS] += (double)(h.q_count < (unsigned)capacity ? h.q_count : (unsigned)capacity);
+    st[MCRT_STAT_PARKED] += (double)h.live;
+    if (queued) st[MCRT_STAT_PEEL_LAUNCHES] += 1.0;
+    if (verbose)
+      std::fprintf(stderr, "[mcrt] round %lld: transport %.3f ms, peel %.3f ms, records %u, parked %u, events %lld, handed out %llu/%lld\n",
+                   (long long)round, ta, tb, h.q_count, h.live, (long long)events, h.next_packet, (long long)npackets);
+    if (h.next_packet < (unsigned long long)npackets) continue;
+    if (h.live == 0) break;
+    if (c->events_per_round <= 0) {
+      // at most 2^20 events per lane per launch keeps every launch short
+      const int64_t grow = queued ? capacity / (int64_t)h.live - slack : ((int64_t)1 << 20);
+      // (h.live <= lanes, so grow >= share >= events)
+      if (grow > events) events = grow < ((int64_t)1 << 20) ? grow : ((int64_t)1 << 20);
+    }
   }
+  st[MCRT_STAT_ROUNDS] = (double)(round + 1);
+  std::memcpy(c->stats, st, sizeof(st));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->ev_valid = true;
   return MCRT_OK;
 }
 
-int mcrt_set_launch(mcrt_ctx* c, int64_t batch_packets, int32_t queue_per_packet, int32_t transport_blocks_per_cu,
-                    int32_t peel_blocks_per_cu) {
+int mcrt_set_launch(mcrt_ctx* c, int64_t lanes, int32_t queue_per_lane, int32_t events_per_round,
+                    int32_t peel_blocks_per_cu, int32_t regroup) {
   if (!c) return MCRT_E_ARG;
-  if (batch_packets < 0 || queue_per_packet < -1 || transport_blocks_per_cu < 0 || peel_blocks_per_cu < 0)
+  if (lanes < 0 || queue_per_lane < 0 || events_per_round < 0 || peel_blocks_per_cu < 0 || regroup < 0)
     return fail(c, MCRT_E_ARG, "mcrt_set_launch: negative value");
-  c->batch_packets = batch_packets;
-  c->inline_peel = (queue_per_packet == -1);
-  c->queue_per_packet = queue_per_packet > 0 ? queue_per_packet : 0;
-  c->transport_blocks_per_cu = transport_blocks_per_cu;
+  if (regroup > 64) return fail(c, MCRT_E_ARG, "mcrt_set_launch: regroup is at most 64 (lanes of a wavefront)");
+  c->regroup = regroup;
+  if (lanes % 256 != 0) return fail(c, MCRT_E_ARG, "mcrt_set_launch: lanes must be a multiple of 256");
+  c->lanes = lanes;
+  c->queue_per_lane = queue_per_lane;
+  c->events_per_round = events_per_round;
   c->peel_blocks_per_cu = peel_blocks_per_cu;
+  return MCRT_OK;
+}
+
+int mcrt_run_stats(mcrt_ctx* c, double* out) {
+  if (!c || !out) return MCRT_E_ARG;
+  if (!c->ev_valid) return fail(c, MCRT_E_STATE, "mcrt_run_stats: no run recorded");
+  std::memcpy(out, c->stats, sizeof(c->stats));
   return MCRT_OK;
 }
 

@@ -2,8 +2,9 @@
 //
 // Templated on the transport real type T: T=double is the FP64 path
 // (compiled with -ffp-contract=off so it reproduces the reference arithmetic
-// bit for bit), T=float is the mixed-precision path (FP32 transport, sampling
-// and Stokes; FP64 radial-shell roots, FP64 absorbed-energy tally).
+// bit for bit), T=float is the mixed-precision path: FP32 opacities, optical
+// depths, sampling, Stokes algebra and peel weights; FP64 packet geometry
+// (position, direction, all face roots) and FP64 absorbed-energy tally.
 //
 // Reference routines restated here (paths relative to src3_mpi_cfitsio/):
 //   closest_wall.f90, radface.f90, thetaface.f90, phiface.f90, testgrid.f90,
@@ -36,12 +37,14 @@ struct DeviceModel {
   // grid walls, 1-based reference semantics stored 0-based (wall k at [k-1])
   int nrg, ntg, npg;
   const double *rarr, *r2arr, *thetarr, *costarr, *tan2arr, *phiarr, *aarr, *barr;
+  double phi_tol2;               // sin^2 of half testgrid's tolerance on the narrowest phi wedge
   const double* vcell;           // cm^3 per cell
   // medium: one thermal species per cell
   const double* rho64;
   const float* rho32;
   const uint8_t* species;
-  int species_opac[11];
+  const int* species_opac;        // [11] in device memory (a runtime-indexed array
+                                 // inside the kernel argument would be copied to scratch)
   // dust
   int nopac, nlam;
   const double *nudust, *kappad, *adust, *gdust, *pldust;
@@ -105,6 +108,18 @@ __device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet)
   r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32);
   r.p0 = (uint32_t)packet; r.p1 = (uint32_t)(packet >> 32);
   r.n = 0;
+}
+
+// Continue a stream at draw n (the block holding draw n is regenerated when
+// n is not at a block boundary; rng_word regenerates it otherwise).
+__device__ __forceinline__ void rng_resume(Rng& r, uint64_t seed, uint64_t packet, uint32_t n) {
+  rng_init(r, seed, packet);
+  r.n = n;
+  if (n & 3u) {
+    uint32_t o[4];
+    philox(n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
+    r.b0 = o[0]; r.b1 = o[1]; r.b2 = o[2]; r.b3 = o[3];
+  }
 }
 
 __device__ __forceinline__ uint32_t rng_word(Rng& r) {
@@ -314,52 +329,58 @@ __device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
   return v;
 }
 
-// Per-thread accumulators flushed once per packet batch.
+// Event counters of a workgroup, held in LDS (no registers per lane) and
+// flushed to the device counters once per launch.
 struct LocalCounts {
-  unsigned long long c[C_NCOUNTERS];
-  __device__ void zero() {
-#pragma unroll
-    for (int i = 0; i < C_NCOUNTERS; ++i) c[i] = 0ull;
-  }
+  unsigned long long* s;    // C_NCOUNTERS LDS slots of this workgroup
+  __device__ __forceinline__ void add(int i, unsigned long long v = 1ull) const { atomicAdd(&s[i], v); }
 };
 
 template <class T>
 __device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int order) {
 #pragma unroll
   for (int id = 0; id < kMaxOpac; ++id) {
-    if (id >= m.nopac) break;
+    if (id >= m.nopac) continue;
     if (order == 0) p.kapd[id] = (T)(p.op[id].kappa * (T)(m.rstar * kRsol) * (T)m.kappav[id]);
     else if (order == 1) p.kapd[id] = (T)(p.op[id].kappa * (T)m.rstar * (T)kRsol * (T)m.kappav[id]);
     else p.kapd[id] = (T)(p.op[id].kappa * (T)kRsol * (T)m.rstar * (T)m.kappav[id]);
   }
 }
 
-// dust_5.f90:9-109 (dust species, no gas mixing)
+// dust_5.f90:9-109 (dust species, no gas mixing) for opacity table id
+template <class T>
+__device__ __forceinline__ Opac<T> opac_one(const DeviceModel& m, T nu, int id) {
+  const double* nd = m.nudust + (size_t)id * m.nlam;
+  const double *kd = m.kappad + (size_t)id * m.nlam, *ad = m.adust + (size_t)id * m.nlam;
+  const double *gd = m.gdust + (size_t)id * m.nlam, *pd = m.pldust + (size_t)id * m.nlam;
+  const int nl = m.nlam;
+  Opac<T> o;
+  if (nu >= (T)nd[0]) { o.g = (T)gd[0]; o.pl = (T)pd[0]; o.kappa = (T)kd[0]; o.albedo = (T)ad[0]; }
+  else if (nu <= (T)nd[nl - 1]) { o.g = (T)gd[nl - 1]; o.pl = (T)pd[nl - 1]; o.kappa = (T)kd[nl - 1]; o.albedo = (T)ad[nl - 1]; }
+  else {
+    int i0 = 1, i1 = nl, i;
+    while (i1 - i0 > 1) {
+      i = (i1 + i0) / 2;
+      if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
+    }
+    T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
+    T fm1 = (T)1 - f;
+    o.g = fm1 * (T)gd[i1 - 1] + f * (T)gd[i0 - 1];
+    o.pl = fm1 * (T)pd[i1 - 1] + f * (T)pd[i0 - 1];
+    o.kappa = fm1 * (T)kd[i1 - 1] + f * (T)kd[i0 - 1];
+    o.albedo = fm1 * (T)ad[i1 - 1] + f * (T)ad[i0 - 1];
+  }
+  return o;
+}
+
+// every opacity table at frequency nu (constant indices: o stays in registers)
 template <class T>
 __device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T> (&o)[kMaxOpac]) {
-#pragma unroll
-  for (int id = 0; id < kMaxOpac; ++id) {
-    if (id >= m.nopac) break;
-    const double* nd = m.nudust + (size_t)id * m.nlam;
-    const double *kd = m.kappad + (size_t)id * m.nlam, *ad = m.adust + (size_t)id * m.nlam;
-    const double *gd = m.gdust + (size_t)id * m.nlam, *pd = m.pldust + (size_t)id * m.nlam;
-    int nl = m.nlam;
-    if (nu >= (T)nd[0]) { o[id].g = (T)gd[0]; o[id].pl = (T)pd[0]; o[id].kappa = (T)kd[0]; o[id].albedo = (T)ad[0]; }
-    else if (nu <= (T)nd[nl - 1]) { o[id].g = (T)gd[nl - 1]; o[id].pl = (T)pd[nl - 1]; o[id].kappa = (T)kd[nl - 1]; o[id].albedo = (T)ad[nl - 1]; }
-    else {
-      int i0 = 1, i1 = nl, i;
-      while (i1 - i0 > 1) {
-        i = (i1 + i0) / 2;
-        if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
-      }
-      T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
-      T fm1 = (T)1 - f;
-      o[id].g = fm1 * (T)gd[i1 - 1] + f * (T)gd[i0 - 1];
-      o[id].pl = fm1 * (T)pd[i1 - 1] + f * (T)pd[i0 - 1];
-      o[id].kappa = fm1 * (T)kd[i1 - 1] + f * (T)kd[i0 - 1];
-      o[id].albedo = fm1 * (T)ad[i1 - 1] + f * (T)ad[i0 - 1];
-    }
-  }
+  static_assert(kMaxOpac == 4, "opacset unrolls four tables");
+  o[0] = opac_one<T>(m, nu, 0);
+  if (m.nopac > 1) o[1] = opac_one<T>(m, nu, 1);
+  if (m.nopac > 2) o[2] = opac_one<T>(m, nu, 2);
+  if (m.nopac > 3) o[3] = opac_one<T>(m, nu, 3);
 }
 
 // random.f90:44-92 bbfreq: the zeta(4) partial sums are computed once on the
@@ -403,14 +424,42 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
     }
     return T0 * (T)nui[nnu - 1];
   }
+  // The reference bisects (lucydustfreq_2_test.f90:96-126) with the
+  // invariant P(i0) <= xi < P(i1).  Every row of the table is a running sum
+  // over frequency divided by its total, so P(i) is non-decreasing in i (the
+  // interpolation between two non-decreasing rows, rounded, stays
+  // non-decreasing) and the bracket it ends with is the unique one with
+  // i1 = i0 + 1.  The same bracket is found here by an 8-way search: each
+  // pass probes up to 7 indices with independent loads, so a 513-entry row
+  // takes 3 dependent rounds instead of 9.  P0 and P1 are the same values.
+  (void)i;
   while (i1 - i0 > 1) {
-    i = (i0 + i1) / 2;
-    P = ((T)1 - f) * KAP(j, i) + f * KAP(j + 1, i);
-    if (P > (T)1) P = (T)1;
-    if (P < P0) { P0 = P; i0 = i; }
-    if (P > P1) { P1 = P; i1 = 1; }
-    if (P > xi) { i1 = i; P1 = P; } else { i0 = i; P0 = P; }
+    const int step = (i1 - i0 + 7) >> 3;
+    T Pq[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int q = i0 + (k + 1) * step;
+      const int qc = q < i1 ? q : i1;
+      T v = ((T)1 - f) * KAP(j, qc) + f * KAP(j + 1, qc);
+      Pq[k] = v > (T)1 ? (T)1 : v;
+    }
+    int n0 = i0, n1 = i1;
+    T p0 = P0, p1 = P1;
+#pragma unroll
+    for (int k = 6; k >= 0; --k) {
+      const int q = i0 + (k + 1) * step;
+      if (q < i1) {
+        if (Pq[k] > xi) { n1 = q; p1 = Pq[k]; }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int q = i0 + (k + 1) * step;
+      if (q < i1 && !(Pq[k] > xi)) { n0 = q; p0 = Pq[k]; }
+    }
+    i0 = n0; i1 = n1; P0 = p0; P1 = p1;
   }
+  (void)P;
   if (xi < P0) f = (T)0;
   else if (xi > P1) f = (T)1;
   else if (P0 <= (T)0) f = (T)0.5;
@@ -622,6 +671,24 @@ __device__ __forceinline__ void findangle3(T cost1, T sint1, T phi1, T cosp1, T 
 }
 
 // ---------------------------------------------------------------------------
+// Rarely needed FP64 transcendentals of testgrid, kept out of line so their
+// registers do not weigh on every walk step.
+__device__ __attribute__((noinline)) double azimuth(double x, double y) {
+  double phi = atan2(y, x);
+  if (phi < 0.0) phi = phi + kR2p;
+  return phi;
+}
+// testgrid.f90:61-63: radius of a random re-placement, uniform in volume
+__device__ __attribute__((noinline)) double shell_radius(double u, double ra, double rb) {
+  return pow(u * (pow(ra, 3.0) - pow(rb, 3.0)) + pow(rb, 3.0), (double)(1.0f / 3.0f));
+}
+struct Point3 { double x, y, z; };
+// testgrid.f90:80-86
+__device__ __attribute__((noinline)) Point3 sphere_point(double rtot, double cosb, double phi) {
+  double s = sqrt(1.0 - pow(cosb, 2.0));
+  return Point3{rtot * s * cos(phi), rtot * s * sin(phi), rtot * cosb};
+}
+
 // testgrid.f90:40-124, on the FP64 geometry.  Returns true when state changed.
 template <class T>
 __device__ __forceinline__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& rng) {
@@ -634,26 +701,35 @@ __device__ __forceinline__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& r
   bool reset = false, changed = false;
   double cosb = check1(p.zp / p.rtot);
   double phi = 0.0;
-  // The azimuth (atan2) only decides anything when the packet is not inside
-  // its phi wedge: phi = atan2(..) lies in [0, 2pi] so the whole-grid phi
-  // test never fires, and inside the wedge pfrac is in [0, 1].  When the
-  // plane signs sin(phi_k) x - cos(phi_k) y prove the packet inside (cells
-  // narrower than pi), atan2 is skipped; every decision is unchanged.
+  // The azimuth (atan2) only decides anything when the packet lies outside
+  // its phi wedge by more than tol of the wedge: phi = atan2(..) is in
+  // [0, 2pi] = [phiarr(1), phiarr(npg)], so the whole-grid phi test never
+  // fires, and inside the widened wedge pfrac is within [-tol, 1+tol].  The
+  // wall planes give s = sin(phi_wall) x - cos(phi_wall) y =
+  // rho sin(phi_wall - phi); s1 <= rho sin(eps), s2 >= -rho sin(eps) with
+  // eps = tol/2 of the narrowest wedge (phi_tol2 = sin(eps)^2) proves the
+  // packet inside the wedge widened by eps (wedges narrower than pi, npg >= 4),
+  // so atan2 is skipped, packets on a phi wall included.  Every decision is
+  // the one atan2 would give.  On the z axis (rho = 0) atan2(0,0) = 0 does
+  // decide, and is computed.
   bool have_phi = true;
   if (npg > 2) {
     have_phi = false;
     if (npg >= 4 && p.ip >= 1 && p.ip <= npg - 1) {
-      double s1 = m.aarr[p.ip - 1] * p.xp + m.barr[p.ip - 1] * p.yp;
-      double s2 = m.aarr[p.ip] * p.xp + m.barr[p.ip] * p.yp;
-      if (!(s1 < 0.0 && s2 > 0.0)) have_phi = true;
+      const double s1 = m.aarr[p.ip - 1] * p.xp + m.barr[p.ip - 1] * p.yp;
+      const double s2 = m.aarr[p.ip] * p.xp + m.barr[p.ip] * p.yp;
+      const double rho2 = p.xp * p.xp + p.yp * p.yp;
+      const double d2 = m.phi_tol2 * rho2;
+      const bool in1 = s1 < 0.0 || s1 * s1 <= d2;
+      const bool in2 = s2 > 0.0 || s2 * s2 <= d2;
+      if (!(rho2 > 0.0 && in1 && in2)) have_phi = true;
     } else {
       have_phi = true;
     }
-    if (have_phi) { phi = atan2(p.yp, p.xp); if (phi < 0.0) phi = phi + R2P; }
+    if (have_phi) phi = azimuth(p.xp, p.yp);
   }
   if ((p.rsq - r2[0]) / (r2[1] - r2[0]) < -tol || (p.rsq - r2[nrg - 2]) / (r2[nrg - 1] - r2[nrg - 2]) > 1.0 + tol) {
-    double ra = m.rarr[nrg - 1], rb = m.rarr[0];
-    p.rtot = pow((double)uniform<T>(rng) * (pow(ra, 3.0) - pow(rb, 3.0)) + pow(rb, 3.0), (double)(1.0f / 3.0f));
+    p.rtot = shell_radius((double)uniform<T>(rng), m.rarr[nrg - 1], m.rarr[0]);
     reset = true;
   }
   if ((cosb - ca[0]) / (ca[1] - ca[0]) < -tol || (cosb - ca[ntg - 2]) / (ca[ntg - 1] - ca[ntg - 2]) > 1.0 + tol) {
@@ -664,19 +740,23 @@ __device__ __forceinline__ bool testgrid(const DeviceModel& m, Pkt<T>& p, Rng& r
     phi = (double)uniform<T>(rng) * (pa[npg - 1] - pa[0]) + pa[0];
     reset = true;
   }
-  if (reset && !have_phi) { phi = atan2(p.yp, p.xp); if (phi < 0.0) phi = phi + R2P; have_phi = true; }
+  if (reset && !have_phi) { phi = azimuth(p.xp, p.yp); have_phi = true; }
   if (reset) {
-    double s = sqrt(1.0 - pow(cosb, 2.0));
-    p.xp = p.rtot * s * cos(phi);
-    p.yp = p.rtot * s * sin(phi);
-    p.zp = p.rtot * cosb;
+    const Point3 q = sphere_point(p.rtot, cosb, phi);
+    p.xp = q.x;
+    p.yp = q.y;
+    p.zp = q.z;
     p.rsq = p.rtot * p.rtot;
     p.on_wall = false;
     changed = true;
   }
   if (p.ir < 1 || p.ir > nrg) { p.ir = locate<double>(r2, nrg, p.rsq); changed = true; }
   if (p.it < 1 || p.it > ntg) { p.it = locate<double>(ca, ntg, cosb); changed = true; }
-  if (p.ip < 1 || p.ip > npg) { p.ip = locate<double>(pa, npg, phi); changed = true; }
+  if (p.ip < 1 || p.ip > npg) {
+    if (!have_phi) { phi = azimuth(p.xp, p.yp); have_phi = true; }
+    p.ip = locate<double>(pa, npg, phi);
+    changed = true;
+  }
   double rfrac = (p.rsq - r2[p.ir - 1]) / (r2[p.ir] - r2[p.ir - 1]);
   double tfrac = (cosb - ca[p.it - 1]) / (ca[p.it] - ca[p.it - 1]);
   bool phierr = false;

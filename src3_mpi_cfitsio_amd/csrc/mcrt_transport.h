@@ -25,8 +25,15 @@ struct alignas(16) PeelRec {
 
 struct PeelQueue {
   void* recs;              // PeelRec<T>[capacity]
-  unsigned* count;         // slots reserved (may exceed capacity: overflow runs inline)
+  unsigned* count;         // slots reserved this round
   unsigned capacity;
+};
+
+// Device control block of one run (16 bytes, zeroed by the host at the start).
+struct RunCtl {
+  unsigned long long next_packet;   // packets handed out so far (run-relative)
+  unsigned q_count;                 // peel records reserved this round
+  unsigned live;                    // lanes holding a packet at the end of this round
 };
 
 struct RunParams {
@@ -35,7 +42,16 @@ struct RunParams {
   uint64_t seed;
   const double* inczeta;   // 145 partial sums of zeta(4), random.f90:74-92
   PeelQueue q;             // q.capacity == 0: every peel-off walk runs inline
-  uint64_t trace_base;     // index of this launch's first packet in the trace buffer
+  // resident packet lanes: a lane owns one packet at a time and keeps it
+  // across rounds in lane_words (structure of arrays, word w of lane l at
+  // lane_words[w * lanes + l]); lane_live[l] != 0 while it holds one
+  uint32_t* lane_words;
+  uint8_t* lane_live;
+  uint64_t lanes;
+  uint32_t events;         // interaction events per lane in this round
+  uint32_t round;          // 0 = first round of the run (no saved lanes)
+  uint32_t regroup;        // waiting lanes of a wavefront that trigger an interaction / emission / tally pass
+  RunCtl* ctl;
 };
 
 template <class T>
@@ -46,20 +62,38 @@ struct Ctx {
   LocalCounts cnt;
   const double* inczeta;
   PeelQueue q;
+  unsigned slot_next, slot_end;   // this lane's reserved, not yet written queue slots
 };
+
+// Peel-off queue slots are reserved kSlotChunk at a time per lane, so an
+// interaction usually writes its record without waiting on an atomic.  Slots
+// a lane reserved but did not fill are marked kSlotHole at the end of the
+// round; the peel kernel skips them.
+constexpr unsigned kSlotChunk = 8;
+constexpr uint32_t kSlotHole = 0xFFFFFFFFu;
 
 template <class T> __device__ __forceinline__ void atomic_add_dtau(double* a, T v) { atomicAdd(a, (double)v); }
 
-// One atomic per wavefront: the active lanes reserve consecutive slots.
-__device__ __forceinline__ unsigned reserve_slot(unsigned* counter) {
+// One atomic per wavefront: each active lane reserves `per` consecutive slots.
+__device__ __forceinline__ unsigned reserve_slots(unsigned* counter, unsigned per) {
   unsigned long long mask = __ballot(1);
   int lane = __lane_id();
   int leader = __ffsll((long long)mask) - 1;
   unsigned rank = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
   unsigned base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask));
+  if (lane == leader) base = atomicAdd(counter, (unsigned)__popcll(mask) * per);
   base = __shfl(base, leader, 64);
-  return base + rank;
+  return base + rank * per;
+}
+
+// the next queue slot of this lane (>= capacity: the queue is full)
+template <class T>
+__device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
+  if (C.slot_next == C.slot_end) {
+    C.slot_next = reserve_slots(C.q.count, kSlotChunk);
+    C.slot_end = C.slot_next + kSlotChunk;
+  }
+  return C.slot_next++;
 }
 
 // initp_common.f90:1-31 (limb = 0)
@@ -113,58 +147,84 @@ __device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
   p.nub = bbfreq<T>(C.rng, (T)C.m.tstar, C.inczeta);
 }
 
-// tauint_3d_2_fuv.f90:5-210.  Returns false on a walk error.
+// ---------------------------------------------------------------------------
+// A packet's life, in resumable pieces.  propagate_5_fuv.f90:1-326 is a loop
+// of (interaction, flight), and a flight (tauint_3d_2_fuv.f90) is a loop of
+// face crossings.  Life<T> is the state between two crossings.  Run back to
+// back, the pieces below are exactly propagate.  Split, they let a
+// wavefront advance every walking lane by one crossing per step and run the
+// costly interaction code only when enough of its lanes are waiting for it.
+// ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ bool tauint(Ctx<T>& C, Pkt<T>& p, T tau) {
+struct Life {
+  Pkt<T> p;
+  G xpold, ypold, zpold;
+  int idust2, iscat, iint, nint;
+  uint64_t pid;        // run-relative packet index (trace slot)
+  T tau, tsum;         // current flight: optical depth drawn / integrated so far
+  long long nseg;      // face crossings of the current flight
+  int pir, pit, pip;   // cell the flight started in
+  bool first;          // first flight after the emission
+};
+
+template <class T>
+__device__ __forceinline__ void flight_begin(Life<T>& L, T tau) {
+  L.tau = tau; L.tsum = (T)0; L.nseg = 0;
+  L.pir = L.p.ir; L.pit = L.p.it; L.pip = L.p.ip;
+}
+
+// One pass of tauint_3d_2_fuv.f90's face loop (:40-205).  0 = the flight
+// goes on, 1 = the interaction point was reached inside the cell, 2 = the
+// packet left the grid, -1 = walk error.
+template <class T>
+__device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
-  T tsum = (T)0;
+  Pkt<T>& p = L.p;
   G dt;
   int iface;
-  long long nseg = 0;
-  for (;;) {
-    if (testgrid<T>(m, p, C.rng)) C.cnt.c[C_RESETS]++;
-    if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) return false;
-    if (++nseg > m.max_segments) return false;
-    C.cnt.c[C_SEGMENTS]++;
-    size_t c = cell_index(m, p.ir, p.it, p.ip);
-    int iop;
-    T rho;
-    T opac = cell_opac<T>(m, p, c, iop, rho);
-    T dtau = opac * (T)dt;
-    if ((tsum + dtau) > tau) {
-      T ds = (tau - tsum) / opac;
-      if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
-      p.xp = p.xp + (G)ds * p.ux;
-      p.yp = p.yp + (G)ds * p.uy;
-      p.zp = p.zp + (G)ds * p.uz;
-      p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-      p.rtot = sqrt(p.rsq);
-      p.on_wall = false;
-      return true;
-    }
-    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
-    cross_face(m, p.ir, p.it, p.ip, iface);
-    p.xp = p.xp + dt * p.ux;
-    p.yp = p.yp + dt * p.uy;
-    p.zp = p.zp + dt * p.uz;
-    p.on_wall = true;
+  if (testgrid<T>(m, p, C.rng)) C.cnt.add(C_RESETS);
+  if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) return -1;
+  if (++L.nseg > m.max_segments) return -1;
+  C.cnt.add(C_SEGMENTS);
+  size_t c = cell_index(m, p.ir, p.it, p.ip);
+  int iop;
+  T rho;
+  T opac = cell_opac<T>(m, p, c, iop, rho);
+  T dtau = opac * (T)dt;
+  if ((L.tsum + dtau) > L.tau) {
+    T ds = (L.tau - L.tsum) / opac;
+    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+    p.xp = p.xp + (G)ds * p.ux;
+    p.yp = p.yp + (G)ds * p.uy;
+    p.zp = p.zp + (G)ds * p.uz;
     p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
     p.rtot = sqrt(p.rsq);
-    tsum = tsum + dtau;
-    if (p.ir == m.nrg) { p.exitflag = 1; return true; }
-    if (p.ir == 0) {
-      C.cnt.c[C_AFLUX]++;
-      p.rtot = sqrt(p.rsq);
-      initpabs(C, p);
-      opacset<T>(m, p.nub, p.op);
-      p.wave = (T)1.2398 / p.nub;
-      p.i_orig = 1;
-      p.iflag = 0;
-      set_kapd(p, m, 2);
-      p.ir = 1;
-      tsum = (T)0;
-    }
+    p.on_wall = false;
+    return 1;
   }
+  if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+  cross_face(m, p.ir, p.it, p.ip, iface);
+  p.xp = p.xp + dt * p.ux;
+  p.yp = p.yp + dt * p.uy;
+  p.zp = p.zp + dt * p.uz;
+  p.on_wall = true;
+  p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+  p.rtot = sqrt(p.rsq);
+  L.tsum = L.tsum + dtau;
+  if (p.ir == m.nrg) { p.exitflag = 1; return 2; }
+  if (p.ir == 0) {     // back into the star: re-emitted from its surface (:170-200)
+    C.cnt.add(C_AFLUX);
+    p.rtot = sqrt(p.rsq);
+    initpabs(C, p);
+    opacset<T>(m, p.nub, p.op);
+    p.wave = (T)1.2398 / p.nub;
+    p.i_orig = 1;
+    p.iflag = 0;
+    set_kapd(p, m, 2);
+    p.ir = 1;
+    L.tsum = (T)0;
+  }
+  return 0;
 }
 
 __device__ __forceinline__ int i_orig_of(int sp) {  // propagate_5_fuv.f90:123-132
@@ -181,50 +241,81 @@ __device__ __forceinline__ int freq_bin(const DeviceModel& m, T nub) {
   return inub;
 }
 
-// peeloff_3d_2_test.f90:7-347.  `q` is the caller's packet (copied).
+// ---------------------------------------------------------------------------
+// peeloff_3d_2_test.f90:7-347 in three pieces: the set-up of the walk toward
+// observer direction k (peel_begin), one face crossing of that walk
+// (peel_step), and the weighting and binning once it leaves the grid
+// (peel_tally).  In order they are exactly peeloff.
+// ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, int peelid, T gsc, T plsc) {
-  const DeviceModel& m = C.m;
-  const DeviceTallies& t = C.t;
-  Pkt<T> p = q;
-  T s[4] = {sipnew, q.sqp, q.sup, q.svp};
-  T tsum = (T)0;
-  G dt;
-  int iface;
-  const int k = peelid - 1;
+struct PeelWalk {
+  Pkt<T> p;          // position, cell, kapd of the walk
+  G ux, uy, uz;      // observer direction
+  T tsum;
+  long long nseg;
+};
+
+// :65-89.  false: the walk starts on the stellar surface heading inwards
+// (nothing reaches the observer).
+template <class T>
+__device__ __forceinline__ bool peel_begin(const DeviceModel& m, const Pkt<T>& q, int k, PeelWalk<T>& w) {
   const double* pe = m.peel + 5 * k;
-  const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
-  const G ux1 = pe[1] * pe[2], uy1 = pe[1] * pe[3], uz1 = pe[0];
-  const double* r2 = m.r2arr;
+  w.p = q;
+  w.ux = pe[1] * pe[2]; w.uy = pe[1] * pe[3]; w.uz = pe[0];
+  w.tsum = (T)0;
+  w.nseg = 0;
+  Pkt<T>& p = w.p;
   if (p.on_wall) {
+    const double* r2 = m.r2arr;
     bool inner = fabs(p.rsq - r2[p.ir - 1]) < fabs(p.rsq - r2[p.ir]);
-    bool outwards = ux1 * p.xp + uy1 * p.yp + uz1 * p.zp > 0.0;
-    if (inner && !outwards) { if (p.ir == 1) return; p.ir = p.ir - 1; }
+    bool outwards = w.ux * p.xp + w.uy * p.yp + w.uz * p.zp > 0.0;
+    if (inner && !outwards) { if (p.ir == 1) return false; p.ir = p.ir - 1; }
     if (!inner && outwards) p.ir = p.ir + 1;
   }
-  long long nseg = 0;
-  for (;;) {
-    if (testgrid<T>(m, p, C.rng)) C.cnt.c[C_RESETS]++;
-    if (!next_face<T>(m, p, ux1, uy1, uz1, dt, iface) || ++nseg > m.max_segments) { C.cnt.c[C_ERRORS]++; return; }
-    C.cnt.c[C_PEELSEG]++;
-    size_t c = cell_index(m, p.ir, p.it, p.ip);
-    int iop;
-    T rho;
-    T opac = cell_opac<T>(m, p, c, iop, rho);
-    T dtau = opac * (T)dt;
-    tsum = tsum + dtau;
-    if (tsum > (T)m.taumax) return;
-    cross_face(m, p.ir, p.it, p.ip, iface);
-    p.xp = p.xp + dt * ux1;
-    p.yp = p.yp + dt * uy1;
-    p.zp = p.zp + dt * uz1;
-    p.on_wall = true;
-    p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
-    p.rtot = sqrt(p.rsq);
-    if (p.ir == m.nrg) break;
-    if (p.ir == 0) return;
-  }
-  C.cnt.c[C_PEELEVT]++;
+  return true;
+}
+
+// :91-190.  0 = the walk goes on, 1 = it left the grid toward the observer,
+// 2 = it ended (tau > taumax, the star, or a walk error).
+template <class T>
+__device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w) {
+  const DeviceModel& m = C.m;
+  Pkt<T>& p = w.p;
+  G dt;
+  int iface;
+  if (testgrid<T>(m, p, C.rng)) C.cnt.add(C_RESETS);
+  if (!next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface) || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  C.cnt.add(C_PEELSEG);
+  size_t c = cell_index(m, p.ir, p.it, p.ip);
+  int iop;
+  T rho;
+  T opac = cell_opac<T>(m, p, c, iop, rho);
+  T dtau = opac * (T)dt;
+  w.tsum = w.tsum + dtau;
+  if (w.tsum > (T)m.taumax) return 2;
+  cross_face(m, p.ir, p.it, p.ip, iface);
+  p.xp = p.xp + dt * w.ux;
+  p.yp = p.yp + dt * w.uy;
+  p.zp = p.zp + dt * w.uz;
+  p.on_wall = true;
+  p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+  p.rtot = sqrt(p.rsq);
+  if (p.ir == m.nrg) return 1;
+  if (p.ir == 0) return 2;
+  return 0;
+}
+
+// :192-347.  q: the packet at the event; s0: its (weighted) Stokes I;
+// (xe, ye, ze): where the walk left the grid; tsum: its optical depth.
+template <class T>
+__device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int k, T gsc, T plsc,
+                                           G xe, G ye, G ze, T tsum) {
+  const DeviceModel& m = C.m;
+  const DeviceTallies& t = C.t;
+  const double* pe = m.peel + 5 * k;
+  const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
+  C.cnt.add(C_PEELEVT);
+  T s[4] = {s0, q.sqp, q.sup, q.svp};
   T factor = RT<T>::exp_(-tsum);
   s[0] = s[0] * factor; s[1] = s[1] * factor; s[2] = s[2] * factor; s[3] = s[3] * factor;
   if (q.iflag == 1) {
@@ -232,8 +323,8 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
     findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
     stokespeel<T>(gsc, plsc, s, coste, sinte, ri1, bmu, q.cost, q.sint);
   }
-  G yimage = p.zp * pe[1] - p.yp * pe[0] * pe[3] - p.xp * pe[0] * pe[2];
-  G ximage = p.yp * pe[2] - p.xp * pe[3];
+  G yimage = ze * pe[1] - ye * pe[0] * pe[3] - xe * pe[0] * pe[2];
+  G ximage = ye * pe[2] - xe * pe[3];
   G rmaxi = m.rmaxi;
   G fractxh = 2.0 * rmaxi / (G)m.nx;
   int ix = (int)((ximage + rmaxi) / fractxh) + 1;
@@ -285,8 +376,8 @@ __device__ __forceinline__ void peeloff(Ctx<T>& C, const Pkt<T>& q, T sipnew, in
 }
 
 // ---------------------------------------------------------------------------
-// Peel-off events: queued for the peel kernel, or walked inline when the
-// queue is full or disabled (same peeloff arithmetic either way).
+// Peel-off events: the transport kernel queues one record per event; the
+// peel kernel walks every record in every observer direction.
 // ---------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned slot, int sp, bool direct,
@@ -308,27 +399,19 @@ __device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned s
 // scattering / re-emission peel (propagate_5_fuv.f90:83-90, :162-170)
 template <class T>
 __device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp) {
-  if (C.q.capacity) {
-    unsigned slot = reserve_slot(C.q.count);
-    if (slot < C.q.capacity) { write_rec<T>(C, p, slot, sp, false, (T)0, (T)0); return; }
-  }
-  const Opac<T> o = pick(p.op, C.m.species_opac[sp]);
-  for (int pid = 1; pid <= C.m.npeel; ++pid) peeloff<T>(C, p, p.sip, pid, o.g, o.pl);
+  if (C.q.capacity == 0) return;          // no observer directions
+  const unsigned slot = take_slot(C);
+  if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, false, (T)0, (T)0);
+  else C.cnt.add(C_ERRORS);              // the host sizes the queue so this cannot happen
 }
 
 // direct stellar peel, weight 4 I cos(norm) (newdisktherm_5_fuv.f90:371-414)
 template <class T>
 __device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp, T sinlp) {
-  if (C.q.capacity) {
-    unsigned slot = reserve_slot(C.q.count);
-    if (slot < C.q.capacity) { write_rec<T>(C, p, slot, 0, true, coslp, sinlp); return; }
-  }
-  for (int pk = 1; pk <= C.m.npeel; ++pk) {
-    const double* pe = C.m.peel + 5 * (pk - 1);
-    T cosnorm = p.cosb * (T)pe[0] + (p.sinb * (T)pe[1] * ((T)pe[2] * coslp + (T)pe[3] * sinlp));
-    T sipnew = (T)4 * p.sip * cosnorm;
-    if (cosnorm > (T)0) peeloff<T>(C, p, sipnew, pk, (T)0, (T)0);
-  }
+  if (C.q.capacity == 0) return;
+  const unsigned slot = take_slot(C);
+  if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, true, coslp, sinlp);
+  else C.cnt.add(C_ERRORS);
 }
 
 // stokes.f90:46-137 rejection sampling, then the Mueller update
@@ -339,7 +422,7 @@ __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
   T g = o.g, pl = o.pl;
   T peak = cs_peak<T>(g), bmu, ri1, rprob, pp[4];
   for (;;) {
-    C.cnt.c[C_HTOT]++;
+    C.cnt.add(C_HTOT);
     bmu = one - two * uniform<T>(C.rng);
     T cosb2 = bmu * bmu;
     dustmat<T>(g, pl, bmu, cosb2, pp);
@@ -356,7 +439,7 @@ __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
     if (rprob > peak) peak = rprob;
     T xran = uniform<T>(C.rng);
     if (peak * xran > rprob) continue;
-    C.cnt.c[C_HIT]++;
+    C.cnt.add(C_HIT);
     break;
   }
   T dir[5] = {p.cost, p.sint, p.phi, p.cosp, p.sinp};
@@ -366,68 +449,77 @@ __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
   p.sip = s[0]; p.sqp = s[1]; p.sup = s[2]; p.svp = s[3];
 }
 
-// propagate_5_fuv.f90:1-326.  Returns 1 escaped, 2 killed, -1 error.
+// After a flight that stopped inside the grid: propagate_5_fuv.f90:199-229
+// (after the first flight, :52-60, where iint is still 0).  0 = the packet
+// waits at the loop top for its next interaction, 2 = killed.
 template <class T>
-__device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
+__device__ __forceinline__ int flight_end(Ctx<T>& C, Life<T>& L) {
+  const DeviceModel& m = C.m;
+  Pkt<T>& p = L.p;
+  if (!L.first) {
+    if (p.ir != L.pir || p.it != L.pit || p.ip != L.pip) L.iint = 0;
+    if (L.iint > m.iintmax) {
+      atomicAdd(&C.t.killed[cell_index(m, p.ir, p.it, p.ip)], 1);
+      C.cnt.add(C_ABFLUX);
+      return 2;
+    }
+  }
+  (void)uniform<T>(C.rng);   // select_dust_2.f90:134 draw
+  L.idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
+  if (L.first) C.cnt.add(C_TOT);
+  L.first = false;
+  return 0;
+}
+
+// propagate_5_fuv.f90:62-198: one interaction (scatter, or absorb and
+// re-emit), its peel-off record, and the start of the next flight.
+template <class T>
+__device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
   const T R2P = (T)kR2p;
-  G xpold = p.xp, ypold = p.yp, zpold = p.zp;
-  int iscat = 0, iint = 0, idust2;
-  nint = 0;
-  p.iflag = 0; p.exitflag = 0;
-  T tau = -RT<T>::log_(uniform<T>(C.rng));
-  if (!tauint(C, p, tau)) return -1;
-  if (!p.exitflag) {
-    (void)uniform<T>(C.rng);   // select_dust_2.f90:134 draw
-    idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
-    C.cnt.c[C_TOT]++;
-    for (;;) {
-      size_t c = cell_index(m, p.ir, p.it, p.ip);
-      int iop = m.species_opac[idust2];
-      nint++;
-      if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
-        p.iflag = 1; iscat++; iint++;
-        if (m.ipeel == 1) peel_event<T>(C, p, idust2);
-        stokes<T>(C, p, iop);
-      } else {
-        atomicAdd(&t.dbig[c], 1);
-        p.i_orig = i_orig_of(idust2);
-        atomicAdd(&t.nabs[c], 1);
-        T tave = (T)m.tdust[c];
-        p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
-        p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
-        p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
-        p.phi = R2P * uniform<T>(C.rng);
-        p.cosp = RT<T>::cos_(p.phi);
-        p.sinp = RT<T>::sin_(p.phi);
-        if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
-        else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
-        p.uy = p.sint * p.sinp; p.ux = p.sint * p.cosp; p.uz = p.cost;
-        p.iflag = 0;
-        opacset<T>(m, p.nub, p.op);
-        p.wave = (T)1.2398 / p.nub;
-        set_kapd(p, m, 1);
-        if (m.ipeel == 1) peel_event<T>(C, p, idust2);
-        iint++;
-      }
-      tau = -RT<T>::log_(uniform<T>(C.rng));
-      xpold = p.xp; ypold = p.yp; zpold = p.zp;
-      p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
-      int pir = p.ir, pit = p.it, pip = p.ip;
-      if (!tauint(C, p, tau)) return -1;
-      if (p.exitflag) break;
-      if (p.ir != pir || p.it != pit || p.ip != pip) iint = 0;
-      if (iint > m.iintmax) {
-        atomicAdd(&t.killed[cell_index(m, p.ir, p.it, p.ip)], 1);
-        C.cnt.c[C_ABFLUX]++;
-        return 2;
-      }
-      (void)uniform<T>(C.rng);
-      idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
-    }
+  Pkt<T>& p = L.p;
+  size_t c = cell_index(m, p.ir, p.it, p.ip);
+  int iop = m.species_opac[L.idust2];
+  L.nint++;
+  if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
+    p.iflag = 1; L.iscat++; L.iint++;
+    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2);
+    stokes<T>(C, p, iop);
+  } else {
+    atomicAdd(&t.dbig[c], 1);
+    p.i_orig = i_orig_of(L.idust2);
+    atomicAdd(&t.nabs[c], 1);
+    T tave = (T)m.tdust[c];
+    p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+    p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
+    p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
+    p.phi = R2P * uniform<T>(C.rng);
+    p.cosp = RT<T>::cos_(p.phi);
+    p.sinp = RT<T>::sin_(p.phi);
+    if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
+    else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
+    p.uy = p.sint * p.sinp; p.ux = p.sint * p.cosp; p.uz = p.cost;
+    p.iflag = 0;
+    opacset<T>(m, p.nub, p.op);
+    p.wave = (T)1.2398 / p.nub;
+    set_kapd(p, m, 1);
+    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2);
+    L.iint++;
   }
-  // SED binning, propagate_5_fuv.f90:231-320
+  T tau = -RT<T>::log_(uniform<T>(C.rng));
+  L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
+  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  flight_begin(L, tau);
+}
+
+// SED binning of an escaped packet, propagate_5_fuv.f90:231-320
+template <class T>
+__device__ __forceinline__ void prop_end(Ctx<T>& C, Life<T>& L) {
+  const DeviceModel& m = C.m;
+  const DeviceTallies& t = C.t;
+  const T R2P = (T)kR2p;
+  Pkt<T>& p = L.p;
   int itb = (int)((T)m.nmu * ((T)1 - p.cost) / (T)2) + 1;
   if (p.phi < (T)0) p.phi = p.phi + R2P;
   int ipb = (int)((T)m.nph * p.phi / R2P) + 1;
@@ -435,12 +527,12 @@ __device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
   if (itb > m.nmu) itb = m.nmu;
   if (ipb < 1) ipb = 1;
   if (ipb > m.nph) ipb = m.nph;
-  G x = ypold * (G)p.cosp - xpold * (G)p.sinp;
-  G y = zpold * m.sini_mu[itb - 1] - ypold * m.u_mu[itb - 1] * (G)p.sinp - xpold * m.u_mu[itb - 1] * (G)p.cosp;
+  G x = L.ypold * (G)p.cosp - L.xpold * (G)p.sinp;
+  G y = L.zpold * m.sini_mu[itb - 1] - L.ypold * m.u_mu[itb - 1] * (G)p.sinp - L.xpold * m.u_mu[itb - 1] * (G)p.cosp;
   G rho2 = (x * x + y * y);
   if (rho2 < m.aperture2[m.nap - 1] * 1.0001) {
-    C.cnt.c[C_FLUX]++;
-    if (p.iflag == 1) { C.cnt.c[C_SFLUX]++; C.cnt.c[C_NSCAT] += (unsigned long long)iscat; }
+    C.cnt.add(C_FLUX);
+    if (p.iflag == 1) { C.cnt.add(C_SFLUX); C.cnt.add(C_NSCAT, (unsigned long long)L.iscat); }
   }
   int inub = freq_bin<T>(m, p.nub);
   int i_inter = (p.iflag == 1) ? 2 : ((p.i_orig == 1) ? 1 : 3);
@@ -463,15 +555,15 @@ __device__ __forceinline__ int propagate(Ctx<T>& C, Pkt<T>& p, int& nint) {
       }
     }
   }
-  if (p.iflag == 1) C.cnt.c[C_SCOUNT]++;
-  return 1;
+  if (p.iflag == 1) C.cnt.add(C_SCOUNT);
 }
 
-// newdisktherm_5_fuv.f90:294-413 one stellar photon (ips = 0, no spots)
+// newdisktherm_5_fuv.f90:294-413: one stellar photon (ips = 0, no spots),
+// its direct peel, and the start of its first flight.
 template <class T>
-__device__ __forceinline__ void one_packet(Ctx<T>& C, uint64_t pid) {
+__device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
-  Pkt<T> p;
+  Pkt<T>& p = L.p;
   p.on_wall = false;
   p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
   for (;;) {
@@ -502,85 +594,296 @@ __device__ __forceinline__ void one_packet(Ctx<T>& C, uint64_t pid) {
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
   p.ip = locate<double>(m.phiarr, m.npg, (double)p.lp);
   if (m.ipeel == 1) peel_direct<T>(C, p, coslp, sinlp);
-  int nint = 0;
-  int code = propagate<T>(C, p, nint);
-  if (code == 1) C.cnt.c[C_ESCAPED]++;
-  else if (code == 2) C.cnt.c[C_KILLED]++;
-  else C.cnt.c[C_ERRORS]++;
-  if (C.t.trace) {
-    double* tr = C.t.trace + 8 * pid;
-    tr[0] = (double)p.nub; tr[1] = (double)p.cost; tr[2] = (double)p.phi; tr[3] = (double)p.sip;
-    tr[4] = (double)p.sqp; tr[5] = (double)p.sup; tr[6] = (double)nint; tr[7] = (double)code;
-  }
+  // propagate_5_fuv.f90:20-40: the first flight
+  L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
+  L.iscat = 0; L.iint = 0; L.nint = 0;
+  p.iflag = 0; p.exitflag = 0;
+  T tau = -RT<T>::log_(uniform<T>(C.rng));
+  L.first = true;
+  flight_begin(L, tau);
 }
 
-__device__ __forceinline__ void flush_counts(const DeviceTallies& t, LocalCounts& cnt) {
-  // wavefront reduction then one atomic per counter per wavefront
-#pragma unroll
-  for (int i = 0; i < C_NCOUNTERS; ++i) {
-    unsigned long long v = cnt.c[i];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&t.counters[i], v);
-  }
-}
-
-// Transport kernel: one packet per thread (grid-stride).  Peel-off events go
-// to the queue; k_peel walks them.
+// end of a packet: SED for escaped ones, outcome counters, trace row
 template <class T>
-__global__ void __launch_bounds__(256) k_packets(DeviceModel m, DeviceTallies t, RunParams rp) {
-  Ctx<T> C{m, t, {}, {}, rp.inczeta, rp.q};
-  C.cnt.zero();
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rp.npackets; i += stride) {
-    rng_init(C.rng, rp.seed, rp.first_packet + i);
-    one_packet<T>(C, rp.trace_base + i);
+__device__ __forceinline__ void end_packet(Ctx<T>& C, Life<T>& L, int code) {
+  if (code == 1) { prop_end<T>(C, L); C.cnt.add(C_ESCAPED); }
+  else if (code == 2) C.cnt.add(C_KILLED);
+  else C.cnt.add(C_ERRORS);
+  if (C.t.trace) {
+    const Pkt<T>& p = L.p;
+    double* tr = C.t.trace + 8 * L.pid;
+    tr[0] = (double)p.nub; tr[1] = (double)p.cost; tr[2] = (double)p.phi; tr[3] = (double)p.sip;
+    tr[4] = (double)p.sqp; tr[5] = (double)p.sup; tr[6] = (double)L.nint; tr[7] = (double)code;
   }
-  flush_counts(t, C.cnt);
 }
 
-// Peel kernel: one (event, direction) pair per thread.  Lanes of a
-// wavefront run the same face-walk loop from queued events, so the walk that
-// dominates the path (npeel walks per interaction) runs without the
-// transport kernel's emission / sampling divergence and register load.
+// ---------------------------------------------------------------------------
+// Parking a packet between rounds.  Only what the loop top needs is kept:
+// p.op is opacset(p.nub) at every point of the loop, so it is recomputed;
+// cosb/sinb/lp are rewritten by initpabs before any later use.
+// ---------------------------------------------------------------------------
+struct LaneIO {
+  uint32_t* base;
+  uint64_t stride;
+  int w;
+  __device__ __forceinline__ void put(uint32_t v) { base[(size_t)(w++) * stride] = v; }
+  __device__ __forceinline__ uint32_t get() { return base[(size_t)(w++) * stride]; }
+  __device__ __forceinline__ void put_v(double v) {
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    put((uint32_t)b); put((uint32_t)(b >> 32));
+  }
+  __device__ __forceinline__ void put_v(float v) { put(__float_as_uint(v)); }
+  __device__ __forceinline__ void get_v(double& v) {
+    uint32_t lo = get(), hi = get();
+    v = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+  }
+  __device__ __forceinline__ void get_v(float& v) { v = __uint_as_float(get()); }
+};
+
+// words per lane: 11 FP64 geometry values, 15 T values, 8 integers
+template <class T> constexpr int lane_words() { return 22 + 15 * (int)(sizeof(T) / 4) + 8; }
+
+template <class T>
+__device__ __forceinline__ void park(LaneIO& io, const Ctx<T>& C, const Life<T>& L) {
+  const Pkt<T>& p = L.p;
+  io.put_v(p.xp); io.put_v(p.yp); io.put_v(p.zp); io.put_v(p.rsq); io.put_v(p.rtot);
+  io.put_v(p.ux); io.put_v(p.uy); io.put_v(p.uz);
+  io.put_v(L.xpold); io.put_v(L.ypold); io.put_v(L.zpold);
+  io.put_v(p.cost); io.put_v(p.sint); io.put_v(p.phi); io.put_v(p.cosp); io.put_v(p.sinp);
+  io.put_v(p.sip); io.put_v(p.sqp); io.put_v(p.sup); io.put_v(p.svp);
+  io.put_v(p.nub); io.put_v(p.wave);
+#pragma unroll
+  for (int i = 0; i < kMaxOpac; ++i) io.put_v(p.kapd[i]);
+  io.put((uint32_t)p.ir | ((uint32_t)p.it << 10) | ((uint32_t)p.ip << 20) | ((uint32_t)(p.on_wall ? 1 : 0) << 30));
+  io.put((uint32_t)p.iflag | ((uint32_t)p.i_orig << 1) | ((uint32_t)L.idust2 << 8));
+  io.put((uint32_t)L.iscat); io.put((uint32_t)L.iint); io.put((uint32_t)L.nint);
+  io.put((uint32_t)L.pid); io.put((uint32_t)(L.pid >> 32));
+  io.put(C.rng.n);
+}
+
+template <class T>
+__device__ __forceinline__ void unpark(LaneIO& io, Ctx<T>& C, Life<T>& L, uint64_t seed, uint64_t first_packet) {
+  Pkt<T>& p = L.p;
+  io.get_v(p.xp); io.get_v(p.yp); io.get_v(p.zp); io.get_v(p.rsq); io.get_v(p.rtot);
+  io.get_v(p.ux); io.get_v(p.uy); io.get_v(p.uz);
+  io.get_v(L.xpold); io.get_v(L.ypold); io.get_v(L.zpold);
+  io.get_v(p.cost); io.get_v(p.sint); io.get_v(p.phi); io.get_v(p.cosp); io.get_v(p.sinp);
+  io.get_v(p.sip); io.get_v(p.sqp); io.get_v(p.sup); io.get_v(p.svp);
+  io.get_v(p.nub); io.get_v(p.wave);
+#pragma unroll
+  for (int i = 0; i < kMaxOpac; ++i) io.get_v(p.kapd[i]);
+  uint32_t cell = io.get(), fl = io.get();
+  p.ir = (int)(cell & 1023u); p.it = (int)((cell >> 10) & 1023u); p.ip = (int)((cell >> 20) & 1023u);
+  p.on_wall = ((cell >> 30) & 1u) != 0;
+  p.iflag = (int)(fl & 1u); p.i_orig = (int)((fl >> 1) & 127u); L.idust2 = (int)(fl >> 8);
+  p.exitflag = 0;
+  L.iscat = (int)io.get(); L.iint = (int)io.get(); L.nint = (int)io.get();
+  uint32_t lo = io.get(), hi = io.get();
+  L.pid = ((uint64_t)hi << 32) | lo;
+  uint32_t n = io.get();
+  rng_resume(C.rng, seed, first_packet + L.pid, n);
+  opacset<T>(C.m, p.nub, p.op);
+  L.first = false;
+}
+
+// Workgroup counters: zeroed at the start, flushed with one device atomic
+// per counter at the end (every thread of the workgroup reaches both).
+__device__ __forceinline__ void counts_begin(unsigned long long* s) {
+  if (threadIdx.x < C_NCOUNTERS) s[threadIdx.x] = 0ull;
+  __syncthreads();
+}
+__device__ __forceinline__ void counts_flush(const DeviceTallies& t, unsigned long long* s) {
+  __syncthreads();
+  if (threadIdx.x < C_NCOUNTERS && s[threadIdx.x]) atomicAdd(&t.counters[threadIdx.x], s[threadIdx.x]);
+}
+
+// The active lanes of a wavefront take consecutive packet indices with one
+// atomic.
+__device__ __forceinline__ unsigned long long fetch_packets(unsigned long long* counter) {
+  unsigned long long mask = __ballot(1);
+  int lane = __lane_id();
+  int leader = __ffsll((long long)mask) - 1;
+  unsigned long long rank = (unsigned long long)__popcll(mask & ((1ull << lane) - 1ull));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  return base + rank;
+}
+
+// Transport kernel: one thread per resident lane, rp.lanes threads (a
+// multiple of 256).  A lane owns one packet at a time: walking a flight,
+// waiting at the loop top for its next interaction (READY), or idle.  Each
+// step advances every walking lane of the wavefront by one face crossing.
+// READY lanes run their interaction, and idle lanes take and emit a new
+// packet, only when rp.regroup lanes of the wavefront are waiting for that
+// (or when nobody walks): the long interaction code then runs with most
+// lanes active instead of one lane at a time, and the walk (the bulk of
+// the work) runs with all walking lanes in step.
+// Per round a lane runs at most rp.events events (emissions + interactions),
+// each queueing at most one peel-off record, so a queue of
+// lanes * (events + kSlotChunk - 1) records never overflows.  A packet still
+// waiting for an interaction when its lane's events are spent is parked in
+// HBM for the next round; long-lived packets (hundreds of thousands of
+// interactions in optically thick cells) thereby run beside fresh packets
+// instead of holding the whole launch.
+enum : int { L_IDLE = 0, L_WALK = 1, L_READY = 2 };
+
+template <class T>
+__global__ void __launch_bounds__(256) k_transport(DeviceModel m, DeviceTallies t, RunParams rp) {
+  __shared__ unsigned long long s_cnt[C_NCOUNTERS];
+  counts_begin(s_cnt);
+  Ctx<T> C{m, t, {}, {s_cnt}, rp.inczeta, rp.q};
+  const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  LaneIO io{rp.lane_words + lane, rp.lanes, 0};
+  Life<T> L;
+  int st = L_IDLE;
+  if (rp.round > 0 && rp.lane_live[lane]) {
+    unpark<T>(io, C, L, rp.seed, rp.first_packet);
+    st = L_READY;
+  }
+  uint32_t budget = rp.events;
+  bool dry = false;                 // the packet counter ran out for this lane
+  for (;;) {
+    if (st == L_WALK) {
+      const int r = walk_step<T>(C, L);
+      if (r != 0) {
+        const int code = (r == 1) ? flight_end<T>(C, L) : ((r == 2) ? 1 : -1);
+        if (code == 0) st = L_READY;
+        else { end_packet<T>(C, L, code); st = L_IDLE; }
+      }
+    }
+    const unsigned long long walking = __ballot(st == L_WALK);
+    const unsigned long long ready = __ballot(st == L_READY && budget > 0);
+    const unsigned long long idle = __ballot(st == L_IDLE && budget > 0 && !dry);
+    if ((walking | ready | idle) == 0ull) break;
+    if (ready && (walking == 0ull || (unsigned)__popcll(ready) >= rp.regroup)) {
+      if (st == L_READY && budget > 0) {
+        interact<T>(C, L);
+        budget--;
+        st = L_WALK;
+      }
+    }
+    if (idle && (walking == 0ull || (unsigned)__popcll(idle) >= rp.regroup)) {
+      if (st == L_IDLE && budget > 0 && !dry) {
+        const unsigned long long k = fetch_packets(&rp.ctl->next_packet);
+        if (k >= rp.npackets) {
+          dry = true;
+        } else {
+          rng_init(C.rng, rp.seed, rp.first_packet + k);
+          L.pid = k;
+          emit<T>(C, L);
+          budget--;
+          st = L_WALK;
+        }
+      }
+    }
+  }
+  const bool live = (st == L_READY);
+  if (live) { io.w = 0; park<T>(io, C, L); }
+  rp.lane_live[lane] = live ? 1 : 0;
+  for (unsigned sl = C.slot_next; sl < C.slot_end && sl < C.q.capacity; ++sl)
+    ((PeelRec<T>*)C.q.recs)[sl].flags = kSlotHole;
+  const unsigned long long mask = __ballot(live);
+  if (__lane_id() == 0 && mask) atomicAdd(&rp.ctl->live, (unsigned)__popcll(mask));
+  counts_flush(t, s_cnt);
+}
+
+// the packet fields peeloff reads, from a queued record
+template <class T>
+__device__ __forceinline__ void rec_to_pkt(const PeelRec<T>& r, Pkt<T>& q) {
+  q.xp = r.xp; q.yp = r.yp; q.zp = r.zp; q.rsq = r.rsq; q.rtot = r.rtot;
+  q.sip = r.sip; q.sqp = r.sqp; q.sup = r.sup; q.svp = r.svp;
+  q.cost = r.cost; q.sint = r.sint; q.phi = r.phi; q.cosp = r.cosp; q.sinp = r.sinp;
+  q.nub = r.nub; q.wave = r.wave;
+#pragma unroll
+  for (int k = 0; k < kMaxOpac; ++k) q.kapd[k] = r.kapd[k];
+  q.ir = (int)(r.cell & 1023u); q.it = (int)((r.cell >> 10) & 1023u); q.ip = (int)((r.cell >> 20) & 1023u);
+  q.on_wall = ((r.cell >> 30) & 1u) != 0;
+  q.iflag = (int)(r.flags & 1u);
+  q.i_orig = (int)((r.flags >> 1) & 7u);
+}
+
+// Stokes I sent toward direction k.  For the direct stellar peel
+// (newdisktherm_5_fuv.f90:371-414) it is 4 I cos(norm), and directions
+// behind the emitting surface element get nothing (false).
+template <class T>
+__device__ __forceinline__ bool peel_weight(const DeviceModel& m, const PeelRec<T>& r, int k, T& s0) {
+  s0 = r.sip;
+  if ((r.flags >> 4) & 1u) {
+    const double* pe = m.peel + 5 * k;
+    T cosnorm = r.cost * (T)pe[0] + (r.sint * (T)pe[1] * ((T)pe[2] * r.cosp + (T)pe[3] * r.sinp));
+    if (!(cosnorm > (T)0)) return false;
+    s0 = (T)4 * r.sip * cosnorm;
+  }
+  return true;
+}
+
+// Peel kernel: each lane walks its share of the round's (record, direction)
+// items, item = lane, lane + stride, ...  Every step advances each walking
+// lane by one face crossing; the tally of walks that reached the observer
+// (exponential weight, findangle3, stokespeel, image and SED binning) runs
+// once rp.regroup lanes of the wavefront are waiting for it, or nobody walks.
 template <class T>
 __global__ void __launch_bounds__(256) k_peel(DeviceModel m, DeviceTallies t, RunParams rp) {
-  Ctx<T> C{m, t, {}, {}, rp.inczeta, PeelQueue{nullptr, nullptr, 0}};
-  C.cnt.zero();
+  __shared__ unsigned long long s_cnt[C_NCOUNTERS];
+  counts_begin(s_cnt);
+  Ctx<T> C{m, t, {}, {s_cnt}, rp.inczeta, PeelQueue{nullptr, nullptr, 0}};
   const unsigned nq = *rp.q.count;
   const uint64_t nrec = nq < rp.q.capacity ? nq : rp.q.capacity;
-  const uint64_t total = nrec * (uint64_t)m.npeel;
+  const uint64_t npeel = (uint64_t)m.npeel;
+  const uint64_t total = nrec * npeel;
   const PeelRec<T>* recs = (const PeelRec<T>*)rp.q.recs;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const uint64_t ri = i / (uint64_t)m.npeel;
-    const int pk = (int)(i - ri * (uint64_t)m.npeel) + 1;
-    const PeelRec<T>& r = recs[ri];
-    Pkt<T> q;
-    q.xp = r.xp; q.yp = r.yp; q.zp = r.zp; q.rsq = r.rsq; q.rtot = r.rtot;
-    q.sip = r.sip; q.sqp = r.sqp; q.sup = r.sup; q.svp = r.svp;
-    q.cost = r.cost; q.sint = r.sint; q.phi = r.phi; q.cosp = r.cosp; q.sinp = r.sinp;
-    q.nub = r.nub; q.wave = r.wave;
-#pragma unroll
-    for (int k = 0; k < kMaxOpac; ++k) q.kapd[k] = r.kapd[k];
-    q.ir = (int)(r.cell & 1023u); q.it = (int)((r.cell >> 10) & 1023u); q.ip = (int)((r.cell >> 20) & 1023u);
-    q.on_wall = ((r.cell >> 30) & 1u) != 0;
-    q.iflag = (int)(r.flags & 1u);
-    q.i_orig = (int)((r.flags >> 1) & 7u);
-    const bool direct = ((r.flags >> 4) & 1u) != 0;
-    const int sp = (int)((r.flags >> 8) & 255u);
-    rng_init(C.rng, rp.seed ^ 0x5045454C00000000ull, rp.first_packet * 64u + i);   // testgrid resets only
-    T sipnew = q.sip;
-    if (direct) {
-      const double* pe = m.peel + 5 * (pk - 1);
-      T cosnorm = q.cost * (T)pe[0] + (q.sint * (T)pe[1] * ((T)pe[2] * q.cosp + (T)pe[3] * q.sinp));
-      if (!(cosnorm > (T)0)) continue;
-      sipnew = (T)4 * q.sip * cosnorm;
-      q.cosb = q.cost; q.sinb = q.sint;
+  const uint64_t rkey = rp.seed ^ 0x5045454C00000000ull ^ rp.first_packet;
+  uint64_t item = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  enum : int { P_IDLE = 0, P_WALK = 1, P_DONE = 2 };
+  int st = P_IDLE;
+  PeelWalk<T> w;
+  for (;;) {
+    while (st == P_IDLE && item < total) {
+      const uint64_t ri = item / npeel;
+      const int k = (int)(item - ri * npeel);
+      const PeelRec<T>& r = recs[ri];
+      if (r.flags != kSlotHole) {
+        T s0;
+        if (peel_weight<T>(m, r, k, s0)) {
+          Pkt<T> q;
+          rec_to_pkt<T>(r, q);
+          if (peel_begin<T>(m, q, k, w)) {
+            // testgrid re-placements only: one stream per (run, round, walk)
+            rng_init(C.rng, rkey, ((uint64_t)rp.round << 40) + item);
+            st = P_WALK;
+            break;
+          }
+        }
+      }
+      item += stride;
     }
-    peeloff<T>(C, q, sipnew, pk, r.g, r.pl);
+    if (st == P_WALK) {
+      const int r = peel_step<T>(C, w);
+      if (r == 1) st = P_DONE;
+      else if (r == 2) { st = P_IDLE; item += stride; }
+    }
+    const unsigned long long walking = __ballot(st == P_WALK);
+    const unsigned long long done = __ballot(st == P_DONE);
+    const unsigned long long more = __ballot(st == P_IDLE && item < total);
+    if ((walking | done | more) == 0ull) break;
+    if (done && (walking == 0ull || (unsigned)__popcll(done) >= rp.regroup)) {
+      if (st == P_DONE) {
+        const uint64_t ri = item / npeel;
+        const int k = (int)(item - ri * npeel);
+        const PeelRec<T>& r = recs[ri];
+        T s0;
+        (void)peel_weight<T>(m, r, k, s0);
+        Pkt<T> q;
+        rec_to_pkt<T>(r, q);
+        peel_tally<T>(C, q, s0, k, r.g, r.pl, w.p.xp, w.p.yp, w.p.zp, w.tsum);
+        st = P_IDLE;
+        item += stride;
+      }
+    }
   }
-  flush_counts(t, C.cnt);
+  counts_flush(t, s_cnt);
 }
 
 }  // namespace mcrt

@@ -26,6 +26,8 @@ from .model import LSUN, Model
 
 COUNTERS = ["tot", "flux", "sflux", "aflux", "abflux", "scount", "nscat", "hit", "htot",
             "escaped", "killed", "segments", "peel_segments", "peel_events", "resets", "errors"]
+RUN_STATS = ["rounds", "transport_ms", "peel_ms", "peel_launches", "records", "parked", "lanes", "events0",
+             "record_bytes", "lane_bytes"]
 SED_NAMES = ["si", "sq", "su", "sv", "si2", "sq2", "su2", "sv2", "nums", "aveinc"]
 PSED_NAMES = ["ti", "tq", "tu", "tv", "ti2", "tq2", "tu2", "tv2", "numt"]
 IMAGE_NAMES = ["image_b_i", "image_b_q", "image_b_u", "image_b_v",
@@ -117,10 +119,13 @@ class Engine:
         self._check(self.lib.mcrt_run_packets(self.h, int(first_packet), int(npackets),
                                               C.c_uint64(int(seed)), int(precision)))
 
-    def set_launch(self, batch_packets=0, queue_per_packet=0, transport_blocks_per_cu=0, peel_blocks_per_cu=0):
-        """Launch shape of run_packets (queue_per_packet=-1: peel-offs inline)."""
-        self._check(self.lib.mcrt_set_launch(self.h, int(batch_packets), int(queue_per_packet),
-                                             int(transport_blocks_per_cu), int(peel_blocks_per_cu)))
+    def set_launch(self, lanes=0, queue_per_lane=0, events_per_round=0, peel_blocks_per_cu=0, regroup=0):
+        """Launch shape of run_packets: resident packet lanes (multiple of
+        256), peel-off queue records per lane, events per lane per round,
+        peel-kernel blocks per CU, waiting lanes per wavefront that trigger
+        an interaction / emission / tally pass.  0 = default."""
+        self._check(self.lib.mcrt_set_launch(self.h, int(lanes), int(queue_per_lane),
+                                             int(events_per_round), int(peel_blocks_per_cu), int(regroup)))
 
     def synchronize(self):
         self._check(self.lib.mcrt_synchronize(self.h))
@@ -129,6 +134,12 @@ class Engine:
         v = C.c_float()
         self._check(self.lib.mcrt_last_run_ms(self.h, C.byref(v)))
         return float(v.value)
+
+    def run_stats(self) -> dict:
+        """Rounds, per-kernel HIP-event times and queue counts of the last run."""
+        v = np.zeros(len(RUN_STATS))
+        self._check(self.lib.mcrt_run_stats(self.h, ptr(v, C.c_double)))
+        return {k: float(v[i]) for i, k in enumerate(RUN_STATS)}
 
     def tallies(self) -> Dict[str, np.ndarray]:
         """Post-reduction tallies named as in the reference's modules."""
