@@ -24,7 +24,7 @@ COMMON = [
 # kernels_fp64 must round exactly like the reference: no FMA contraction.
 UNITS = [
     ("kernels_fp64.hip", ["-ffp-contract=off"]),
-    ("kernels_mixed.hip", ["-ffp-contract=fast"]),
+    ("kernels_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("mcrt_abi.hip", ["-ffp-contract=off"]),
 ]
 

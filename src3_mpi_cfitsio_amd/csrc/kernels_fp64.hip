@@ -8,14 +8,14 @@
 namespace mcrt {
 
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
-  hipLaunchKernelGGL(k_transport<double>, dim3((unsigned)(rp.lanes / 256)), dim3(256), 0, s, m, t, rp);
+  hipLaunchKernelGGL(k_transport<double>, dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();
 }
 
 int lane_words_fp64() { return lane_words<double>(); }
 
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel<double>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  hipLaunchKernelGGL(k_peel<double>, dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();
 }
 

@@ -7,14 +7,14 @@
 namespace mcrt {
 
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
-  hipLaunchKernelGGL(k_transport<float>, dim3((unsigned)(rp.lanes / 256)), dim3(256), 0, s, m, t, rp);
+  hipLaunchKernelGGL(k_transport<float>, dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();
 }
 
 int lane_words_mixed() { return lane_words<float>(); }
 
 hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_peel<float>, dim3(grid), dim3(256), 0, s, m, t, rp);
+  hipLaunchKernelGGL(k_peel<float>, dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();
 }
 

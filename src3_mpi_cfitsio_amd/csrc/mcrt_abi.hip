@@ -4,6 +4,7 @@
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +42,34 @@ struct mcrt_ctx {
 };
 
 namespace {
+
+// LDS plan of the transport and peel kernels: the walls, Planck grids, peel
+// directions and zeta sums always (at most kLdsAlways doubles), the dust
+// tables when the total stays within kLdsMax doubles.
+constexpr uint32_t kLdsMax = 62u * 1024u / 8u;
+constexpr uint32_t kLdsAlways = 56u * 1024u / 8u;
+LdsPlan plan_lds(const DeviceModel& m) {
+  LdsPlan P{};
+  const uint32_t nrg = (uint32_t)m.nrg, ntg = (uint32_t)m.ntg, npg = (uint32_t)m.npg;
+  const uint32_t nd = (uint32_t)(m.nopac * m.nlam);
+  const uint32_t sizes[LT_COUNT] = {nrg, nrg, ntg, ntg, ntg, ntg, npg, npg, npg,
+                                    (uint32_t)m.nT, (uint32_t)m.nnu, 5u * (uint32_t)(m.npeel > 0 ? m.npeel : 1), 145u,
+                                    nd, nd, nd, nd, nd};
+  uint32_t used = 0;
+  for (int k = 0; k < LT_COUNT; ++k) {
+    if (k == LT_NUDUST && used + 5u * nd > kLdsMax) break;
+    P.off[k] = used;
+    P.n[k] = sizes[k];
+    used += sizes[k];
+  }
+  P.bytes = used * 8u;
+  return P;
+}
+
+uint32_t lds_always_doubles(const DeviceModel& m) {
+  return 2u * (uint32_t)m.nrg + 4u * (uint32_t)m.ntg + 3u * (uint32_t)m.npg + (uint32_t)m.nT + (uint32_t)m.nnu +
+         5u * (uint32_t)(m.npeel > 0 ? m.npeel : 1) + 145u;
+}
 
 int fail(mcrt_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -253,6 +282,19 @@ int mcrt_set_grid(mcrt_ctx* c, int32_t nrg, int32_t ntg, int32_t npg, const doub
   rc |= upload(c, "aarr", aarr, npg, &m.aarr);
   rc |= upload(c, "barr", barr, npg, &m.barr);
   rc |= upload(c, "vcell", vcell, ncell, &m.vcell);
+  {
+    // double-cone half-angle t = min(theta, pi - theta) of each theta wall
+    std::vector<double> cc(ntg), sc(ntg);
+    for (int k = 0; k < ntg; ++k) {
+      const double t = std::min(thetarr[k], M_PI - thetarr[k]);
+      cc[k] = std::cos(t);
+      sc[k] = std::sin(t);
+    }
+    rc |= upload(c, "ccone", cc.data(), ntg, &m.ccone);
+    rc |= upload(c, "scone", sc.data(), ntg, &m.scone);
+    if (rc) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));   // the host vectors are freed here
+  }
   if (rc) return rc;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->has_grid = true;
@@ -312,6 +354,23 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
   rc |= upload(c, "kappap", kappap, (size_t)nopac * nT, &m.kappap);
   if (rc) return rc;
   for (int i = 0; i < kMaxOpac; ++i) m.kappav[i] = (i < nopac) ? kappav[i] : 1.0;
+  // opacset's index guess: only for a strictly decreasing, log-uniform nudust
+  for (int id = 0; id < kMaxOpac; ++id) {
+    m.nud_log0[id] = 0.0;
+    m.nud_ilog[id] = 0.0;
+    if (id >= nopac) continue;
+    const double* nd = nudust + (size_t)id * nlam;
+    bool ok = nd[0] > 0.0 && nd[nlam - 1] > 0.0;
+    const double step = ok ? std::log(nd[0] / nd[nlam - 1]) / (nlam - 1) : 0.0;
+    for (int k = 1; ok && k < nlam; ++k) {
+      if (!(nd[k] < nd[k - 1])) ok = false;
+      else if (std::fabs(std::log(nd[k - 1] / nd[k]) - step) > 1e-3 * step) ok = false;
+    }
+    if (ok && step > 0.0) {
+      m.nud_log0[id] = std::log(nd[0]);
+      m.nud_ilog[id] = 1.0 / step;
+    }
+  }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->has_dust = true;
   return MCRT_OK;
@@ -443,6 +502,10 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     q.capacity = (unsigned)capacity;
   }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
+  if (lds_always_doubles(c->dm) > kLdsAlways)
+    return fail(c, MCRT_E_ARG, "mcrt_run_packets: walls, Planck grids and observer directions exceed the 56 KB LDS budget "
+                               "(2*nrg + 4*ntg + 3*npg + nT + nnu + 5*npeel must stay below 7023)");
+  const LdsPlan lds = plan_lds(c->dm);
   double st[MCRT_NSTATS] = {0};
   const char* venv = std::getenv("MCRT_VERBOSE");
   const bool verbose = venv && venv[0] && venv[0] != '0';
@@ -468,6 +531,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     rp.round = (uint32_t)(round < 0xffffffffll ? round : 0xffffffffll);
     rp.ctl = ctl;
     rp.regroup = (uint32_t)(c->regroup > 0 ? c->regroup : 32);
+    rp.lds = lds;
     // q_count and live restart every round (a fill kernel, not a memset)
     HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
     HIPCHECK(c, hipEventRecord(c->evr[0], c->stream));

@@ -37,6 +37,7 @@ struct DeviceModel {
   // grid walls, 1-based reference semantics stored 0-based (wall k at [k-1])
   int nrg, ntg, npg;
   const double *rarr, *r2arr, *thetarr, *costarr, *tan2arr, *phiarr, *aarr, *barr;
+  const double *ccone, *scone;   // cos, sin of min(theta, pi - theta) per theta wall
   double phi_tol2;               // sin^2 of half testgrid's tolerance on the narrowest phi wedge
   const double* vcell;           // cm^3 per cell
   // medium: one thermal species per cell
@@ -48,6 +49,8 @@ struct DeviceModel {
   // dust
   int nopac, nlam;
   const double *nudust, *kappad, *adust, *gdust, *pldust;
+  // log-uniform nudust of table id: log(nudust[0]) and 1/step (0: no guess)
+  double nud_log0[kMaxOpac], nud_ilog[kMaxOpac];
   double kappav[kMaxOpac];
   int nT, nnu;
   const double *lTint, *Tint, *nuint, *kapint, *kappap;
@@ -158,16 +161,23 @@ template <> struct RT<double> {
   __device__ static double pow_(double x, double y) { return pow(x, y); }
   __device__ static double fabs_(double x) { return fabs(x); }
 };
+// Mixed path: the hardware transcendentals (v_sin/v_cos on revolutions,
+// v_exp/v_log base 2; a few ulp over the ranges used here: angles in
+// [-2pi, 2pi], exponents of optical depths up to 25) in place of the
+// range-reduced library versions.  kernels_mixed.hip is also built with
+// -fno-hip-fp32-correctly-rounded-divide-sqrt, so FP32 division and sqrt use
+// v_rcp/v_sqrt.  The error this costs against the FP64 path is measured on
+// the SEDs, images and temperatures.
 template <> struct RT<float> {
   static constexpr float huge = FLT_MAX;
   __device__ static float sqrt_(float x) { return sqrtf(x); }
-  __device__ static float sin_(float x) { return sinf(x); }
-  __device__ static float cos_(float x) { return cosf(x); }
+  __device__ static float sin_(float x) { return __sinf(x); }
+  __device__ static float cos_(float x) { return __cosf(x); }
   __device__ static float acos_(float x) { return acosf(x); }
-  __device__ static float exp_(float x) { return expf(x); }
-  __device__ static float log_(float x) { return logf(x); }
+  __device__ static float exp_(float x) { return __expf(x); }
+  __device__ static float log_(float x) { return __logf(x); }
   __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
-  __device__ static float pow_(float x, float y) { return powf(x, y); }
+  __device__ static float pow_(float x, float y) { return __powf(x, y); }
   __device__ static float fabs_(float x) { return fabsf(x); }
 };
 
@@ -336,6 +346,44 @@ struct LocalCounts {
   __device__ __forceinline__ void add(int i, unsigned long long v = 1ull) const { atomicAdd(&s[i], v); }
 };
 
+// Direction of flight from (cost, sint, cosp, sinp).  FP64 path: the
+// reference's products.  Mixed path: FP32 angles give |u| = 1 only to ~1e-7
+// while the face roots assume |u| = 1 (over a 1e4 R* step that error moves a
+// crossing 1e-3 R* off its wall and trips testgrid), so u is renormalized in
+// FP64.
+template <class T> __device__ __forceinline__ void set_dir(Pkt<T>& p);
+template <> __device__ __forceinline__ void set_dir<double>(Pkt<double>& p) {
+  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+}
+template <> __device__ __forceinline__ void set_dir<float>(Pkt<float>& p) {
+  const double ux = (double)p.sint * (double)p.cosp, uy = (double)p.sint * (double)p.sinp, uz = (double)p.cost;
+  const double s = rsqrt(ux * ux + uy * uy + uz * uz);
+  p.ux = ux * s; p.uy = uy * s; p.uz = uz * s;
+}
+
+// A lower bound on the distance from the packet to every wall of its cell,
+// less a margin that covers the rounding of the face roots (the quadratics of
+// radface/thetaface lose at most ~sqrt(eps)*r on grazing rays).  Spheres:
+// r - r_in, r_out - r.  Cones: the quadratic of thetaface is the double cone
+// of half-angle t = min(theta, pi - theta), at distance |rho cos t - |z| sin t|
+// (the equator plane and the pole axis are its t = pi/2 and t = 0 cases).
+// Phi walls: the full planes through the z axis, at |sin(phi) x - cos(phi) y|.
+template <class T>
+__device__ __forceinline__ double wall_clearance(const DeviceModel& m, const Pkt<T>& p) {
+  const double r = p.rtot;
+  double d = fmin(r - m.rarr[p.ir - 1], m.rarr[p.ir] - r);
+  if (m.ntg > 2) {
+    const double rho = sqrt(p.xp * p.xp + p.yp * p.yp), az = fabs(p.zp);
+    d = fmin(d, fabs(rho * m.ccone[p.it - 1] - az * m.scone[p.it - 1]));
+    d = fmin(d, fabs(rho * m.ccone[p.it] - az * m.scone[p.it]));
+    if (m.npg > 2) {
+      d = fmin(d, fabs(m.aarr[p.ip - 1] * p.xp + m.barr[p.ip - 1] * p.yp));
+      d = fmin(d, fabs(m.aarr[p.ip] * p.xp + m.barr[p.ip] * p.yp));
+    }
+  }
+  return d - 1.0e-6 * r;
+}
+
 template <class T>
 __device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int order) {
 #pragma unroll
@@ -358,10 +406,22 @@ __device__ __forceinline__ Opac<T> opac_one(const DeviceModel& m, T nu, int id) 
   if (nu >= (T)nd[0]) { o.g = (T)gd[0]; o.pl = (T)pd[0]; o.kappa = (T)kd[0]; o.albedo = (T)ad[0]; }
   else if (nu <= (T)nd[nl - 1]) { o.g = (T)gd[nl - 1]; o.pl = (T)pd[nl - 1]; o.kappa = (T)kd[nl - 1]; o.albedo = (T)ad[nl - 1]; }
   else {
+    // The bisection of dust_5.f90:40-52 ends on the unique bracket
+    // nd(i0) >= nu > nd(i0+1) of the strictly decreasing table.  On a
+    // log-uniform table that index is guessed from log(nu) and stepped to
+    // the bracket, which gives the same i0 in one or two probes.
     int i0 = 1, i1 = nl, i;
-    while (i1 - i0 > 1) {
-      i = (i1 + i0) / 2;
-      if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
+    if (m.nud_ilog[id] > 0.0) {
+      int g = 1 + (int)(((double)m.nud_log0[id] - (double)RT<T>::log_(nu)) * m.nud_ilog[id]);
+      g = g < 1 ? 1 : (g > nl - 1 ? nl - 1 : g);
+      while (g > 1 && !(nu <= (T)nd[g - 1])) g--;
+      while (g < nl - 1 && nu <= (T)nd[g]) g++;
+      i0 = g; i1 = g + 1;
+    } else {
+      while (i1 - i0 > 1) {
+        i = (i1 + i0) / 2;
+        if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
+      }
     }
     T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
     T fm1 = (T)1 - f;

@@ -29,6 +29,24 @@ struct PeelQueue {
   unsigned capacity;
 };
 
+// Small read-only tables staged in LDS by every workgroup of the transport
+// and peel kernels: the walk reads walls at every face crossing and an
+// interaction searches the dust and Planck tables, so their latency sits on
+// every packet's serial chain.  The tables before LT_NUDUST are always staged
+// (unconditional LDS pointers let the compiler use ds_read); the five dust
+// tables are staged when they fit (n[LT_NUDUST] = 0: they stay in global
+// memory).
+enum LdsTab : int {
+  LT_RARR = 0, LT_R2ARR, LT_COSTARR, LT_TAN2ARR, LT_CCONE, LT_SCONE, LT_PHIARR, LT_AARR, LT_BARR,
+  LT_LTINT, LT_NUINT, LT_PEEL, LT_INCZETA,
+  LT_NUDUST, LT_KAPPAD, LT_ADUST, LT_GDUST, LT_PLDUST, LT_COUNT
+};
+struct LdsPlan {
+  uint32_t off[LT_COUNT];   // offset of table k in doubles
+  uint32_t n[LT_COUNT];     // doubles of table k
+  uint32_t bytes;           // dynamic LDS per workgroup
+};
+
 // Device control block of one run (16 bytes, zeroed by the host at the start).
 struct RunCtl {
   unsigned long long next_packet;   // packets handed out so far (run-relative)
@@ -52,7 +70,40 @@ struct RunParams {
   uint32_t round;          // 0 = first round of the run (no saved lanes)
   uint32_t regroup;        // waiting lanes of a wavefront that trigger an interaction / emission / tally pass
   RunCtl* ctl;
+  LdsPlan lds;
 };
+
+// Copy the planned tables into this workgroup's LDS and point the kernel's
+// model copy at them.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void stage_tables(DeviceModel& m, const double*& inczeta, const LdsPlan& P, double* lds) {
+  const double* src[LT_COUNT] = {m.rarr, m.r2arr, m.costarr, m.tan2arr, m.ccone, m.scone, m.phiarr, m.aarr, m.barr,
+                                 m.lTint, m.nuint, m.peel, inczeta,
+                                 m.nudust, m.kappad, m.adust, m.gdust, m.pldust};
+#pragma unroll
+  for (int k = 0; k < LT_COUNT; ++k)
+    for (uint32_t i = threadIdx.x; i < P.n[k]; i += blockDim.x) lds[P.off[k] + i] = src[k][i];
+  __syncthreads();
+  m.rarr = lds + P.off[LT_RARR];
+  m.r2arr = lds + P.off[LT_R2ARR];
+  m.costarr = lds + P.off[LT_COSTARR];
+  m.tan2arr = lds + P.off[LT_TAN2ARR];
+  m.ccone = lds + P.off[LT_CCONE];
+  m.scone = lds + P.off[LT_SCONE];
+  m.phiarr = lds + P.off[LT_PHIARR];
+  m.aarr = lds + P.off[LT_AARR];
+  m.barr = lds + P.off[LT_BARR];
+  m.lTint = lds + P.off[LT_LTINT];
+  m.nuint = lds + P.off[LT_NUINT];
+  m.peel = lds + P.off[LT_PEEL];
+  inczeta = lds + P.off[LT_INCZETA];
+  if (P.n[LT_NUDUST]) {
+    m.nudust = lds + P.off[LT_NUDUST];
+    m.kappad = lds + P.off[LT_KAPPAD];
+    m.adust = lds + P.off[LT_ADUST];
+    m.gdust = lds + P.off[LT_GDUST];
+    m.pldust = lds + P.off[LT_PLDUST];
+  }
+}
 
 template <class T>
 struct Ctx {
@@ -143,7 +194,7 @@ __device__ __forceinline__ void initpabs(Ctx<T>& C, Pkt<T>& p) {
   p.lp = RT<T>::atan2_((T)p.yp, (T)p.xp);
   initp_sample(C, p);
   initp_transform(p);
-  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  set_dir<T>(p);
   p.nub = bbfreq<T>(C.rng, (T)C.m.tstar, C.inczeta);
 }
 
@@ -165,7 +216,45 @@ struct Life {
   long long nseg;      // face crossings of the current flight
   int pir, pit, pip;   // cell the flight started in
   bool first;          // first flight after the emission
+  // per-cell values already read for the current cell (a packet trapped in
+  // an optically thick cell interacts there thousands of times): density,
+  // species and dust temperature, each with the cell index it belongs to
+  uint32_t c_rho, c_sp, c_td;
+  T rho;
+  int sp, spop;        // species code and its opacity table
+  T td;
 };
+
+template <class T>
+__device__ __forceinline__ void cell_cache_reset(Life<T>& L) { L.c_rho = L.c_sp = L.c_td = 0xFFFFFFFFu; }
+
+template <class T>
+__device__ __forceinline__ int cell_species(const DeviceModel& m, Life<T>& L, size_t c) {
+  if (L.c_sp != (uint32_t)c) { L.sp = m.species[c]; L.spop = m.species_opac[L.sp]; L.c_sp = (uint32_t)c; }
+  return L.sp;
+}
+
+template <class T>
+__device__ __forceinline__ T cell_tdust(const DeviceModel& m, Life<T>& L, size_t c) {
+  if (L.c_td != (uint32_t)c) { L.td = (T)m.tdust[c]; L.c_td = (uint32_t)c; }
+  return L.td;
+}
+
+// cell_opac through the cache
+template <class T>
+__device__ __forceinline__ T cell_opac_l(const DeviceModel& m, Life<T>& L, size_t c, int& iop, T& rho) {
+  if (L.c_rho != (uint32_t)c) { L.rho = cell_rho<T>(m, c); L.c_rho = (uint32_t)c; }
+  rho = L.rho;
+  const Pkt<T>& p = L.p;
+  if (m.nopac == 1) {
+    iop = rho > (T)0 ? 0 : -1;
+    return rho > (T)0 ? (T)0 + p.kapd[0] * rho : (T)0;
+  }
+  int sp = cell_species<T>(m, L, c);
+  iop = -1;
+  if (sp > 0 && rho > (T)0) { iop = m.species_opac[sp]; return (T)0 + pick(p.kapd, iop) * rho; }
+  return (T)0;
+}
 
 template <class T>
 __device__ __forceinline__ void flight_begin(Life<T>& L, T tau) {
@@ -183,13 +272,41 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   G dt;
   int iface;
   if (testgrid<T>(m, p, C.rng)) C.cnt.add(C_RESETS);
+  // In-cell fast path.  At the start of a flight (tsum = 0) off any wall, if
+  // the flight length tau / opac is below half the wall clearance, every face
+  // root is longer than twice the flight, so the reference's test
+  // opac * dt > tau holds whatever root closest_wall would pick (rounding
+  // included).  The interaction point is then computed exactly as below
+  // without solving for the faces.  This is the common case of packets
+  // trapped in optically thick cells.
+  if (!p.on_wall && L.tsum == (T)0) {
+    const size_t c = cell_index(m, p.ir, p.it, p.ip);
+    int iop;
+    T rho;
+    const T opac = cell_opac_l<T>(m, L, c, iop, rho);
+    if (opac > (T)0) {
+      const T ds = (L.tau - L.tsum) / opac;
+      if ((G)ds < 0.5 * wall_clearance<T>(m, p)) {
+        ++L.nseg;
+        C.cnt.add(C_SEGMENTS);
+        if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+        p.xp = p.xp + (G)ds * p.ux;
+        p.yp = p.yp + (G)ds * p.uy;
+        p.zp = p.zp + (G)ds * p.uz;
+        p.rsq = p.xp * p.xp + p.yp * p.yp + p.zp * p.zp;
+        p.rtot = sqrt(p.rsq);
+        p.on_wall = false;
+        return 1;
+      }
+    }
+  }
   if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) return -1;
   if (++L.nseg > m.max_segments) return -1;
   C.cnt.add(C_SEGMENTS);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
   T rho;
-  T opac = cell_opac<T>(m, p, c, iop, rho);
+  T opac = cell_opac_l<T>(m, L, c, iop, rho);
   T dtau = opac * (T)dt;
   if ((L.tsum + dtau) > L.tau) {
     T ds = (L.tau - L.tsum) / opac;
@@ -380,7 +497,7 @@ __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int
 // peel kernel walks every record in every observer direction.
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned slot, int sp, bool direct,
+__device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned slot, int sp, int iop, bool direct,
                                           T coslp, T sinlp) {
   PeelRec<T>* r = (PeelRec<T>*)C.q.recs + slot;
   r->xp = p.xp; r->yp = p.yp; r->zp = p.zp; r->rsq = p.rsq; r->rtot = p.rtot;
@@ -388,7 +505,7 @@ __device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned s
   if (direct) { r->cost = p.cosb; r->sint = p.sinb; r->phi = (T)0; r->cosp = coslp; r->sinp = sinlp; }
   else { r->cost = p.cost; r->sint = p.sint; r->phi = p.phi; r->cosp = p.cosp; r->sinp = p.sinp; }
   r->nub = p.nub; r->wave = p.wave;
-  const Opac<T> o = pick(p.op, C.m.species_opac[sp]);
+  const Opac<T> o = pick(p.op, iop);   // iop = the opacity table of species sp
   r->g = o.g; r->pl = o.pl;
 #pragma unroll
   for (int i = 0; i < kMaxOpac; ++i) r->kapd[i] = p.kapd[i];
@@ -398,10 +515,10 @@ __device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned s
 
 // scattering / re-emission peel (propagate_5_fuv.f90:83-90, :162-170)
 template <class T>
-__device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp) {
+__device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, int iop) {
   if (C.q.capacity == 0) return;          // no observer directions
   const unsigned slot = take_slot(C);
-  if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, false, (T)0, (T)0);
+  if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, iop, false, (T)0, (T)0);
   else C.cnt.add(C_ERRORS);              // the host sizes the queue so this cannot happen
 }
 
@@ -410,7 +527,7 @@ template <class T>
 __device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp, T sinlp) {
   if (C.q.capacity == 0) return;
   const unsigned slot = take_slot(C);
-  if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, true, coslp, sinlp);
+  if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, 0, true, coslp, sinlp);
   else C.cnt.add(C_ERRORS);
 }
 
@@ -465,7 +582,7 @@ __device__ __forceinline__ int flight_end(Ctx<T>& C, Life<T>& L) {
     }
   }
   (void)uniform<T>(C.rng);   // select_dust_2.f90:134 draw
-  L.idust2 = m.species[cell_index(m, p.ir, p.it, p.ip)];
+  L.idust2 = cell_species<T>(m, L, cell_index(m, p.ir, p.it, p.ip));
   if (L.first) C.cnt.add(C_TOT);
   L.first = false;
   return 0;
@@ -480,17 +597,18 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const T R2P = (T)kR2p;
   Pkt<T>& p = L.p;
   size_t c = cell_index(m, p.ir, p.it, p.ip);
-  int iop = m.species_opac[L.idust2];
+  // idust2 is the species of this cell (flight_end), so its opacity table is cached
+  int iop = (L.c_sp == (uint32_t)c) ? L.spop : m.species_opac[L.idust2];
   L.nint++;
   if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
     p.iflag = 1; L.iscat++; L.iint++;
-    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2);
+    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2, iop);
     stokes<T>(C, p, iop);
   } else {
     atomicAdd(&t.dbig[c], 1);
     p.i_orig = i_orig_of(L.idust2);
     atomicAdd(&t.nabs[c], 1);
-    T tave = (T)m.tdust[c];
+    T tave = cell_tdust<T>(m, L, c);
     p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
     p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
     p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
@@ -499,17 +617,17 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     p.sinp = RT<T>::sin_(p.phi);
     if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
     else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
-    p.uy = p.sint * p.sinp; p.ux = p.sint * p.cosp; p.uz = p.cost;
+    set_dir<T>(p);
     p.iflag = 0;
     opacset<T>(m, p.nub, p.op);
     p.wave = (T)1.2398 / p.nub;
     set_kapd(p, m, 1);
-    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2);
+    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2, iop);
     L.iint++;
   }
   T tau = -RT<T>::log_(uniform<T>(C.rng));
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
-  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  set_dir<T>(p);
   flight_begin(L, tau);
 }
 
@@ -588,13 +706,14 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   p.yp = rp * (G)sinlp;
   p.rsq = p.zp * p.zp + rp * rp;
   p.rtot = sqrt(p.rsq);
-  p.ux = p.sint * p.cosp; p.uy = p.sint * p.sinp; p.uz = p.cost;
+  set_dir<T>(p);
   p.ir = 1;
   p.on_wall = true;
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
   p.ip = locate<double>(m.phiarr, m.npg, (double)p.lp);
   if (m.ipeel == 1) peel_direct<T>(C, p, coslp, sinlp);
   // propagate_5_fuv.f90:20-40: the first flight
+  cell_cache_reset(L);
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
   L.iscat = 0; L.iint = 0; L.nint = 0;
   p.iflag = 0; p.exitflag = 0;
@@ -684,6 +803,7 @@ __device__ __forceinline__ void unpark(LaneIO& io, Ctx<T>& C, Life<T>& L, uint64
   rng_resume(C.rng, seed, first_packet + L.pid, n);
   opacset<T>(C.m, p.nub, p.op);
   L.first = false;
+  cell_cache_reset(L);
 }
 
 // Workgroup counters: zeroed at the start, flushed with one device atomic
@@ -729,10 +849,14 @@ __device__ __forceinline__ unsigned long long fetch_packets(unsigned long long* 
 enum : int { L_IDLE = 0, L_WALK = 1, L_READY = 2 };
 
 template <class T>
-__global__ void __launch_bounds__(256) k_transport(DeviceModel m, DeviceTallies t, RunParams rp) {
+__global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies t, RunParams rp) {
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
+  extern __shared__ double lds_tables[];
   counts_begin(s_cnt);
-  Ctx<T> C{m, t, {}, {s_cnt}, rp.inczeta, rp.q};
+  DeviceModel m = m0;
+  const double* inczeta = rp.inczeta;
+  stage_tables(m, inczeta, rp.lds, lds_tables);
+  Ctx<T> C{m, t, {}, {s_cnt}, inczeta, rp.q};
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   LaneIO io{rp.lane_words + lane, rp.lanes, 0};
   Life<T> L;
@@ -824,10 +948,14 @@ __device__ __forceinline__ bool peel_weight(const DeviceModel& m, const PeelRec<
 // (exponential weight, findangle3, stokespeel, image and SED binning) runs
 // once rp.regroup lanes of the wavefront are waiting for it, or nobody walks.
 template <class T>
-__global__ void __launch_bounds__(256) k_peel(DeviceModel m, DeviceTallies t, RunParams rp) {
+__global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, RunParams rp) {
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
+  extern __shared__ double lds_tables[];
   counts_begin(s_cnt);
-  Ctx<T> C{m, t, {}, {s_cnt}, rp.inczeta, PeelQueue{nullptr, nullptr, 0}};
+  DeviceModel m = m0;
+  const double* inczeta = rp.inczeta;
+  stage_tables(m, inczeta, rp.lds, lds_tables);
+  Ctx<T> C{m, t, {}, {s_cnt}, inczeta, PeelQueue{nullptr, nullptr, 0}};
   const unsigned nq = *rp.q.count;
   const uint64_t nrec = nq < rp.q.capacity ? nq : rp.q.capacity;
   const uint64_t npeel = (uint64_t)m.npeel;
