@@ -114,9 +114,10 @@ def main():
         tw = eng.last_run_ms() / 1e3
         eng.dusttemp(nw, update_tdust=True)
         log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s)")
-    rate_guess = nw / tw
-    N = args.packets or int(min(m.npackets, max(1_000_000, rate_guess * 15.0)))
-    N = (N // 1000) * 1000
+    # a fixed step size: the step time is a few seconds of bulk transport
+    # plus the serial tail of the longest packets, so a step size derived
+    # from a timed warm-up would vary from run to run with that tail
+    N = args.packets or int(min(m.npackets, 5_000_000))
     log(f"packets per step: {N}")
 
     seed = 20250101

@@ -535,7 +535,12 @@ double orc_lucydustfreq(const orc_model *m, double T0, int32_t iop, double xi)
     else if (P0 <= 0.0) f = 0.5;
     else if (P1 <= 0.0) { /* reference keeps f */ }
     else if (xi <= 0.0) { /* reference keeps f */ }
-    else f = log(xi / P0) / log(P1 / P0);
+    else {
+        /* P1/P0 rounds to 1 only when P0, P1 are adjacent doubles; the
+         * reference would divide 0 by 0 there, here f = 0 */
+        double den = log(P1 / P0);
+        f = den > 0.0 ? log(xi / P0) / den : 0.0;
+    }
     return T0 * nui[i0 - 1] * pow(nui[i1 - 1] / nui[i0 - 1], f);
 #undef KAP
 }

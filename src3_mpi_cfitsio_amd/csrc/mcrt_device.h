@@ -141,8 +141,11 @@ template <class T> __device__ __forceinline__ T uniform(Rng& r);
 template <> __device__ __forceinline__ double uniform<double>(Rng& r) {
   return ((double)rng_word(r) + 0.5) * 2.3283064365386963e-10;
 }
+// 23 bits: (k + 1/2) 2^-23 is exact in FP32 for every k < 2^23, so the
+// draw is strictly inside (0, 1).  (With 24 bits, k + 1/2 is not
+// representable for k >= 2^23: it rounds to even and k = 2^24 - 1 gives 1.)
 template <> __device__ __forceinline__ float uniform<float>(Rng& r) {
-  return ((float)(rng_word(r) >> 8) + 0.5f) * 5.9604644775390625e-8f;
+  return ((float)(rng_word(r) >> 9) + 0.5f) * 1.1920928955078125e-7f;
 }
 
 // ---------------------------------------------------------------------------
@@ -160,6 +163,8 @@ template <> struct RT<double> {
   __device__ static double atan2_(double y, double x) { return atan2(y, x); }
   __device__ static double pow_(double x, double y) { return pow(x, y); }
   __device__ static double fabs_(double x) { return fabs(x); }
+  __device__ static double log_acc(double x) { return log(x); }
+  __device__ static double tau_of(double u) { return -log(u); }
 };
 // Mixed path: the hardware transcendentals (v_sin/v_cos on revolutions,
 // v_exp/v_log base 2; a few ulp over the ranges used here: angles in
@@ -179,6 +184,12 @@ template <> struct RT<float> {
   __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
   __device__ static float pow_(float x, float y) { return __powf(x, y); }
   __device__ static float fabs_(float x) { return fabsf(x); }
+  // the library log where a ratio near 1 matters: v_log_f32's absolute
+  // error (~2^-22) can turn log(1 + 1e-7) into 0
+  __device__ static float log_acc(float x) { return logf(x); }
+  // optical depth of a flight, -log(u): the hardware log may return a tiny
+  // positive value for u just below 1, so the result is clamped at 0
+  __device__ static float tau_of(float u) { return fmaxf(-__logf(u), 0.0f); }
 };
 
 template <class T> __device__ __forceinline__ T cell_rho(const DeviceModel& m, size_t c);
@@ -525,7 +536,13 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
   else if (P0 <= (T)0) f = (T)0.5;
   else if (P1 <= (T)0) { }
   else if (xi <= (T)0) { }
-  else f = RT<T>::log_(xi / P0) / RT<T>::log_(P1 / P0);
+  else {
+    // lucydustfreq_2_test.f90:150.  P1/P0 can round to 1 when P0 and P1 are
+    // adjacent floating-point values (the top of a CDF row): f = 0 then,
+    // where the reference formula would divide 0 by 0.
+    const T den = RT<T>::log_acc(P1 / P0);
+    f = den > (T)0 ? RT<T>::log_acc(xi / P0) / den : (T)0;
+  }
   return T0 * (T)nui[i0 - 1] * RT<T>::pow_((T)nui[i1 - 1] / (T)nui[i0 - 1], f);
 #undef KAP
 }

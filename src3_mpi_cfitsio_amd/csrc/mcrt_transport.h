@@ -625,7 +625,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     if (m.ipeel == 1) peel_event<T>(C, p, L.idust2, iop);
     L.iint++;
   }
-  T tau = -RT<T>::log_(uniform<T>(C.rng));
+  T tau = RT<T>::tau_of(uniform<T>(C.rng));
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
   set_dir<T>(p);
   flight_begin(L, tau);
@@ -717,7 +717,7 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
   L.iscat = 0; L.iint = 0; L.nint = 0;
   p.iflag = 0; p.exitflag = 0;
-  T tau = -RT<T>::log_(uniform<T>(C.rng));
+  T tau = RT<T>::tau_of(uniform<T>(C.rng));
   L.first = true;
   flight_begin(L, tau);
 }
