@@ -125,6 +125,22 @@ int mcrt_get_tallies(mcrt_ctx *ctx, double *dtauabs, int32_t *nabs, int32_t *dbi
                      int32_t *killed, float *sed, float *psed, float *image,
                      double *counters);
 
+/* Adjoint of the peeled images (no counterpart in the reference; the
+ * gradient for model fitting).  For packets first_packet ..
+ * first_packet+npackets-1 of the run `seed` (the same Philox streams as
+ * mcrt_run_packets), *loss receives L = sum over the image pixels of
+ * W * image_I (W: nband*nx*nx*npeel floats in the image layout of
+ * mcrt_get_tallies, plane image_b_i) and, when grad is not NULL, grad[cell]
+ * receives dL/drho for every cell (FP64, the density of mcrt_set_medium).
+ * Each packet is run twice from its counters (path replay: a forward pass
+ * for its loss, a replay pass that scatters the derivative of every flight's
+ * sampling density and of every peel-off's exp(-tau)); no path is stored.
+ * The gradient is taken at the current dust temperature.  The run's own
+ * tallies are left untouched. */
+int mcrt_adjoint_image(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
+                       uint64_t seed, int32_t precision, const float *W,
+                       double *loss, double *grad);
+
 /* Lucy temperature (tfinal_5_test1.f90 + dusttemp_5.f90) from the device
  * dtauabs tally, FP64.  nphot = number of stellar packets of the iteration.
  * mode MCRT_T_REFERENCE: upper hemisphere at ip=1 averaged with its mirror,

@@ -47,6 +47,7 @@ SIGNATURES = {
     "mcrt_get_trace": (C.c_int, [_vp, _dp, C.c_int64]),
     "mcrt_last_run_ms": (C.c_int, [_vp, _fp]),
     "mcrt_run_stats": (C.c_int, [_vp, _dp]),
+    "mcrt_adjoint_image": (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int32, _fp, _dp, _dp]),
 }
 
 _lib = None

@@ -14,6 +14,11 @@ hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, c
 
 int lane_words_fp64() { return lane_words<double>(); }
 
+hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint<double>, dim3(grid), dim3(256), ap.lds.bytes, s, m, t, ap);
+  return hipGetLastError();
+}
+
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_peel<double>, dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();

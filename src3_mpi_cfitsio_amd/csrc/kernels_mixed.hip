@@ -13,6 +13,11 @@ hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, 
 
 int lane_words_mixed() { return lane_words<float>(); }
 
+hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_adjoint<float>, dim3(grid), dim3(256), ap.lds.bytes, s, m, t, ap);
+  return hipGetLastError();
+}
+
 hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_peel<float>, dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();

@@ -588,6 +588,64 @@ int mcrt_set_launch(mcrt_ctx* c, int64_t lanes, int32_t queue_per_lane, int32_t 
   return MCRT_OK;
 }
 
+int mcrt_adjoint_image(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64_t seed, int32_t precision,
+                       const float* W, double* loss, double* grad) {
+  if (!c) return MCRT_E_ARG;
+  if (!ready(c)) return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: grid, medium, dust, source, observer and tdust must be set");
+  if (npackets < 0 || first_packet < 0) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: negative packet range");
+  if (precision != MCRT_FP64 && precision != MCRT_MIXED) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: unknown precision");
+  if (!W || !loss) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: null W or loss");
+  if (c->dm.ipeel != 1 || c->dm.npeel <= 0 || c->dm.nband <= 0)
+    return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: the observer has no peeled images");
+  if (lds_always_doubles(c->dm) > kLdsAlways) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: tables exceed the LDS budget");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const bool fp64 = (precision == MCRT_FP64);
+  const size_t nw = (size_t)c->dm.nband * c->dm.npeel * c->dm.nx * c->dm.nx;
+  const float* dW = nullptr;
+  int rc = upload(c, "adj.W", W, nw, &dW);
+  if (rc) return rc;
+  void* p;
+  if ((rc = dev_alloc(c, "adj.loss", sizeof(double), &p))) return rc;
+  double* dloss = (double*)p;
+  double* dgrad = nullptr;
+  if (grad) {
+    if ((rc = dev_alloc(c, "adj.grad", c->ncell * sizeof(double), &p))) return rc;
+    dgrad = (double*)p;
+    HIPCHECK(c, launch_fill_f64(dgrad, c->ncell, 0.0, c->stream));
+  }
+  HIPCHECK(c, launch_fill_f64(dloss, 1, 0.0, c->stream));
+  // the transport code writes the Lucy tallies and counters: scratch copies
+  DeviceTallies t{};
+  if ((rc = dev_alloc(c, "adj.dtauabs", c->ncell * sizeof(double), &p))) return rc;
+  t.dtauabs = (double*)p;
+  if ((rc = dev_alloc(c, "adj.int3", 3 * c->ncell * sizeof(int), &p))) return rc;
+  t.nabs = (int*)p; t.dbig = t.nabs + c->ncell; t.killed = t.dbig + c->ncell;
+  if ((rc = dev_alloc(c, "adj.counters", C_NCOUNTERS * sizeof(unsigned long long), &p))) return rc;
+  t.counters = (unsigned long long*)p;
+  t.sed = t.psed = t.image = nullptr;
+  t.trace = nullptr;
+  AdjParams ap;
+  ap.first_packet = (uint64_t)first_packet;
+  ap.npackets = (uint64_t)npackets;
+  ap.seed = seed;
+  ap.inczeta = (const double*)c->buf["inczeta"];
+  ap.W = dW;
+  ap.grad = dgrad;
+  ap.loss = dloss;
+  ap.lds = plan_lds(c->dm);
+  if (npackets > 0) {
+    const int64_t blocks = (npackets + 255) / 256;
+    const int64_t cap = (int64_t)c->cu_count * 8;
+    const int grid = (int)(blocks < cap ? blocks : cap);
+    hipError_t e = fp64 ? launch_adjoint_fp64(c->dm, t, ap, grid, c->stream) : launch_adjoint_mixed(c->dm, t, ap, grid, c->stream);
+    if (e != hipSuccess) return hipfail(c, e, "adjoint kernel launch");
+  }
+  HIPCHECK(c, hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (grad) HIPCHECK(c, hipMemcpyAsync(grad, dgrad, c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
 int mcrt_run_stats(mcrt_ctx* c, double* out) {
   if (!c || !out) return MCRT_E_ARG;
   if (!c->ev_valid) return fail(c, MCRT_E_STATE, "mcrt_run_stats: no run recorded");

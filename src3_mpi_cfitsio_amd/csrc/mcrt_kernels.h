@@ -4,6 +4,7 @@
 // kernels_fp64.hip and kernels_mixed.hip.
 #pragma once
 #include "mcrt_transport.h"
+#include "mcrt_adjoint.h"
 
 namespace mcrt {
 

@@ -7,12 +7,15 @@ namespace mcrt {
 
 struct RunParams;
 struct TraceArgs;
+struct AdjParams;
 
 // rp.lanes must be a multiple of 256
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
 int lane_words_fp64();
 int lane_words_mixed();
+hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
+hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
 hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
 size_t peel_record_bytes_fp64();

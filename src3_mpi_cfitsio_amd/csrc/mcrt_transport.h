@@ -105,6 +105,18 @@ __device__ __forceinline__ void stage_tables(DeviceModel& m, const double*& incz
   }
 }
 
+// Path-replay adjoint of the peeled images (k_adjoint): the loss weights,
+// the gradient, and the bookkeeping of one packet's two passes.
+struct AdjState {
+  const float* W;      // loss weights in the image layout [nband][nx][nx][npeel]
+  double* grad;        // d loss / d rho per cell
+  double R;            // replay: loss of the packet's events still ahead
+  double total;        // forward: loss of the packet
+  int replay;          // 0 forward pass, 1 replay pass
+  uint64_t pkey;       // RNG key of the packet's peel walks
+  uint32_t nevent;     // peel-off events of the packet so far
+};
+
 template <class T>
 struct Ctx {
   const DeviceModel& m;
@@ -114,7 +126,16 @@ struct Ctx {
   const double* inczeta;
   PeelQueue q;
   unsigned slot_next, slot_end;   // this lane's reserved, not yet written queue slots
+  AdjState* adj;                  // null except in k_adjoint (the hooks then fold away)
 };
+
+template <class T> struct Life;
+template <class T> struct Pkt;
+// adjoint hooks (defined with k_adjoint)
+template <class T>
+__device__ __forceinline__ void adj_segment(Ctx<T>& C, size_t c, T kapd, G len, T rho, bool ends_here);
+template <class T>
+__device__ __forceinline__ void adj_peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, int iop, bool direct, T coslp, T sinlp);
 
 // Peel-off queue slots are reserved kSlotChunk at a time per lane, so an
 // interaction usually writes its record without waiting on an atomic.  Slots
@@ -265,7 +286,7 @@ __device__ __forceinline__ void flight_begin(Life<T>& L, T tau) {
 // One pass of tauint_3d_2_fuv.f90's face loop (:40-205).  0 = the flight
 // goes on, 1 = the interaction point was reached inside the cell, 2 = the
 // packet left the grid, -1 = walk error.
-template <class T>
+template <class T, bool kAdj = false>
 __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
@@ -290,6 +311,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
         ++L.nseg;
         C.cnt.add(C_SEGMENTS);
         if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+        if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), (G)ds, rho, true); }
         p.xp = p.xp + (G)ds * p.ux;
         p.yp = p.yp + (G)ds * p.uy;
         p.zp = p.zp + (G)ds * p.uz;
@@ -311,6 +333,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   if ((L.tsum + dtau) > L.tau) {
     T ds = (L.tau - L.tsum) / opac;
     if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+    if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), (G)ds, rho, true); }
     p.xp = p.xp + (G)ds * p.ux;
     p.yp = p.yp + (G)ds * p.uy;
     p.zp = p.zp + (G)ds * p.uz;
@@ -320,6 +343,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
     return 1;
   }
   if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+  if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), dt, rho, false); }
   cross_face(m, p.ir, p.it, p.ip, iface);
   p.xp = p.xp + dt * p.ux;
   p.yp = p.yp + dt * p.uy;
@@ -393,20 +417,23 @@ __device__ __forceinline__ bool peel_begin(const DeviceModel& m, const Pkt<T>& q
 }
 
 // :91-190.  0 = the walk goes on, 1 = it left the grid toward the observer,
-// 2 = it ended (tau > taumax, the star, or a walk error).
+// 2 = it ended (tau > taumax, the star, or a walk error).  rng: testgrid's
+// re-placements.  grad != null (adjoint): adds coef * kapd * length to every
+// crossed cell, the derivative of the walk's optical depth times coef.
 template <class T>
-__device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w) {
+__device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w, Rng& rng, double* grad = nullptr, double coef = 0.0) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = w.p;
   G dt;
   int iface;
-  if (testgrid<T>(m, p, C.rng)) C.cnt.add(C_RESETS);
+  if (testgrid<T>(m, p, rng)) C.cnt.add(C_RESETS);
   if (!next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface) || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
   C.cnt.add(C_PEELSEG);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
   T rho;
   T opac = cell_opac<T>(m, p, c, iop, rho);
+  if (grad && iop >= 0) atomicAdd(&grad[c], coef * (double)pick(p.kapd, iop) * (double)dt);
   T dtau = opac * (T)dt;
   w.tsum = w.tsum + dtau;
   if (w.tsum > (T)m.taumax) return 2;
@@ -422,43 +449,72 @@ __device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w) {
   return 0;
 }
 
-// :192-347.  q: the packet at the event; s0: its (weighted) Stokes I;
-// (xe, ye, ze): where the walk left the grid; tsum: its optical depth.
+// :192-240.  The Stokes vector a walk brings to the observer (exponential
+// weight, then stokespeel for scattered light) and where it lands.
+// q: the packet at the event; s0: its (weighted) Stokes I; (xe, ye, ze):
+// where the walk left the grid; tsum: its optical depth.
+template <class T>
+struct PeelHit {
+  T s[4];
+  G ximage, yimage;
+  int ix, iy;
+  T freq;
+};
+
+template <class T>
+__device__ __forceinline__ void peel_weigh(const DeviceModel& m, const Pkt<T>& q, T s0, int k, T gsc, T plsc,
+                                           G xe, G ye, G ze, T tsum, PeelHit<T>& h) {
+  const double* pe = m.peel + 5 * k;
+  const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
+  h.s[0] = s0; h.s[1] = q.sqp; h.s[2] = q.sup; h.s[3] = q.svp;
+  T factor = RT<T>::exp_(-tsum);
+  h.s[0] = h.s[0] * factor; h.s[1] = h.s[1] * factor; h.s[2] = h.s[2] * factor; h.s[3] = h.s[3] * factor;
+  if (q.iflag == 1) {
+    T bmu, sinbm, phinew, ri1;
+    findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
+    stokespeel<T>(gsc, plsc, h.s, coste, sinte, ri1, bmu, q.cost, q.sint);
+  }
+  h.yimage = ze * pe[1] - ye * pe[0] * pe[3] - xe * pe[0] * pe[2];
+  h.ximage = ye * pe[2] - xe * pe[3];
+  G rmaxi = m.rmaxi;
+  G fractxh = 2.0 * rmaxi / (G)m.nx;
+  h.ix = (int)((h.ximage + rmaxi) / fractxh) + 1;
+  h.iy = (int)((h.yimage + rmaxi) / fractxh) + 1;
+  h.freq = (T)2.9979e14 / q.wave;
+}
+
+// peeloff_3d_2_test.f90:245-260: the filter weight of band b at the walk's
+// frequency; false when the frequency is outside the filter
+template <class T>
+__device__ __forceinline__ bool band_weight(const DeviceModel& m, int b, T freq, T& wt) {
+  const double* fw = m.band_freq + (size_t)b * m.nwmax;
+  const double* fp = m.band_phi + (size_t)b * m.nwmax;
+  int nf = m.band_nw[b] - 1;
+  if (!(freq > (T)fw[0] && freq < (T)fw[nf - 1])) return false;
+  int iw = 1;
+  while (freq > (T)fw[iw - 1]) iw++;
+  iw = iw - 1;
+  wt = (T)fp[iw - 1];
+  return true;
+}
+
+// :241-347: image and peeled-SED binning
 template <class T>
 __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int k, T gsc, T plsc,
                                            G xe, G ye, G ze, T tsum) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
-  const double* pe = m.peel + 5 * k;
-  const T coste = (T)pe[0], sinte = (T)pe[1], cospe = (T)pe[2], sinpe = (T)pe[3], phie = (T)pe[4];
   C.cnt.add(C_PEELEVT);
-  T s[4] = {s0, q.sqp, q.sup, q.svp};
-  T factor = RT<T>::exp_(-tsum);
-  s[0] = s[0] * factor; s[1] = s[1] * factor; s[2] = s[2] * factor; s[3] = s[3] * factor;
-  if (q.iflag == 1) {
-    T bmu, sinbm, phinew, ri1;
-    findangle3<T>(q.cost, q.sint, q.phi, q.cosp, q.sinp, coste, sinte, phie, cospe, sinpe, bmu, sinbm, phinew, ri1);
-    stokespeel<T>(gsc, plsc, s, coste, sinte, ri1, bmu, q.cost, q.sint);
-  }
-  G yimage = ze * pe[1] - ye * pe[0] * pe[3] - xe * pe[0] * pe[2];
-  G ximage = ye * pe[2] - xe * pe[3];
-  G rmaxi = m.rmaxi;
-  G fractxh = 2.0 * rmaxi / (G)m.nx;
-  int ix = (int)((ximage + rmaxi) / fractxh) + 1;
-  int iy = (int)((yimage + rmaxi) / fractxh) + 1;
-  T freq = (T)2.9979e14 / q.wave;
+  PeelHit<T> h;
+  peel_weigh<T>(m, q, s0, k, gsc, plsc, xe, ye, ze, tsum, h);
+  const T* s = h.s;
+  const int ix = h.ix, iy = h.iy;
   if (m.nband > 0 && ix <= m.nx && iy <= m.nx && ix > 0 && iy > 0) {
     size_t npix = (size_t)m.npeel * m.nx * m.nx;
     size_t plane = (size_t)m.nband * npix;
     for (int b = 0; b < m.nband; ++b) {
-      const double* fw = m.band_freq + (size_t)b * m.nwmax;
-      const double* fp = m.band_phi + (size_t)b * m.nwmax;
-      int nf = m.band_nw[b] - 1;
-      if (freq > (T)fw[0] && freq < (T)fw[nf - 1]) {
-        int iw = 1;
-        while (freq > (T)fw[iw - 1]) iw++;
-        iw = iw - 1;
-        T wt = (T)fp[iw - 1];
+      T wt;
+      if (band_weight<T>(m, b, h.freq, wt)) {
         size_t o = (size_t)b * npix + (size_t)k + (size_t)m.npeel * ((size_t)(ix - 1) + (size_t)m.nx * (size_t)(iy - 1));
         atomicAdd(&t.image[0 * plane + o], (float)(s[0] * wt));
         atomicAdd(&t.image[1 * plane + o], (float)(s[1] * wt));
@@ -469,7 +525,7 @@ __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int
       }
     }
   }
-  G rho2 = ximage * ximage + yimage * yimage;
+  G rho2 = h.ximage * h.ximage + h.yimage * h.yimage;
   int inub = freq_bin<T>(m, q.nub);
   int i_inter = (q.iflag == 1) ? 2 : ((q.i_orig == 1) ? 1 : 3);
   int indx[3] = {1, 1 + q.i_orig, 5 + i_inter};
@@ -497,25 +553,32 @@ __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int
 // peel kernel walks every record in every observer direction.
 // ---------------------------------------------------------------------------
 template <class T>
+__device__ __forceinline__ void make_rec(PeelRec<T>& r, const Pkt<T>& p, int sp, int iop, bool direct, T coslp, T sinlp) {
+  r.xp = p.xp; r.yp = p.yp; r.zp = p.zp; r.rsq = p.rsq; r.rtot = p.rtot;
+  r.sip = p.sip; r.sqp = p.sqp; r.sup = p.sup; r.svp = p.svp;
+  if (direct) { r.cost = p.cosb; r.sint = p.sinb; r.phi = (T)0; r.cosp = coslp; r.sinp = sinlp; }
+  else { r.cost = p.cost; r.sint = p.sint; r.phi = p.phi; r.cosp = p.cosp; r.sinp = p.sinp; }
+  r.nub = p.nub; r.wave = p.wave;
+  const Opac<T> o = pick(p.op, iop);   // iop = the opacity table of species sp
+  r.g = o.g; r.pl = o.pl;
+#pragma unroll
+  for (int i = 0; i < kMaxOpac; ++i) r.kapd[i] = p.kapd[i];
+  r.cell = (uint32_t)p.ir | ((uint32_t)p.it << 10) | ((uint32_t)p.ip << 20) | ((uint32_t)(p.on_wall ? 1 : 0) << 30);
+  r.flags = (uint32_t)p.iflag | ((uint32_t)p.i_orig << 1) | ((uint32_t)(direct ? 1 : 0) << 4) | ((uint32_t)sp << 8);
+}
+
+template <class T>
 __device__ __forceinline__ void write_rec(Ctx<T>& C, const Pkt<T>& p, unsigned slot, int sp, int iop, bool direct,
                                           T coslp, T sinlp) {
-  PeelRec<T>* r = (PeelRec<T>*)C.q.recs + slot;
-  r->xp = p.xp; r->yp = p.yp; r->zp = p.zp; r->rsq = p.rsq; r->rtot = p.rtot;
-  r->sip = p.sip; r->sqp = p.sqp; r->sup = p.sup; r->svp = p.svp;
-  if (direct) { r->cost = p.cosb; r->sint = p.sinb; r->phi = (T)0; r->cosp = coslp; r->sinp = sinlp; }
-  else { r->cost = p.cost; r->sint = p.sint; r->phi = p.phi; r->cosp = p.cosp; r->sinp = p.sinp; }
-  r->nub = p.nub; r->wave = p.wave;
-  const Opac<T> o = pick(p.op, iop);   // iop = the opacity table of species sp
-  r->g = o.g; r->pl = o.pl;
-#pragma unroll
-  for (int i = 0; i < kMaxOpac; ++i) r->kapd[i] = p.kapd[i];
-  r->cell = (uint32_t)p.ir | ((uint32_t)p.it << 10) | ((uint32_t)p.ip << 20) | ((uint32_t)(p.on_wall ? 1 : 0) << 30);
-  r->flags = (uint32_t)p.iflag | ((uint32_t)p.i_orig << 1) | ((uint32_t)(direct ? 1 : 0) << 4) | ((uint32_t)sp << 8);
+  PeelRec<T> r;
+  make_rec<T>(r, p, sp, iop, direct, coslp, sinlp);
+  ((PeelRec<T>*)C.q.recs)[slot] = r;
 }
 
 // scattering / re-emission peel (propagate_5_fuv.f90:83-90, :162-170)
-template <class T>
+template <class T, bool kAdj = false>
 __device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, int iop) {
+  if constexpr (kAdj) { adj_peel_event<T>(C, p, sp, iop, false, (T)0, (T)0); return; }
   if (C.q.capacity == 0) return;          // no observer directions
   const unsigned slot = take_slot(C);
   if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, iop, false, (T)0, (T)0);
@@ -523,8 +586,9 @@ __device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, i
 }
 
 // direct stellar peel, weight 4 I cos(norm) (newdisktherm_5_fuv.f90:371-414)
-template <class T>
+template <class T, bool kAdj = false>
 __device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp, T sinlp) {
+  if constexpr (kAdj) { adj_peel_event<T>(C, p, 0, 0, true, coslp, sinlp); return; }
   if (C.q.capacity == 0) return;
   const unsigned slot = take_slot(C);
   if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, 0, true, coslp, sinlp);
@@ -590,7 +654,7 @@ __device__ __forceinline__ int flight_end(Ctx<T>& C, Life<T>& L) {
 
 // propagate_5_fuv.f90:62-198: one interaction (scatter, or absorb and
 // re-emit), its peel-off record, and the start of the next flight.
-template <class T>
+template <class T, bool kAdj = false>
 __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
@@ -602,7 +666,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   L.nint++;
   if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
     p.iflag = 1; L.iscat++; L.iint++;
-    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2, iop);
+    if (m.ipeel == 1) peel_event<T, kAdj>(C, p, L.idust2, iop);
     stokes<T>(C, p, iop);
   } else {
     atomicAdd(&t.dbig[c], 1);
@@ -622,7 +686,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     opacset<T>(m, p.nub, p.op);
     p.wave = (T)1.2398 / p.nub;
     set_kapd(p, m, 1);
-    if (m.ipeel == 1) peel_event<T>(C, p, L.idust2, iop);
+    if (m.ipeel == 1) peel_event<T, kAdj>(C, p, L.idust2, iop);
     L.iint++;
   }
   T tau = RT<T>::tau_of(uniform<T>(C.rng));
@@ -678,7 +742,7 @@ __device__ __forceinline__ void prop_end(Ctx<T>& C, Life<T>& L) {
 
 // newdisktherm_5_fuv.f90:294-413: one stellar photon (ips = 0, no spots),
 // its direct peel, and the start of its first flight.
-template <class T>
+template <class T, bool kAdj = false>
 __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
@@ -711,7 +775,7 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   p.on_wall = true;
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
   p.ip = locate<double>(m.phiarr, m.npg, (double)p.lp);
-  if (m.ipeel == 1) peel_direct<T>(C, p, coslp, sinlp);
+  if (m.ipeel == 1) peel_direct<T, kAdj>(C, p, coslp, sinlp);
   // propagate_5_fuv.f90:20-40: the first flight
   cell_cache_reset(L);
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
@@ -988,7 +1052,7 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
       item += stride;
     }
     if (st == P_WALK) {
-      const int r = peel_step<T>(C, w);
+      const int r = peel_step<T>(C, w, C.rng);
       if (r == 1) st = P_DONE;
       else if (r == 2) { st = P_IDLE; item += stride; }
     }

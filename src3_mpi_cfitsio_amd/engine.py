@@ -107,6 +107,28 @@ class Engine:
         self.set_tdust(m.tdust_initial())
         self._check(L.mcrt_zero_tallies(self.h))
 
+    def set_density(self, rho):
+        """Replace the cell densities (g/cm^3, cell_shape); species unchanged."""
+        m = self.model
+        r = _f64(np.asarray(rho, dtype=np.float64).reshape(m.cell_shape))
+        sp = np.ascontiguousarray(m.species, dtype=np.uint8)
+        so = np.ascontiguousarray(m.species_opac, dtype=np.int32)
+        self._check(self.lib.mcrt_set_medium(self.h, ptr(r, C.c_double), ptr(sp, C.c_uint8), ptr(so, C.c_int32)))
+
+    def adjoint_image(self, W, npackets: int, seed: int, precision: int = MCRT_MIXED, first_packet: int = 0,
+                      grad: bool = True):
+        """Loss sum(W * image_I) over the peeled images (W: nband x nx x nx x
+        npeel) and, with grad, dLoss/drho per cell (cell_shape), by path
+        replay (mcrt_adjoint_image)."""
+        m = self.model
+        Wf = np.ascontiguousarray(np.asarray(W, dtype=np.float32).reshape(m.nband, m.nx, m.nx, m.npeel))
+        loss = C.c_double()
+        g = np.zeros(m.cell_shape) if grad else None
+        self._check(self.lib.mcrt_adjoint_image(self.h, int(first_packet), int(npackets), C.c_uint64(int(seed)),
+                                                int(precision), ptr(Wf, C.c_float), C.byref(loss),
+                                                ptr(g, C.c_double) if grad else None))
+        return float(loss.value), g
+
     def set_tdust(self, tdust):
         t = _f64(tdust)
         self._check(self.lib.mcrt_set_tdust(self.h, ptr(t, C.c_double)))

@@ -98,6 +98,15 @@ module mcrt_mod
       integer(c_int32_t), value :: mode, update_tdust
       type(c_ptr), value :: tdave
     end function
+    ! loss sum(W * image_I) and d loss / d rho per cell (path replay)
+    integer(c_int) function mcrt_adjoint_image(ctx, first_packet, npackets, seed, precision, w, loss, grad) &
+         bind(C, name='mcrt_adjoint_image')
+      import :: c_int, c_ptr, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: first_packet, npackets, seed
+      integer(c_int32_t), value :: precision
+      type(c_ptr), value :: w, loss, grad
+    end function
   end interface
 
 contains
