@@ -10,6 +10,17 @@ A step is one Lucy iteration: zero the tallies, run the step's packets
 FP64 temperature inversion that the next step re-emits with.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+
+The default step is the workload's full Lucy iteration (10^8 packets on the
+200x180x64 model). At that size the bulk of the packets sets the step time;
+the serial chains of the few longest packets (up to ~10^6 interactions, a
+few seconds) overlap with it except at the very end.  The FP64 comparison
+runs a quarter of the step's packets in both precisions on identical RNG
+counters and dust temperatures.
+
+python bench.py --copy-probe BYTES only copies BYTES of device memory with
+torch (one copy kernel): the known-size kernel that scales rocprofv3's
+FETCH_SIZE / WRITE_SIZE readings to bytes.
 """
 import argparse
 import json
@@ -81,14 +92,24 @@ def cpu_baseline(m, tdust, seconds, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--packets", type=int, default=0, help="packets per step (default: auto)")
     ap.add_argument("--config", default="c1")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes on the GPU")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {x.numel() * 4} bytes")
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the path shards by packet ranges with one tally all-reduce per Lucy iteration; "
@@ -114,10 +135,8 @@ def main():
         tw = eng.last_run_ms() / 1e3
         eng.dusttemp(nw, update_tdust=True)
         log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s)")
-    # a fixed step size: the step time is a few seconds of bulk transport
-    # plus the serial tail of the longest packets, so a step size derived
-    # from a timed warm-up would vary from run to run with that tail
-    N = args.packets or int(min(m.npackets, 5_000_000))
+    # the workload's packets per Lucy iteration
+    N = args.packets or int(m.npackets)
     log(f"packets per step: {N}")
 
     seed = 20250101
@@ -162,7 +181,7 @@ def main():
     tdust_state = None
     extra = {}
     if not args.no_fp64:
-        Nc = max(1_000_000, N // 4)
+        Nc = min(N, max(1_000_000, N // 4))
         tfix = eng.dusttemp(N, update_tdust=False)
         eng.set_tdust(tfix)
         res = {}

@@ -141,6 +141,59 @@ int mcrt_adjoint_image(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
                        uint64_t seed, int32_t precision, const float *W,
                        double *loss, double *grad);
 
+/* ---- Batches of axisymmetric models in one persistent launch (no
+ * counterpart in the reference, where every model of a grid is a separate
+ * program run over the same packet loop).  The models share the context's
+ * dust, source and SED binning (mcrt_set_dust / mcrt_set_source /
+ * mcrt_set_observer with ipeel = 0: SEDs only); each has its own walls,
+ * medium, dust temperatures and tallies.  Arrays are the per-model arrays of
+ * mcrt_set_grid / mcrt_set_medium concatenated model after model: walls by
+ * nrg[k] (rarr, r2arr) and ntg[k] (thetarr, costarr, tan2arr), cells by
+ * (nrg[k]-1)*(ntg[k]-1).  Every model has one azimuthal cell: phiarr, aarr,
+ * barr hold its two walls (0 and 2*pi).  Packet p of model k runs with seed
+ * seed + k, so it follows the path it follows in a single-model run of
+ * model k with that seed. */
+int mcrt_set_batch(mcrt_ctx *ctx, int32_t nmodels, const int32_t *nrg, const int32_t *ntg,
+                   const double *rarr, const double *r2arr, const double *thetarr,
+                   const double *costarr, const double *tan2arr, const double *phiarr,
+                   const double *aarr, const double *barr, const double *vcell,
+                   const double *rho, const uint8_t *species, const int32_t *species_opac);
+/* dust temperatures of every model, concatenated cells */
+int mcrt_set_batch_tdust(mcrt_ctx *ctx, const double *tdust);
+/* Launch shape of mcrt_run_batch: block 64, 128 or 256 threads; resident
+ * waves per SIMD (1..8, clamped to what the kernel's registers allow);
+ * regroup = waiting lanes per wavefront that trigger an interaction /
+ * emission pass (1..64); combine != 0: write-combined dtauabs (one FP64
+ * atomic per cell visit instead of per segment); graph != 0: the run is a
+ * captured hipGraph (tally reset + launch), replayed while the shape and
+ * buffers stay the same.  0 keeps the current value (block, waves,
+ * regroup) — use mcrt_batch_autotune to choose them. */
+int mcrt_set_batch_launch(mcrt_ctx *ctx, int32_t block, int32_t waves_per_simd, int32_t regroup,
+                          int32_t combine, int32_t graph);
+/* One persistent launch over npackets[k] packets of every model k (zeroes
+ * the batch tallies first).  Models run most expensive first: by the
+ * interactions per packet of the previous run, before any run by the
+ * column density along the midplane. */
+int mcrt_run_batch(mcrt_ctx *ctx, const int64_t *npackets, uint64_t seed, int32_t precision);
+/* Batch tallies of the last run; any pointer may be NULL.  dtauabs, nabs,
+ * dbig, killed: concatenated cells; sed: nmodels blocks of the
+ * mcrt_get_tallies SED layout; work: interactions + packets per model;
+ * counters[16] (the mcrt_get_tallies counters) summed over the batch. */
+int mcrt_get_batch_tallies(mcrt_ctx *ctx, double *dtauabs, int32_t *nabs, int32_t *dbig,
+                           int32_t *killed, float *sed, double *work, double *counters);
+/* Lucy temperatures of every model from the last run (nphot = that
+ * model's packets), as mcrt_dusttemp; tdave: concatenated cells, may be NULL. */
+int mcrt_batch_dusttemp(mcrt_ctx *ctx, int32_t mode, int32_t update_tdust, double *tdave);
+/* Autotuner: times the batch kernel over block {64,128,256} x waves per SIMD
+ * {1,2,4} (where they fit) x regroup {16,32,48} x combine {0,1} on
+ * packets_per_model packets of every model, measuring the bulk rate (each
+ * trial stops when the queue is handed out), and keeps the fastest shape.
+ * best[4] = block, waves, regroup, combine; table (may be NULL) receives up
+ * to max_rows rows of 6 doubles (the four, ms, packets/s); *nrows = rows
+ * tried.  Tallies of the trials are void. */
+int mcrt_batch_autotune(mcrt_ctx *ctx, int64_t packets_per_model, uint64_t seed, int32_t precision,
+                        int32_t *best, double *table, int32_t max_rows, int32_t *nrows);
+
 /* Lucy temperature (tfinal_5_test1.f90 + dusttemp_5.f90) from the device
  * dtauabs tally, FP64.  nphot = number of stellar packets of the iteration.
  * mode MCRT_T_REFERENCE: upper hemisphere at ip=1 averaged with its mirror,

@@ -48,6 +48,13 @@ SIGNATURES = {
     "mcrt_last_run_ms": (C.c_int, [_vp, _fp]),
     "mcrt_run_stats": (C.c_int, [_vp, _dp]),
     "mcrt_adjoint_image": (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int32, _fp, _dp, _dp]),
+    "mcrt_set_batch": (C.c_int, [_vp, C.c_int32, _ip, _ip] + [_dp] * 10 + [_u8p, _ip]),
+    "mcrt_set_batch_tdust": (C.c_int, [_vp, _dp]),
+    "mcrt_set_batch_launch": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "mcrt_run_batch": (C.c_int, [_vp, C.POINTER(C.c_int64), C.c_uint64, C.c_int32]),
+    "mcrt_get_batch_tallies": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _fp, _dp, _dp]),
+    "mcrt_batch_dusttemp": (C.c_int, [_vp, C.c_int32, C.c_int32, _dp]),
+    "mcrt_batch_autotune": (C.c_int, [_vp, C.c_int64, C.c_uint64, C.c_int32, _ip, _dp, C.c_int32, _ip]),
 }
 
 _lib = None

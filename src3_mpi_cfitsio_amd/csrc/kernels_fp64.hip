@@ -26,6 +26,41 @@ hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const 
 
 size_t peel_record_bytes_fp64() { return sizeof(PeelRec<double>); }
 
+template <int BS, bool kComb>
+static hipError_t batch_launch_fp64(const BatchLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch<double, BS, kComb>), dim3(L.grid), dim3(BS), L.lds.bytes, s, L.m, L.t, L.bd, L.run, L.inczeta, L.lds);
+  return hipGetLastError();
+}
+
+template <int BS, bool kComb>
+static hipError_t batch_occ_fp64(const BatchLaunch& L, int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<double, BS, kComb>, BS, L.lds.bytes);
+}
+
+hipError_t launch_batch_fp64(const BatchLaunch& L, hipStream_t s) {
+  switch (L.block * 2 + (L.combine ? 1 : 0)) {
+    case 128: return batch_launch_fp64<64, false>(L, s);
+    case 129: return batch_launch_fp64<64, true>(L, s);
+    case 256: return batch_launch_fp64<128, false>(L, s);
+    case 257: return batch_launch_fp64<128, true>(L, s);
+    case 512: return batch_launch_fp64<256, false>(L, s);
+    case 513: return batch_launch_fp64<256, true>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t batch_occupancy_fp64(const BatchLaunch& L, int* blocks_per_cu) {
+  switch (L.block * 2 + (L.combine ? 1 : 0)) {
+    case 128: return batch_occ_fp64<64, false>(L, blocks_per_cu);
+    case 129: return batch_occ_fp64<64, true>(L, blocks_per_cu);
+    case 256: return batch_occ_fp64<128, false>(L, blocks_per_cu);
+    case 257: return batch_occ_fp64<128, true>(L, blocks_per_cu);
+    case 512: return batch_occ_fp64<256, false>(L, blocks_per_cu);
+    case 513: return batch_occ_fp64<256, true>(L, blocks_per_cu);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_trace<double>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
   return hipGetLastError();
@@ -46,6 +81,22 @@ hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nph
     hipLaunchKernelGGL(k_tfinal_fill, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, m, tdave);
   } else {
     hipLaunchKernelGGL(k_tcell, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, s, m, dtauabs, nphot, tdave);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* npk, int mode, double* tdave, size_t max_cells,
+                               hipStream_t s) {
+  const unsigned nm = (unsigned)L.bd.nmodels;
+  const unsigned gx = (unsigned)((max_cells + 255) / 256);
+  if (nm == 0 || gx == 0) return hipSuccess;
+  if (mode == 0) {
+    hipLaunchKernelGGL(k_batch_tfinal_half, dim3(gx, nm), dim3(256), 0, s, L.m, L.bd, npk, (const double*)L.t.dtauabs, tdave);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_tfinal_fill, dim3(gx, nm), dim3(256), 0, s, L.m, L.bd, npk, tdave);
+  } else {
+    hipLaunchKernelGGL(k_batch_tcell, dim3(gx, nm), dim3(256), 0, s, L.m, L.bd, npk, (const double*)L.t.dtauabs, tdave);
   }
   return hipGetLastError();
 }

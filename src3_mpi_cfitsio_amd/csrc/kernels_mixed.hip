@@ -25,6 +25,41 @@ hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const
 
 size_t peel_record_bytes_mixed() { return sizeof(PeelRec<float>); }
 
+template <int BS, bool kComb>
+static hipError_t batch_launch_mixed(const BatchLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch<float, BS, kComb>), dim3(L.grid), dim3(BS), L.lds.bytes, s, L.m, L.t, L.bd, L.run, L.inczeta, L.lds);
+  return hipGetLastError();
+}
+
+template <int BS, bool kComb>
+static hipError_t batch_occ_mixed(const BatchLaunch& L, int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_batch<float, BS, kComb>, BS, L.lds.bytes);
+}
+
+hipError_t launch_batch_mixed(const BatchLaunch& L, hipStream_t s) {
+  switch (L.block * 2 + (L.combine ? 1 : 0)) {
+    case 128: return batch_launch_mixed<64, false>(L, s);
+    case 129: return batch_launch_mixed<64, true>(L, s);
+    case 256: return batch_launch_mixed<128, false>(L, s);
+    case 257: return batch_launch_mixed<128, true>(L, s);
+    case 512: return batch_launch_mixed<256, false>(L, s);
+    case 513: return batch_launch_mixed<256, true>(L, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t batch_occupancy_mixed(const BatchLaunch& L, int* blocks_per_cu) {
+  switch (L.block * 2 + (L.combine ? 1 : 0)) {
+    case 128: return batch_occ_mixed<64, false>(L, blocks_per_cu);
+    case 129: return batch_occ_mixed<64, true>(L, blocks_per_cu);
+    case 256: return batch_occ_mixed<128, false>(L, blocks_per_cu);
+    case 257: return batch_occ_mixed<128, true>(L, blocks_per_cu);
+    case 512: return batch_occ_mixed<256, false>(L, blocks_per_cu);
+    case 513: return batch_occ_mixed<256, true>(L, blocks_per_cu);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_trace<float>, dim3((a.n + 255) / 256), dim3(256), 0, s, m, a);
   return hipGetLastError();

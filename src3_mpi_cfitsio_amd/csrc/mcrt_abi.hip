@@ -39,6 +39,27 @@ struct mcrt_ctx {
   RunCtl* host_ctl = nullptr;             // pinned read-back of the run control block
   hipEvent_t evr[3] = {nullptr, nullptr, nullptr};   // per-round kernel bracketing
   double stats[MCRT_NSTATS] = {0};        // mcrt_run_stats of the last run
+  // batch of axisymmetric models (mcrt_set_batch)
+  bool has_batch = false, has_batch_tdust = false;
+  int nmodels = 0;
+  std::vector<int> b_nrg, b_ntg;
+  std::vector<long long> b_coff;
+  size_t b_ncell = 0, b_max_cells = 0, b_nsed = 0;
+  BatchDesc bd{};
+  DeviceTallies bt{};
+  const int* b_species_opac = nullptr;
+  BatchRun* b_run = nullptr;              // device
+  BatchRun* b_run_h = nullptr;            // pinned staging
+  std::vector<int> b_order;
+  std::vector<unsigned long long> b_prefix, b_npk, b_work;
+  std::vector<double> b_cost;             // per-packet cost of each model (ordering)
+  int b_block = 256, b_waves = 2, b_regroup = 32, b_combine = 0, b_graph = 1;
+  hipGraphExec_t b_exec = nullptr;        // captured run of the current shape
+  unsigned long long b_exec_key[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long b_gen = 1;           // bumped by every call that changes what a launch embeds
+  const double *b_phiarr = nullptr, *b_aarr = nullptr, *b_barr = nullptr;
+  unsigned long long* b_npk_dev = nullptr;
+  bool b_ran = false;
 };
 
 namespace {
@@ -225,6 +246,8 @@ int mcrt_destroy(mcrt_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
   if (c->host_ctl) (void)hipHostFree(c->host_ctl);
+  if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
+  if (c->b_run_h) (void)hipHostFree(c->b_run_h);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return MCRT_OK;
@@ -233,6 +256,7 @@ int mcrt_destroy(mcrt_ctx* c) {
 const char* mcrt_last_error(const mcrt_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
 int mcrt_set_stream(mcrt_ctx* c, void* stream) {
+  if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->stream = stream ? (hipStream_t)stream : c->own_stream;
@@ -332,6 +356,7 @@ int mcrt_set_medium(mcrt_ctx* c, const double* rho, const uint8_t* species, cons
 int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust, const double* kappad, const double* adust,
                   const double* gdust, const double* pldust, const double* kappav, int32_t nT, int32_t nnu,
                   const double* lTint, const double* Tint, const double* nuint, const double* kapint, const double* kappap) {
+  if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
   if (nopac < 1 || nopac > kMaxOpac) return fail(c, MCRT_E_ARG, "mcrt_set_dust: nopac must be 1..4");
   if (nlam < 2 || nT < 2 || nnu < 2) return fail(c, MCRT_E_ARG, "mcrt_set_dust: nlam, nT, nnu must be >= 2");
@@ -377,6 +402,7 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
 }
 
 int mcrt_set_source(mcrt_ctx* c, double rstar, double tstar, double rmin) {
+  if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
   if (!(rstar > 0.0) || !(tstar > 0.0) || !(rmin > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_source: rstar, tstar, rmin must be > 0");
   c->dm.rstar = rstar; c->dm.tstar = tstar; c->dm.rmin = rmin;
@@ -388,6 +414,7 @@ int mcrt_set_observer(mcrt_ctx* c, int32_t ipeel, int32_t npeel, const double* t
                       double rmaxi, int32_t nband, int32_t nwmax, const int32_t* band_nw, const double* band_freq,
                       const double* band_phi, int32_t nfreq, int32_t nmu, int32_t nph, int32_t nap,
                       const double* aperture2, const double* u_mu, double numin, double lnurat) {
+  if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
   if (npeel < 0 || (ipeel == 1 && npeel < 1) || nx < 1 || nband < 0 || nfreq < 1 || nmu < 1 || nph < 1 || nap < 1)
     return fail(c, MCRT_E_ARG, "mcrt_set_observer: bad sizes");
@@ -643,6 +670,422 @@ int mcrt_adjoint_image(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint
   HIPCHECK(c, hipMemcpyAsync(loss, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (grad) HIPCHECK(c, hipMemcpyAsync(grad, dgrad, c->ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// LDS plan of the batch kernel: only the tables every model shares
+LdsPlan plan_lds_shared(const DeviceModel& m) {
+  LdsPlan P{};
+  const uint32_t npg = (uint32_t)m.npg, nd = (uint32_t)(m.nopac * m.nlam);
+  const uint32_t sizes[LT_COUNT] = {0u, 0u, 0u, 0u, 0u, 0u, npg, npg, npg,
+                                    (uint32_t)m.nT, (uint32_t)m.nnu, 5u * (uint32_t)(m.npeel > 0 ? m.npeel : 1), 145u,
+                                    nd, nd, nd, nd, nd};
+  uint32_t used = 0;
+  for (int k = 0; k < LT_COUNT; ++k) {
+    if (k == LT_NUDUST && used + 5u * nd > kLdsMax) break;
+    P.off[k] = used;
+    P.n[k] = sizes[k];
+    used += sizes[k];
+  }
+  P.bytes = used * 8u;
+  return P;
+}
+
+BatchLaunch batch_launch(mcrt_ctx* c, int block, bool combine) {
+  BatchLaunch L;
+  L.m = c->dm;
+  L.m.npg = 2;
+  L.m.phiarr = c->b_phiarr;
+  L.m.aarr = c->b_aarr;
+  L.m.barr = c->b_barr;
+  {
+    const double e = std::sin(0.5 * 1.0e-4 * 2.0 * M_PI);
+    L.m.phi_tol2 = e * e;
+  }
+  L.m.species_opac = c->b_species_opac;
+  L.t = c->bt;
+  L.bd = c->bd;
+  L.run = c->b_run;
+  L.inczeta = (const double*)c->buf["inczeta"];
+  L.lds = plan_lds_shared(L.m);
+  L.block = block;
+  L.combine = combine;
+  L.grid = 0;
+  return L;
+}
+
+// grid of `waves` resident waves per SIMD on every CU, at most what the
+// kernel's registers and LDS allow; returns the waves per SIMD launched
+int batch_grid(mcrt_ctx* c, bool fp64, BatchLaunch& L, int waves) {
+  int bpc = 0;
+  hipError_t e = fp64 ? batch_occupancy_fp64(L, &bpc) : batch_occupancy_mixed(L, &bpc);
+  if (e != hipSuccess || bpc <= 0) return -1;
+  int want = waves * 4 * 64 / L.block;
+  if (want < 1) want = 1;
+  const int per_cu = want < bpc ? want : bpc;
+  L.grid = (unsigned)(per_cu * c->cu_count);
+  return per_cu * L.block / 256 > 0 ? per_cu * L.block / 256 : 1;
+}
+
+bool batch_ready(mcrt_ctx* c) {
+  return c->has_batch && c->has_batch_tdust && c->has_dust && c->has_source && c->has_observer;
+}
+
+int batch_zero(mcrt_ctx* c, hipStream_t s) {
+  HIPCHECK(c, launch_fill_f64(c->bt.dtauabs, c->b_ncell, 0.0, s));
+  HIPCHECK(c, launch_fill_i32(c->bt.nabs, 3 * c->b_ncell, 0, s));
+  HIPCHECK(c, launch_fill_f32(c->bt.sed, (size_t)c->nmodels * c->b_nsed, 0.f, s));
+  HIPCHECK(c, launch_fill_u64(c->bt.counters, C_NCOUNTERS, 0ull, s));
+  HIPCHECK(c, launch_fill_u64(c->bd.work, (size_t)c->nmodels, 0ull, s));
+  return MCRT_OK;
+}
+
+// Queue order (most expensive model first) and packet prefix for npk, then
+// the run block; trial runs stop once every packet is handed out.
+int batch_stage(mcrt_ctx* c, const int64_t* npk, uint64_t seed, bool trial) {
+  const int nm = c->nmodels;
+  c->b_npk.assign(nm, 0ull);
+  for (int k = 0; k < nm; ++k) c->b_npk[k] = (unsigned long long)npk[k];
+  c->b_order.resize(nm);
+  for (int k = 0; k < nm; ++k) c->b_order[k] = k;
+  std::stable_sort(c->b_order.begin(), c->b_order.end(), [&](int a, int b) {
+    return c->b_cost[a] * (double)c->b_npk[a] > c->b_cost[b] * (double)c->b_npk[b];
+  });
+  c->b_prefix.assign(nm + 1, 0ull);
+  for (int j = 0; j < nm; ++j) c->b_prefix[j + 1] = c->b_prefix[j] + c->b_npk[c->b_order[j]];
+  hipStream_t s = c->stream;
+  HIPCHECK(c, hipMemcpyAsync((void*)c->bd.order, c->b_order.data(), nm * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync((void*)c->bd.prefix, c->b_prefix.data(), (nm + 1) * sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+  HIPCHECK(c, hipMemcpyAsync(c->b_npk_dev, c->b_npk.data(), nm * sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+  BatchRun& r = *c->b_run_h;
+  r.seed = seed;
+  r.total = c->b_prefix[nm];
+  r.head = 0ull;
+  r.regroup = (uint32_t)c->b_regroup;
+  r.trial = trial ? 1u : 0u;
+  HIPCHECK(c, hipMemcpyAsync(c->b_run, c->b_run_h, sizeof(BatchRun), hipMemcpyHostToDevice, s));
+  return MCRT_OK;
+}
+
+int batch_launch_direct(mcrt_ctx* c, bool fp64, const BatchLaunch& L, hipStream_t s) {
+  int rc = batch_zero(c, s);
+  if (rc) return rc;
+  hipError_t e = fp64 ? launch_batch_fp64(L, s) : launch_batch_mixed(L, s);
+  if (e != hipSuccess) return hipfail(c, e, "batch kernel launch");
+  return MCRT_OK;
+}
+
+// The run as a graph: tally reset (fill kernels) and the batch kernel,
+// captured once per launch shape and replayed.
+int batch_launch_graph(mcrt_ctx* c, bool fp64, const BatchLaunch& L) {
+  const unsigned long long key[6] = {c->b_gen, fp64 ? 1ull : 0ull, (unsigned long long)L.block,
+                                     L.combine ? 1ull : 0ull, (unsigned long long)L.grid, (unsigned long long)(uintptr_t)c->stream};
+  if (!c->b_exec || std::memcmp(key, c->b_exec_key, sizeof(key)) != 0) {
+    if (c->b_exec) { HIPCHECK(c, hipGraphExecDestroy(c->b_exec)); c->b_exec = nullptr; }
+    HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_launch_direct(c, fp64, L, c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hipfail(c, e, "batch graph capture");
+    e = hipGraphInstantiate(&c->b_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { c->b_exec = nullptr; return hipfail(c, e, "batch graph instantiate"); }
+    std::memcpy(c->b_exec_key, key, sizeof(key));
+  }
+  HIPCHECK(c, hipGraphLaunch(c->b_exec, c->stream));
+  return MCRT_OK;
+}
+
+int batch_run(mcrt_ctx* c, const int64_t* npk, uint64_t seed, bool fp64, bool trial, int block, int waves, bool combine,
+              bool graph, float* ms) {
+  int rc = batch_stage(c, npk, seed, trial);
+  if (rc) return rc;
+  BatchLaunch L = batch_launch(c, block, combine);
+  if (batch_grid(c, fp64, L, waves) < 0) return fail(c, MCRT_E_ARG, "batch launch shape does not fit the device");
+  HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
+  rc = graph ? batch_launch_graph(c, fp64, L) : batch_launch_direct(c, fp64, L, c->stream);
+  if (rc) return rc;
+  HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->ev_valid = true;
+  if (ms) HIPCHECK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return MCRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcrt_set_batch(mcrt_ctx* c, int32_t nmodels, const int32_t* nrg, const int32_t* ntg, const double* rarr,
+                   const double* r2arr, const double* thetarr, const double* costarr, const double* tan2arr,
+                   const double* phiarr, const double* aarr, const double* barr, const double* vcell, const double* rho,
+                   const uint8_t* species, const int32_t* species_opac) {
+  if (!c) return MCRT_E_ARG;
+  if (nmodels < 1 || !nrg || !ntg || !rarr || !r2arr || !thetarr || !costarr || !tan2arr || !phiarr || !aarr || !barr ||
+      !vcell || !rho || !species || !species_opac)
+    return fail(c, MCRT_E_ARG, "mcrt_set_batch: need nmodels >= 1 and all arrays");
+  if (std::fabs(phiarr[0]) > 1e-12 || std::fabs(phiarr[1] - 2.0 * M_PI) > 1e-9)
+    return fail(c, MCRT_E_ARG, "mcrt_set_batch: the azimuthal walls must be 0 and 2*pi");
+  for (int i = 0; i <= 10; ++i)
+    if (species_opac[i] < 0 || species_opac[i] >= kMaxOpac) return fail(c, MCRT_E_ARG, "mcrt_set_batch: species_opac out of range");
+  std::vector<long long> roff(nmodels), toff(nmodels), coff(nmodels);
+  long long nr = 0, nt = 0, nc = 0;
+  size_t maxc = 0;
+  for (int k = 0; k < nmodels; ++k) {
+    if (nrg[k] < 3 || ntg[k] < 2 || nrg[k] > 1023 || ntg[k] > 1023 || (ntg[k] > 2 && ntg[k] % 2 == 0))
+      return fail(c, MCRT_E_ARG, "mcrt_set_batch: model walls need 3 <= nrg <= 1023 and odd 3 <= ntg <= 1023 (or 2)");
+    roff[k] = nr; toff[k] = nt; coff[k] = nc;
+    nr += nrg[k];
+    nt += ntg[k];
+    const size_t cells = (size_t)(nrg[k] - 1) * (ntg[k] - 1);
+    nc += (long long)cells;
+    if (cells > maxc) maxc = cells;
+  }
+  if (nc >= (1ll << 31)) return fail(c, MCRT_E_ARG, "mcrt_set_batch: more than 2^31 cells");
+  for (long long i = 0; i < nc; ++i) {
+    const int sp = species[i];
+    if (sp > 10) return fail(c, MCRT_E_ARG, "mcrt_set_batch: species code above 10");
+    if (sp >= 5 && sp <= 8 && rho[i] > 0.0) return fail(c, MCRT_E_ARG, "mcrt_set_batch: small-grain species (5-8) are not supported");
+    if (!(rho[i] >= 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_batch: negative or NaN density");
+    if ((rho[i] > 0.0) != (sp > 0)) return fail(c, MCRT_E_ARG, "mcrt_set_batch: a cell has density without species or species without density");
+  }
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_batch = c->has_batch_tdust = false;
+  c->b_gen++;
+  BatchDesc& b = c->bd;
+  b.nmodels = nmodels;
+  int rc = 0;
+  rc |= upload(c, "b.nrg", (const int*)nrg, nmodels, &b.nrg);
+  rc |= upload(c, "b.ntg", (const int*)ntg, nmodels, &b.ntg);
+  rc |= upload(c, "b.roff", roff.data(), nmodels, &b.roff);
+  rc |= upload(c, "b.toff", toff.data(), nmodels, &b.toff);
+  rc |= upload(c, "b.coff", coff.data(), nmodels, &b.coff);
+  rc |= upload(c, "b.rarr", rarr, nr, &b.rarr);
+  rc |= upload(c, "b.r2arr", r2arr, nr, &b.r2arr);
+  rc |= upload(c, "b.thetarr", thetarr, nt, &b.thetarr);
+  rc |= upload(c, "b.costarr", costarr, nt, &b.costarr);
+  rc |= upload(c, "b.tan2arr", tan2arr, nt, &b.tan2arr);
+  rc |= upload(c, "b.vcell", vcell, nc, &b.vcell);
+  rc |= upload(c, "b.rho64", rho, nc, &b.rho64);
+  rc |= upload_cast<float>(c, "b.rho32", rho, nc, &b.rho32);
+  rc |= upload(c, "b.species", species, nc, &b.species);
+  rc |= upload(c, "b.species_opac", (const int*)species_opac, 11, &c->b_species_opac);
+  rc |= upload(c, "b.phiarr", phiarr, 2, &c->b_phiarr);
+  rc |= upload(c, "b.aarr", aarr, 2, &c->b_aarr);
+  rc |= upload(c, "b.barr", barr, 2, &c->b_barr);
+  std::vector<double> cc(nt), sc(nt);
+  for (long long i = 0; i < nt; ++i) {
+    const double t = std::min(thetarr[i], M_PI - thetarr[i]);
+    cc[i] = std::cos(t);
+    sc[i] = std::sin(t);
+  }
+  rc |= upload(c, "b.ccone", cc.data(), nt, &b.ccone);
+  rc |= upload(c, "b.scone", sc.data(), nt, &b.scone);
+  if (rc) return rc;
+  void* p;
+  if ((rc = dev_alloc(c, "b.tdust", nc * 8, &p))) return rc;
+  b.tdust = (const double*)p;
+  if ((rc = dev_alloc(c, "b.order", nmodels * sizeof(int), &p))) return rc;
+  b.order = (const int*)p;
+  if ((rc = dev_alloc(c, "b.prefix", (nmodels + 1) * sizeof(unsigned long long), &p))) return rc;
+  b.prefix = (const unsigned long long*)p;
+  if ((rc = dev_alloc(c, "b.work", nmodels * sizeof(unsigned long long), &p))) return rc;
+  b.work = (unsigned long long*)p;
+  if ((rc = dev_alloc(c, "b.npk", nmodels * sizeof(unsigned long long), &p))) return rc;
+  c->b_npk_dev = (unsigned long long*)p;
+  if ((rc = dev_alloc(c, "b.run", sizeof(BatchRun), &p))) return rc;
+  c->b_run = (BatchRun*)p;
+  if (!c->b_run_h && hipHostMalloc((void**)&c->b_run_h, sizeof(BatchRun), hipHostMallocDefault) != hipSuccess) {
+    c->b_run_h = nullptr;
+    return fail(c, MCRT_E_HIP, "mcrt_set_batch: pinned host allocation");
+  }
+  // tallies
+  c->nmodels = nmodels;
+  c->b_ncell = (size_t)nc;
+  c->b_max_cells = maxc;
+  c->b_nsed = (size_t)10 * c->dm.nfreq * c->dm.nmu * c->dm.nph * c->dm.nap * kNo;
+  b.sed_stride = c->b_nsed;
+  if ((rc = dev_alloc(c, "b.t.dtauabs", nc * 8, &p))) return rc;
+  c->bt.dtauabs = (double*)p;
+  if ((rc = dev_alloc(c, "b.t.int3", 3 * nc * 4, &p))) return rc;
+  c->bt.nabs = (int*)p;
+  c->bt.dbig = c->bt.nabs + nc;
+  c->bt.killed = c->bt.dbig + nc;
+  if ((rc = dev_alloc(c, "b.t.sed", (size_t)nmodels * c->b_nsed * 4 + 4, &p))) return rc;
+  c->bt.sed = (float*)p;
+  if ((rc = dev_alloc(c, "b.t.counters", C_NCOUNTERS * 8, &p))) return rc;
+  c->bt.counters = (unsigned long long*)p;
+  c->bt.psed = nullptr;
+  c->bt.image = nullptr;
+  c->bt.trace = nullptr;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  // first queue order: column density along the row of cells above the
+  // equator (optically thick models hold the longest-lived packets)
+  c->b_nrg.assign(nrg, nrg + nmodels);
+  c->b_ntg.assign(ntg, ntg + nmodels);
+  c->b_coff = coff;
+  c->b_cost.assign(nmodels, 0.0);
+  for (int k = 0; k < nmodels; ++k) {
+    const int ncr = nrg[k] - 1, it = (ntg[k] - 1) / 2 > 0 ? (ntg[k] - 1) / 2 : 1;
+    double col = 0.0;
+    for (int ir = 1; ir <= ncr; ++ir) {
+      const size_t cell = (size_t)coff[k] + (size_t)(ir - 1) + (size_t)ncr * (size_t)(it - 1);
+      col += rho[cell] * (rarr[roff[k] + ir] - rarr[roff[k] + ir - 1]);
+    }
+    c->b_cost[k] = col;
+  }
+  c->b_ran = false;
+  c->has_batch = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_batch_tdust(mcrt_ctx* c, const double* tdust) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->has_batch) return fail(c, MCRT_E_STATE, "mcrt_set_batch_tdust: set the batch first");
+  if (!tdust) return fail(c, MCRT_E_ARG, "mcrt_set_batch_tdust: null array");
+  HIPCHECK(c, hipSetDevice(c->device));
+  HIPCHECK(c, hipMemcpyAsync((void*)c->bd.tdust, tdust, c->b_ncell * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->has_batch_tdust = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_batch_launch(mcrt_ctx* c, int32_t block, int32_t waves_per_simd, int32_t regroup, int32_t combine, int32_t graph) {
+  if (!c) return MCRT_E_ARG;
+  if (block != 0 && block != 64 && block != 128 && block != 256) return fail(c, MCRT_E_ARG, "mcrt_set_batch_launch: block must be 64, 128 or 256");
+  if (waves_per_simd < 0 || waves_per_simd > 8) return fail(c, MCRT_E_ARG, "mcrt_set_batch_launch: waves per SIMD must be 1..8");
+  if (regroup < 0 || regroup > 64) return fail(c, MCRT_E_ARG, "mcrt_set_batch_launch: regroup must be 1..64");
+  if (block) c->b_block = block;
+  if (waves_per_simd) c->b_waves = waves_per_simd;
+  if (regroup) c->b_regroup = regroup;
+  c->b_combine = combine ? 1 : 0;
+  c->b_graph = graph ? 1 : 0;
+  return MCRT_OK;
+}
+
+int mcrt_run_batch(mcrt_ctx* c, const int64_t* npackets, uint64_t seed, int32_t precision) {
+  if (!c) return MCRT_E_ARG;
+  if (!batch_ready(c)) return fail(c, MCRT_E_STATE, "mcrt_run_batch: batch, batch tdust, dust, source and observer must be set");
+  if (c->dm.ipeel != 0) return fail(c, MCRT_E_STATE, "mcrt_run_batch: batches bin SEDs only; set the observer with ipeel = 0");
+  if (!npackets) return fail(c, MCRT_E_ARG, "mcrt_run_batch: null packet counts");
+  if (precision != MCRT_FP64 && precision != MCRT_MIXED) return fail(c, MCRT_E_ARG, "mcrt_run_batch: unknown precision");
+  for (int k = 0; k < c->nmodels; ++k)
+    if (npackets[k] < 0) return fail(c, MCRT_E_ARG, "mcrt_run_batch: negative packet count");
+  if (c->b_nsed != (size_t)10 * c->dm.nfreq * c->dm.nmu * c->dm.nph * c->dm.nap * kNo)
+    return fail(c, MCRT_E_STATE, "mcrt_run_batch: the SED binning changed after mcrt_set_batch; set the batch again");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const bool fp64 = precision == MCRT_FP64;
+  int rc = batch_run(c, npackets, seed, fp64, false, c->b_block, c->b_waves, c->b_combine != 0, c->b_graph != 0, nullptr);
+  if (rc) return rc;
+  // interactions per packet of each model: the next run's queue order
+  c->b_work.assign(c->nmodels, 0ull);
+  HIPCHECK(c, hipMemcpyAsync(c->b_work.data(), c->bd.work, c->nmodels * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nmodels; ++k)
+    if (c->b_npk[k] > 0) c->b_cost[k] = (double)c->b_work[k] / (double)c->b_npk[k];
+  c->b_ran = true;
+  return MCRT_OK;
+}
+
+int mcrt_get_batch_tallies(mcrt_ctx* c, double* dtauabs, int32_t* nabs, int32_t* dbig, int32_t* killed, float* sed,
+                           double* work, double* counters) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->has_batch) return fail(c, MCRT_E_STATE, "mcrt_get_batch_tallies: no batch");
+  HIPCHECK(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const size_t nc = c->b_ncell;
+  if (dtauabs) HIPCHECK(c, hipMemcpyAsync(dtauabs, c->bt.dtauabs, nc * 8, hipMemcpyDeviceToHost, s));
+  if (nabs) HIPCHECK(c, hipMemcpyAsync(nabs, c->bt.nabs, nc * 4, hipMemcpyDeviceToHost, s));
+  if (dbig) HIPCHECK(c, hipMemcpyAsync(dbig, c->bt.dbig, nc * 4, hipMemcpyDeviceToHost, s));
+  if (killed) HIPCHECK(c, hipMemcpyAsync(killed, c->bt.killed, nc * 4, hipMemcpyDeviceToHost, s));
+  if (sed) HIPCHECK(c, hipMemcpyAsync(sed, c->bt.sed, (size_t)c->nmodels * c->b_nsed * 4, hipMemcpyDeviceToHost, s));
+  std::vector<unsigned long long> w(c->nmodels);
+  unsigned long long cnt[C_NCOUNTERS];
+  HIPCHECK(c, hipMemcpyAsync(w.data(), c->bd.work, w.size() * 8, hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipMemcpyAsync(cnt, c->bt.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  HIPCHECK(c, hipStreamSynchronize(s));
+  if (work)
+    for (int k = 0; k < c->nmodels; ++k) work[k] = (double)w[k];
+  if (counters)
+    for (int i = 0; i < C_NCOUNTERS; ++i) counters[i] = (double)cnt[i];
+  return MCRT_OK;
+}
+
+int mcrt_batch_dusttemp(mcrt_ctx* c, int32_t mode, int32_t update_tdust, double* tdave) {
+  if (!c) return MCRT_E_ARG;
+  if (!(c->has_batch && c->has_dust && c->has_source)) return fail(c, MCRT_E_STATE, "mcrt_batch_dusttemp: batch not set");
+  if (!c->b_ran) return fail(c, MCRT_E_STATE, "mcrt_batch_dusttemp: no batch run yet");
+  if (mode != MCRT_T_REFERENCE && mode != MCRT_T_PERCELL) return fail(c, MCRT_E_ARG, "mcrt_batch_dusttemp: unknown mode");
+  HIPCHECK(c, hipSetDevice(c->device));
+  void* p;
+  int rc = dev_alloc(c, "b.tdave", c->b_ncell * 8, &p);
+  if (rc) return rc;
+  double* d_tdave = (double*)p;
+  BatchLaunch L = batch_launch(c, 256, false);
+  HIPCHECK(c, launch_batch_tfinal(L, c->b_npk_dev, mode, d_tdave, c->b_max_cells, c->stream));
+  if (update_tdust) {
+    HIPCHECK(c, hipMemcpyAsync((void*)c->bd.tdust, d_tdave, c->b_ncell * 8, hipMemcpyDeviceToDevice, c->stream));
+    c->has_batch_tdust = true;
+  }
+  if (tdave) HIPCHECK(c, hipMemcpyAsync(tdave, d_tdave, c->b_ncell * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCRT_OK;
+}
+
+int mcrt_batch_autotune(mcrt_ctx* c, int64_t packets_per_model, uint64_t seed, int32_t precision, int32_t* best,
+                        double* table, int32_t max_rows, int32_t* nrows) {
+  if (!c) return MCRT_E_ARG;
+  if (!batch_ready(c)) return fail(c, MCRT_E_STATE, "mcrt_batch_autotune: batch, batch tdust, dust, source and observer must be set");
+  if (c->dm.ipeel != 0) return fail(c, MCRT_E_STATE, "mcrt_batch_autotune: batches bin SEDs only; set the observer with ipeel = 0");
+  if (packets_per_model < 1 || !best) return fail(c, MCRT_E_ARG, "mcrt_batch_autotune: need packets_per_model >= 1 and best[4]");
+  if (precision != MCRT_FP64 && precision != MCRT_MIXED) return fail(c, MCRT_E_ARG, "mcrt_batch_autotune: unknown precision");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const bool fp64 = precision == MCRT_FP64;
+  std::vector<int64_t> npk(c->nmodels, packets_per_model);
+  const double total = (double)packets_per_model * c->nmodels;
+  const int blocks[3] = {64, 128, 256}, waves[3] = {1, 2, 4}, regroups[3] = {16, 32, 48};
+  double best_rate = -1.0;
+  int rows = 0;
+  for (int ib = 0; ib < 3; ++ib)
+    for (int iw = 0; iw < 3; ++iw)
+      for (int ig = 0; ig < 3; ++ig)
+        for (int comb = 0; comb < 2; ++comb) {
+          BatchLaunch L = batch_launch(c, blocks[ib], comb != 0);
+          const int got = batch_grid(c, fp64, L, waves[iw]);
+          if (got < 0) return fail(c, MCRT_E_HIP, "mcrt_batch_autotune: occupancy query failed");
+          if (got != waves[iw]) continue;        // does not fit: same as a smaller wave count
+          const int keep = c->b_regroup;
+          c->b_regroup = regroups[ig];
+          float ms = 0.f, ms2 = 0.f;
+          // the first launch of a shape loads its code object: time the second
+          int rc = batch_run(c, npk.data(), seed, fp64, true, blocks[ib], waves[iw], comb != 0, false, &ms);
+          if (!rc) rc = batch_run(c, npk.data(), seed, fp64, true, blocks[ib], waves[iw], comb != 0, false, &ms2);
+          c->b_regroup = keep;
+          if (rc) return rc;
+          ms = ms2;
+          const double rate = ms > 0.f ? total / (ms * 1e-3) : 0.0;
+          if (table && rows < max_rows) {
+            double* row = table + 6 * rows;
+            row[0] = blocks[ib]; row[1] = waves[iw]; row[2] = regroups[ig]; row[3] = comb; row[4] = ms; row[5] = rate;
+          }
+          ++rows;
+          if (rate > best_rate) {
+            best_rate = rate;
+            best[0] = blocks[ib]; best[1] = waves[iw]; best[2] = regroups[ig]; best[3] = comb;
+          }
+        }
+  if (nrows) *nrows = rows;
+  if (best_rate < 0.0) return fail(c, MCRT_E_HIP, "mcrt_batch_autotune: no launch shape fits");
+  c->b_block = best[0];
+  c->b_waves = best[1];
+  c->b_regroup = best[2];
+  c->b_combine = best[3];
   return MCRT_OK;
 }
 

@@ -340,13 +340,26 @@ template <class T> struct Pkt {
 };
 
 // Runtime-indexed reads of the per-packet opacity arrays through an unrolled
-// select chain, so the arrays stay in VGPRs instead of private memory.
+// select chain, so the arrays stay in VGPRs instead of private memory.  Each
+// element passes an empty asm first: without it the optimizer folds the
+// chain of selects between loads into one load at a selected address, which
+// forces the arrays into scratch (one private-memory load per use, on every
+// walk segment).
+__device__ __forceinline__ void in_vgpr(float& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void in_vgpr(double& x) { asm volatile("" : "+v"(x)); }
+template <class T>
+__device__ __forceinline__ void in_vgpr(Opac<T>& o) { in_vgpr(o.kappa); in_vgpr(o.albedo); in_vgpr(o.g); in_vgpr(o.pl); }
+
 template <class V>
 __device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
   V v = a[0];
+  in_vgpr(v);
 #pragma unroll
-  for (int k = 1; k < kMaxOpac; ++k)
-    if (i == k) v = a[k];
+  for (int k = 1; k < kMaxOpac; ++k) {
+    V x = a[k];
+    in_vgpr(x);
+    v = (i == k) ? x : v;
+  }
   return v;
 }
 

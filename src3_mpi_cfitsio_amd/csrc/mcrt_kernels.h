@@ -5,6 +5,7 @@
 #pragma once
 #include "mcrt_transport.h"
 #include "mcrt_adjoint.h"
+#include "mcrt_batch.h"
 
 namespace mcrt {
 
@@ -134,11 +135,10 @@ __device__ double dusttemp_dev(const DeviceModel& m, double na, double nphot, in
   return Td;
 }
 
-// mode 0, step 1: upper hemisphere at ip = 1, averaged with the mirror cell
-__global__ void __launch_bounds__(256) k_tfinal_half(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
-  const int nr = m.nrg - 1, nh = (m.ntg - 1) / 2;
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nr * nh) return;
+// mode 0, step 1: upper hemisphere at ip = 1, averaged with the mirror cell;
+// idx runs over (nrg-1) * ((ntg-1)/2) source cells
+__device__ __forceinline__ void tfinal_half_at(const DeviceModel& m, const double* dtauabs, double nphot, double* tdave, int idx) {
+  const int nr = m.nrg - 1;
   int ir = idx % nr + 1, it = idx / nr + 1;
   size_t c = cell_index(m, ir, it, 1), cm = cell_index(m, ir, m.ntg - it, 1);
   double t = 0.1 / 11605.0;
@@ -157,10 +157,8 @@ __global__ void __launch_bounds__(256) k_tfinal_half(DeviceModel m, const double
 }
 
 // mode 0, step 2: mirror to the lower hemisphere and replicate in phi
-__global__ void __launch_bounds__(256) k_tfinal_fill(DeviceModel m, double* tdave) {
-  const int nr = m.nrg - 1, nt = m.ntg - 1, np = m.npg - 1, nh = (m.ntg - 1) / 2;
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)nr * nt * np) return;
+__device__ __forceinline__ void tfinal_fill_at(const DeviceModel& m, double* tdave, size_t idx) {
+  const int nr = m.nrg - 1, nt = m.ntg - 1, nh = (m.ntg - 1) / 2;
   int ir = (int)(idx % nr) + 1;
   int it = (int)((idx / nr) % nt) + 1;
   int ip = (int)(idx / ((size_t)nr * nt)) + 1;
@@ -170,10 +168,7 @@ __global__ void __launch_bounds__(256) k_tfinal_fill(DeviceModel m, double* tdav
 }
 
 // mode 1: every cell from its own absorbed energy (no mirror, no phi copy)
-__global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
-  size_t ncell = (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1);
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+__device__ __forceinline__ void tcell_at(const DeviceModel& m, const double* dtauabs, double nphot, double* tdave, size_t c) {
   double t = 0.1 / 11605.0;
   int sp = m.species[c];
   if (sp > 0 && m.rho64[c] > 0.0) {
@@ -182,6 +177,74 @@ __global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtau
     if (t * 11605.0 >= kTmaxK) t = kTmaxK / 11605.0;
   }
   tdave[c] = t;
+}
+
+__global__ void __launch_bounds__(256) k_tfinal_half(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (m.nrg - 1) * ((m.ntg - 1) / 2)) return;
+  tfinal_half_at(m, dtauabs, nphot, tdave, idx);
+}
+
+__global__ void __launch_bounds__(256) k_tfinal_fill(DeviceModel m, double* tdave) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1)) return;
+  tfinal_fill_at(m, tdave, idx);
+}
+
+__global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtauabs, double nphot, double* tdave) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1)) return;
+  tcell_at(m, dtauabs, nphot, tdave, c);
+}
+
+// Lucy temperatures of every model of a batch: blockIdx.y = model, with the
+// per-model packet count as nphot.  mode 0: tfinal's upper hemisphere and
+// mirror; mode 1: every cell on its own.
+__device__ __forceinline__ void bind_cells(DeviceModel& m, const BatchDesc& b, int k) {
+  const long long ro = b.roff[k], to = b.toff[k], co = b.coff[k];
+  m.nrg = b.nrg[k];
+  m.ntg = b.ntg[k];
+  m.rarr = b.rarr + ro;
+  m.r2arr = b.r2arr + ro;
+  m.thetarr = b.thetarr + to;
+  m.costarr = b.costarr + to;
+  m.tan2arr = b.tan2arr + to;
+  m.rho64 = b.rho64 + co;
+  m.rho32 = b.rho32 + co;
+  m.species = b.species + co;
+  m.vcell = b.vcell + co;
+}
+
+__global__ void __launch_bounds__(256) k_batch_tfinal_half(DeviceModel m0, BatchDesc bd, const unsigned long long* npk,
+                                                           const double* dtauabs, double* tdave) {
+  const int k = blockIdx.y;
+  DeviceModel m = m0;
+  bind_cells(m, bd, k);
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (m.nrg - 1) * ((m.ntg - 1) / 2) || npk[k] == 0) return;
+  const long long co = bd.coff[k];
+  tfinal_half_at(m, dtauabs + co, (double)npk[k], tdave + co, idx);
+}
+
+__global__ void __launch_bounds__(256) k_batch_tfinal_fill(DeviceModel m0, BatchDesc bd, const unsigned long long* npk,
+                                                           double* tdave) {
+  const int k = blockIdx.y;
+  DeviceModel m = m0;
+  bind_cells(m, bd, k);
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1) || npk[k] == 0) return;
+  tfinal_fill_at(m, tdave + bd.coff[k], idx);
+}
+
+__global__ void __launch_bounds__(256) k_batch_tcell(DeviceModel m0, BatchDesc bd, const unsigned long long* npk,
+                                                     const double* dtauabs, double* tdave) {
+  const int k = blockIdx.y;
+  DeviceModel m = m0;
+  bind_cells(m, bd, k);
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1) || npk[k] == 0) return;
+  const long long co = bd.coff[k];
+  tcell_at(m, dtauabs + co, (double)npk[k], tdave + co, c);
 }
 
 template <class V>

@@ -1,13 +1,26 @@
 // mcrt_launch.h -- host-side launchers exported by the per-precision kernel
 // translation units (kernels_fp64.hip: -ffp-contract=off; kernels_mixed.hip).
 #pragma once
-#include "mcrt_device.h"
+#include "mcrt_batch.h"
 
 namespace mcrt {
 
 struct RunParams;
 struct TraceArgs;
 struct AdjParams;
+
+// one launch of the batch kernel k_batch (block 64, 128 or 256)
+struct BatchLaunch {
+  DeviceModel m;
+  DeviceTallies t;
+  BatchDesc bd;
+  BatchRun* run;
+  const double* inczeta;
+  LdsPlan lds;
+  int block;
+  bool combine;
+  unsigned grid;
+};
 
 // rp.lanes must be a multiple of 256
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
@@ -24,6 +37,13 @@ hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream
 hipError_t launch_trace_mixed(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
 hipError_t launch_mueller_fp64(int n, int kind, const double* par, double* dir, double* st, hipStream_t s);
 hipError_t launch_mueller_mixed(int n, int kind, const double* par, double* dir, double* st, hipStream_t s);
+hipError_t launch_batch_fp64(const BatchLaunch& L, hipStream_t s);
+hipError_t launch_batch_mixed(const BatchLaunch& L, hipStream_t s);
+hipError_t batch_occupancy_fp64(const BatchLaunch& L, int* blocks_per_cu);
+hipError_t batch_occupancy_mixed(const BatchLaunch& L, int* blocks_per_cu);
+// Lucy temperatures of every model (npk: packets per model; max_cells: cells of the largest model)
+hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* npk, int mode, double* tdave, size_t max_cells,
+                               hipStream_t s);
 hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);

@@ -243,6 +243,9 @@ struct Life {
   uint32_t c_rho, c_sp, c_td;
   T rho;
   int sp, spop;        // species code and its opacity table
+  // write-combined dtauabs of the cell visit in progress (walk_step<.., kComb>)
+  uint32_t acc_c;
+  double acc_v;
   T td;
 };
 
@@ -277,6 +280,29 @@ __device__ __forceinline__ T cell_opac_l(const DeviceModel& m, Life<T>& L, size_
   return (T)0;
 }
 
+// The Lucy tally of one segment (tauint_3d_2_fuv.f90:83-90,122-130).  With
+// kComb the lane sums its segments in one cell and adds the sum when it
+// leaves the cell (or ends the packet: tally_flush): a packet trapped in a
+// thick cell then issues one FP64 atomic per visit instead of one per flight.
+template <bool kComb, class T>
+__device__ __forceinline__ void tally_abs(Ctx<T>& C, Life<T>& L, size_t c, double v) {
+  if constexpr (kComb) {
+    if (L.acc_c == (uint32_t)c) { L.acc_v += v; return; }
+    if (L.acc_c != 0xFFFFFFFFu) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
+    L.acc_c = (uint32_t)c;
+    L.acc_v = v;
+  } else {
+    atomicAdd(&C.t.dtauabs[c], v);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void tally_flush(Ctx<T>& C, Life<T>& L) {
+  if (L.acc_c != 0xFFFFFFFFu) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
+  L.acc_c = 0xFFFFFFFFu;
+  L.acc_v = 0.0;
+}
+
 template <class T>
 __device__ __forceinline__ void flight_begin(Life<T>& L, T tau) {
   L.tau = tau; L.tsum = (T)0; L.nseg = 0;
@@ -286,7 +312,7 @@ __device__ __forceinline__ void flight_begin(Life<T>& L, T tau) {
 // One pass of tauint_3d_2_fuv.f90's face loop (:40-205).  0 = the flight
 // goes on, 1 = the interaction point was reached inside the cell, 2 = the
 // packet left the grid, -1 = walk error.
-template <class T, bool kAdj = false>
+template <class T, bool kAdj = false, bool kComb = false>
 __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
@@ -310,7 +336,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
       if ((G)ds < 0.5 * wall_clearance<T>(m, p)) {
         ++L.nseg;
         C.cnt.add(C_SEGMENTS);
-        if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+        if (iop >= 0) tally_abs<kComb>(C, L, c, (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
         if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), (G)ds, rho, true); }
         p.xp = p.xp + (G)ds * p.ux;
         p.yp = p.yp + (G)ds * p.uy;
@@ -332,7 +358,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   T dtau = opac * (T)dt;
   if ((L.tsum + dtau) > L.tau) {
     T ds = (L.tau - L.tsum) / opac;
-    if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+    if (iop >= 0) tally_abs<kComb>(C, L, c, (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
     if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), (G)ds, rho, true); }
     p.xp = p.xp + (G)ds * p.ux;
     p.yp = p.yp + (G)ds * p.uy;
@@ -342,7 +368,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
     p.on_wall = false;
     return 1;
   }
-  if (iop >= 0) atomicAdd(&C.t.dtauabs[c], (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
+  if (iop >= 0) tally_abs<kComb>(C, L, c, (double)dt * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
   if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), dt, rho, false); }
   cross_face(m, p.ir, p.it, p.ip, iface);
   p.xp = p.xp + dt * p.ux;

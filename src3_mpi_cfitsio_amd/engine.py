@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import MCRT_FP64, MCRT_MIXED, MCRT_T_PERCELL, MCRT_T_REFERENCE, ptr
-from .model import LSUN, Model
+from .model import LSUN, Batch, Model
 
 COUNTERS = ["tot", "flux", "sflux", "aflux", "abflux", "scount", "nscat", "hit", "htot",
             "escaped", "killed", "segments", "peel_segments", "peel_events", "resets", "errors"]
@@ -245,6 +245,95 @@ class Engine:
         t = np.zeros((n, 8))
         self._check(self.lib.mcrt_get_trace(self.h, ptr(t, C.c_double), int(n)))
         return t
+
+
+
+class BatchEngine(Engine):
+    """A batch of independent axisymmetric models (model.Batch) in one
+    context: the shared dust, star and SED binning from batch.shared, the
+    per-model walls and media concatenated (mcrt_set_batch).  Packet p of
+    model k runs with seed + k: the path it takes in Engine(model k).run_packets
+    with that seed."""
+
+    def __init__(self, batch: Batch, device: int = 0, stream: Optional[int] = None):
+        super().__init__(batch.shared, device=device, stream=stream)
+        self.batch = batch
+        cat = batch.concat()
+        self._cat = cat
+        self.coff = batch.cell_offsets()
+        d = C.c_double
+        sh = batch.shared
+        phi = [_f64(x) for x in (sh.phiarr, sh.aarr, sh.barr)]
+        so = np.ascontiguousarray(sh.species_opac, dtype=np.int32)
+        self._check(self.lib.mcrt_set_batch(
+            self.h, batch.nmodels, ptr(cat["nrg"], C.c_int32), ptr(cat["ntg"], C.c_int32),
+            *[ptr(cat[k], d) for k in ("rarr", "r2arr", "thetarr", "costarr", "tan2arr")],
+            *[ptr(x, d) for x in phi], ptr(cat["vcell"], d), ptr(cat["rho"], d),
+            ptr(cat["species"], C.c_uint8), ptr(so, C.c_int32)))
+        self.set_batch_tdust(cat["tdust"])
+
+    def set_batch_tdust(self, tdust):
+        t = _f64(tdust)
+        self._check(self.lib.mcrt_set_batch_tdust(self.h, ptr(t, C.c_double)))
+
+    def set_batch_launch(self, block=0, waves_per_simd=0, regroup=0, combine=0, graph=1):
+        self._check(self.lib.mcrt_set_batch_launch(self.h, int(block), int(waves_per_simd), int(regroup),
+                                                   int(bool(combine)), int(bool(graph))))
+
+    def run_batch(self, npackets, seed: int, precision: int = MCRT_MIXED):
+        """One persistent launch over npackets[k] packets of every model k
+        (an int gives every model that many)."""
+        n = np.broadcast_to(np.asarray(npackets, dtype=np.int64), (self.batch.nmodels,))
+        self._npk = np.ascontiguousarray(n)
+        self._check(self.lib.mcrt_run_batch(self.h, self._npk.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            C.c_uint64(int(seed)), int(precision)))
+
+    def batch_tallies(self):
+        """Per-model tallies of the last run: list of dicts (dtauabs, nabs,
+        dbig, killed in the model's cell_shape; the SED arrays as
+        Engine.tallies), plus the per-model work and the summed counters."""
+        b = self.batch
+        nc = int(self.coff[-1])
+        sh = b.shared
+        nsed = 10 * NO * sh.nap * sh.nph * sh.nmu * sh.nfreq
+        dt = np.zeros(nc)
+        ints = [np.zeros(nc, np.int32) for _ in range(3)]
+        sed = np.zeros(b.nmodels * nsed, np.float32)
+        work = np.zeros(b.nmodels)
+        cnt = np.zeros(16)
+        self._check(self.lib.mcrt_get_batch_tallies(
+            self.h, ptr(dt, C.c_double), *[ptr(x, C.c_int32) for x in ints], ptr(sed, C.c_float),
+            ptr(work, C.c_double), ptr(cnt, C.c_double)))
+        out = []
+        for k, m in enumerate(b.models):
+            a, e = int(self.coff[k]), int(self.coff[k + 1])
+            t = dict(dtauabs=dt[a:e].reshape(m.cell_shape), nabs=ints[0][a:e].reshape(m.cell_shape),
+                     dbig=ints[1][a:e].reshape(m.cell_shape), killed=ints[2][a:e].reshape(m.cell_shape))
+            s = sed[k * nsed:(k + 1) * nsed].reshape(10, NO, sh.nap, sh.nph, sh.nmu, sh.nfreq)
+            for i, name in enumerate(SED_NAMES):
+                t[name] = s[i]
+            out.append(t)
+        return out, work, {k: cnt[i] for i, k in enumerate(COUNTERS)}
+
+    def batch_dusttemp(self, mode: int = MCRT_T_REFERENCE, update_tdust: bool = True):
+        """Lucy temperatures of every model from the last run; list of
+        per-model arrays in cell_shape."""
+        nc = int(self.coff[-1])
+        t = np.zeros(nc)
+        self._check(self.lib.mcrt_batch_dusttemp(self.h, int(mode), int(bool(update_tdust)), ptr(t, C.c_double)))
+        return [t[int(self.coff[k]):int(self.coff[k + 1])].reshape(m.cell_shape)
+                for k, m in enumerate(self.batch.models)]
+
+    def autotune(self, packets_per_model: int, seed: int = 1, precision: int = MCRT_MIXED):
+        """Time every launch shape on a sample and keep the fastest: returns
+        (best (block, waves, regroup, combine), table rows of block, waves,
+        regroup, combine, ms, packets/s)."""
+        best = np.zeros(4, np.int32)
+        table = np.zeros((64, 6))
+        nrows = C.c_int32()
+        self._check(self.lib.mcrt_batch_autotune(self.h, int(packets_per_model), C.c_uint64(int(seed)), int(precision),
+                                                 ptr(best, C.c_int32), ptr(table, C.c_double), 64, C.byref(nrows)))
+        return tuple(int(x) for x in best), table[:min(nrows.value, 64)]
 
 
 def diskflux(tal: Dict[str, np.ndarray], model: Model, npackets: int, ltot: Optional[float] = None):

@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Dict, Optional
+from typing import Dict, List, Optional
 
 import numpy as np
 
@@ -106,6 +106,7 @@ class Model:
     npackets: int = 0
     niter: int = 1
     tdust0: Optional[np.ndarray] = None
+    tables: Optional[tuple] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def ncell(self) -> int:
@@ -358,19 +359,22 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
           clump_sigma=0.0, clump_dim=2.6, clump_seed=1001, pl=0.0,
           npeel=10, nx=101, bands=((0.01, 5000.0),), nfreq=100, nmu=10,
           npackets=1_000_000, niter=5, nlam=100, nT=500, nnu=513,
-          thete_deg=None, phie_deg=0.0) -> Model:
+          thete_deg=None, phie_deg=0.0, flare=1.25, _tables=None) -> Model:
     rarr = radial_walls(nr, rin_au, rout_au, rstar)
     thetarr = theta_walls(nt) if nt > 1 else np.array([0.0, math.pi])
     costarr, tan2arr = theta_tables(thetarr)
     phiarr, aarr, barr = phi_walls(nphi)
     vcell = cell_volumes(rarr, thetarr, phiarr, rstar)
     rho, species = yso_density(rarr, thetarr, phiarr, vcell, rstar, mdisk, rin_au, rout_au,
-                               mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
+                               flare=flare, mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
                                clump_dim=clump_dim, clump_seed=clump_seed)
-    dust = synthetic_dust(nlam=nlam, pl=pl)
-    em = emission_tables(dust, nT=nT, nnu=nnu)
+    if _tables is None:          # (dust, emission tables) reused across models of a batch
+        dust = synthetic_dust(nlam=nlam, pl=pl)
+        em = emission_tables(dust, nT=nT, nnu=nnu)
+    else:
+        dust, em = _tables
     if thete_deg is None:
-        thete_deg = (np.arange(npeel) + 0.5) * (90.0 / npeel)
+        thete_deg = (np.arange(npeel) + 0.5) * (90.0 / max(npeel, 1))
     thete = np.radians(np.asarray(thete_deg, dtype=np.float64))
     if np.any(np.min(np.abs(thete[:, None] - thetarr[None, :]), axis=1) == 0.0):
         raise ValueError("a peel-off angle equals a wall angle (gridset_5_model13new.f90:695-701)")
@@ -394,7 +398,7 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
         nband=len(bands), nwmax=3, band_nw=band_nw, band_freq=band_freq, band_phi=band_phi,
         nfreq=nfreq, nmu=nmu, nph=1, nap=1, aperture2=np.array([(2.0 * rmax) ** 2]),
         u_mu=1.0 - (2.0 / nmu) * (np.arange(nmu) + 0.5), numin=numin,
-        lnurat=math.log(numax / numin), npackets=npackets, niter=niter)
+        lnurat=math.log(numax / numin), npackets=npackets, niter=niter, tables=(dust, em))
 
 
 CONFIGS: Dict[str, dict] = {
@@ -413,6 +417,64 @@ CONFIGS: Dict[str, dict] = {
     "c4": dict(nr=400, nt=360, nphi=128, mdisk=0.1, pl=0.5, npeel=10, nx=512,
                bands=((2.0, 2.4),), npackets=1_000_000_000, niter=10),
 }
+
+
+# ---------------------------------------------------------------------------
+# batched model grid: many independent axisymmetric models
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Batch:
+    """Independent axisymmetric models with one shared star, dust and SED
+    binning (`shared`, a Model whose grid is the first model's), each with
+    its own walls and medium.  Arrays are per model, concatenated for the
+    device in the order of `models`."""
+    shared: Model
+    models: List[Model]
+    params: List[dict]
+
+    @property
+    def nmodels(self) -> int:
+        return len(self.models)
+
+    def concat(self):
+        ms = self.models
+        cat = lambda f: np.ascontiguousarray(np.concatenate([np.asarray(f(x)).ravel() for x in ms]))
+        return dict(
+            nrg=np.array([x.nrg for x in ms], dtype=np.int32), ntg=np.array([x.ntg for x in ms], dtype=np.int32),
+            rarr=cat(lambda x: x.rarr), r2arr=cat(lambda x: x.r2arr), thetarr=cat(lambda x: x.thetarr),
+            costarr=cat(lambda x: x.costarr), tan2arr=cat(lambda x: x.tan2arr),
+            vcell=cat(lambda x: x.vcell), rho=cat(lambda x: x.rho),
+            species=np.ascontiguousarray(np.concatenate([x.species.ravel() for x in ms]).astype(np.uint8)),
+            tdust=cat(lambda x: x.tdust_initial()))
+
+    def cell_offsets(self) -> np.ndarray:
+        n = np.array([x.ncell for x in self.models], dtype=np.int64)
+        return np.concatenate([[0], np.cumsum(n)])
+
+
+def batch_models(n: int = 512, seed: int = 1002, nr_range=(50, 400), nt_range=(40, 200), npeel_sed: int = 10,
+                 nfreq: int = 100, npackets: int = 1_000_000, nlam: int = 100, nT: int = 500, nnu: int = 513) -> Batch:
+    """Seeded grid of axisymmetric disk + envelope models with uneven grids:
+    nr uniform in nr_range, n_theta uniform even in nt_range, Mdisk
+    log-uniform 1e-4..1e-1 Msun, flaring exponent 1.0..1.3, Rin 0.1..1 AU,
+    envelope infall rate log-uniform 1e-7..1e-5 Msun/yr.  Same star and dust
+    as the 100x90 disk; SEDs binned at npeel_sed inclinations (cos theta
+    bins of the escaping packets), no peel-off images."""
+    rng = np.random.default_rng(seed)
+    tables = None
+    models, params = [], []
+    for k in range(n):
+        nr = int(rng.integers(nr_range[0], nr_range[1] + 1))
+        nt = int(rng.integers(nt_range[0] // 2, nt_range[1] // 2 + 1)) * 2
+        p = dict(nr=nr, nt=nt, mdisk=float(10 ** rng.uniform(-4.0, -1.0)), flare=float(rng.uniform(1.0, 1.3)),
+                 rin_au=float(rng.uniform(0.1, 1.0)), mdot=float(10 ** rng.uniform(-7.0, -5.0)))
+        m = build(f"batch{k}", nr=nr, nt=nt, nphi=1, rin_au=p["rin_au"], mdisk=p["mdisk"], mdot=p["mdot"],
+                  flare=p["flare"], npeel=0, nx=1, bands=(), nfreq=nfreq, nmu=npeel_sed, npackets=npackets, niter=1,
+                  nlam=nlam, nT=nT, nnu=nnu, _tables=tables)
+        tables = m.tables
+        models.append(m)
+        params.append(p)
+    return Batch(shared=models[0], models=models, params=params)
 
 
 def config(name: str, **over) -> Model:

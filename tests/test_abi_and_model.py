@@ -101,3 +101,25 @@ def test_ulrich_mu0_solves_cubic():
     m0 = M.ulrich_mu0(mu, rx)
     res = m0 ** 3 + m0 * (rx - 1) - mu * rx
     assert np.max(np.abs(res) / (1 + rx)) < 1e-10
+
+
+def test_batch_models_follow_their_ranges():
+    b = M.batch_models(24, seed=1002, nfreq=20, nlam=40, nT=100, nnu=65)
+    assert b.nmodels == 24
+    for p, m in zip(b.params, b.models):
+        assert 50 <= p["nr"] <= 400 and 40 <= p["nt"] <= 200 and p["nt"] % 2 == 0
+        assert 1e-4 <= p["mdisk"] <= 1e-1 and 1.0 <= p["flare"] <= 1.3
+        assert 0.1 <= p["rin_au"] <= 1.0 and 1e-7 <= p["mdot"] <= 1e-5
+        assert m.nrg == p["nr"] + 1 and m.ntg == p["nt"] + 1 and m.npg == 2 and m.ipeel == 0 and m.nmu == 10
+        # the disk is normalized to the drawn mass; the envelope adds to it
+        from src3_mpi_cfitsio_amd.model import MSUN
+        assert np.sum(m.rho * m.vcell) >= p["mdisk"] * MSUN * (1.0 - 1e-9)
+    # seeded: the same seed gives the same grid
+    b2 = M.batch_models(3, seed=1002, nfreq=20, nlam=40, nT=100, nnu=65)
+    assert [p for p in b2.params] == b.params[:3]
+    cat = b.concat()
+    off = b.cell_offsets()
+    assert cat["rho"].size == off[-1] == sum(m.ncell for m in b.models)
+    k = 5
+    assert np.array_equal(cat["rho"][off[k]:off[k + 1]], b.models[k].rho.ravel())
+    assert cat["rarr"].size == sum(m.nrg for m in b.models)
