@@ -14,6 +14,12 @@ hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, c
 
 int lane_words_fp64() { return lane_words<double>(); }
 
+int transport_blocks_per_cu_fp64(unsigned lds_bytes) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<double>, 256, lds_bytes) != hipSuccess) return 0;
+  return n;
+}
+
 hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_adjoint<double>, dim3(grid), dim3(256), ap.lds.bytes, s, m, t, ap);
   return hipGetLastError();

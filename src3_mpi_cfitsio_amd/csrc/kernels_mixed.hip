@@ -13,6 +13,12 @@ hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, 
 
 int lane_words_mixed() { return lane_words<float>(); }
 
+int transport_blocks_per_cu_mixed(unsigned lds_bytes) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<float>, 256, lds_bytes) != hipSuccess) return 0;
+  return n;
+}
+
 hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s) {
   hipLaunchKernelGGL(k_adjoint<float>, dim3(grid), dim3(256), ap.lds.bytes, s, m, t, ap);
   return hipGetLastError();

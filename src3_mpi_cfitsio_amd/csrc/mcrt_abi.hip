@@ -488,30 +488,31 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   DeviceTallies t = c->dt;
   t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
   const bool fp64 = (precision == MCRT_FP64);
-  // Resident lanes: four waves per SIMD on every CU (never more than the
-  // packets need).  Each round, every lane runs up to `events` events and
-  // queues at most one peel-off record per event; the peel kernel then walks
-  // the round's records.  Packets alive at the end of a round stay parked in
-  // their lane.  Once no new packets are left, the few parked ones get the
-  // whole queue's worth of events per round, so the longest packet's chain of
-  // interactions runs in a handful of rounds.
-  int64_t lanes = c->lanes > 0 ? c->lanes : (int64_t)c->cu_count * 4 * 4 * 64;
+  // Resident lanes: as many 256-thread blocks per CU as the transport
+  // kernel's registers and LDS allow (never more than the packets need).
+  // Each round every lane runs events while it can reserve slots in the
+  // round's shared peel-off queue (one record per event); the peel kernel
+  // then walks the round's records.  Packets alive at the end of a round
+  // stay parked in their lane.
+  if (lds_always_doubles(c->dm) > kLdsAlways)
+    return fail(c, MCRT_E_ARG, "mcrt_run_packets: walls, Planck grids and observer directions exceed the 56 KB LDS budget "
+                               "(2*nrg + 4*ntg + 3*npg + nT + nnu + 5*npeel must stay below 7023)");
+  const LdsPlan lds = plan_lds(c->dm);
+  int64_t lanes = c->lanes;
+  if (lanes <= 0) {
+    int bpc = fp64 ? transport_blocks_per_cu_fp64(lds.bytes) : transport_blocks_per_cu_mixed(lds.bytes);
+    if (bpc < 1) bpc = 1;
+    lanes = (int64_t)c->cu_count * bpc * 256;
+  }
   const int64_t need = ((npackets + 255) / 256) * 256;
   if (lanes > need) lanes = need;
   lanes = ((lanes + 255) / 256) * 256;
   const bool queued = c->dm.ipeel == 1 && c->dm.npeel > 0;
-  // a lane fills at most `events` records of the kSlotChunk-slot chunks it
-  // reserves, so events <= capacity / lanes - (kSlotChunk - 1) keeps every
-  // record of a round inside the queue
-  const int64_t slack = (int64_t)kSlotChunk - 1;
-  int64_t per = c->queue_per_lane > 0 ? c->queue_per_lane : 40;
-  if (per < slack + 1) per = slack + 1;
+  int64_t per = c->queue_per_lane > 0 ? c->queue_per_lane : 80;
+  if (per < (int64_t)kSlotChunk) per = (int64_t)kSlotChunk;
   int64_t capacity = queued ? lanes * per : 0;
   if (capacity > ((int64_t)1 << 30)) capacity = ((int64_t)1 << 30) / lanes * lanes;
-  const int64_t share = queued ? capacity / lanes - slack : ((int64_t)1 << 20);
-  int64_t events = c->events_per_round > 0 ? c->events_per_round : (queued ? share : 64);
-  if (events > share) events = share;
-  if (events < 1) events = 1;
+  int64_t events = c->events_per_round > 0 ? c->events_per_round : ((int64_t)1 << 20);
   const int words = fp64 ? lane_words_fp64() : lane_words_mixed();
   void* p;
   int rc;
@@ -529,10 +530,6 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     q.capacity = (unsigned)capacity;
   }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
-  if (lds_always_doubles(c->dm) > kLdsAlways)
-    return fail(c, MCRT_E_ARG, "mcrt_run_packets: walls, Planck grids and observer directions exceed the 56 KB LDS budget "
-                               "(2*nrg + 4*ntg + 3*npg + nT + nnu + 5*npeel must stay below 7023)");
-  const LdsPlan lds = plan_lds(c->dm);
   double st[MCRT_NSTATS] = {0};
   const char* venv = std::getenv("MCRT_VERBOSE");
   const bool verbose = venv && venv[0] && venv[0] != '0';
@@ -586,12 +583,6 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
                    (long long)round, ta, tb, h.q_count, h.live, (long long)events, h.next_packet, (long long)npackets);
     if (h.next_packet < (unsigned long long)npackets) continue;
     if (h.live == 0) break;
-    if (c->events_per_round <= 0) {
-      // at most 2^20 events per lane per launch keeps every launch short
-      const int64_t grow = queued ? capacity / (int64_t)h.live - slack : ((int64_t)1 << 20);
-      // (h.live <= lanes, so grow >= share >= events)
-      if (grow > events) events = grow < ((int64_t)1 << 20) ? grow : ((int64_t)1 << 20);
-    }
   }
   st[MCRT_STAT_ROUNDS] = (double)(round + 1);
   std::memcpy(c->stats, st, sizeof(st));

@@ -27,6 +27,9 @@ hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, c
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
 int lane_words_fp64();
 int lane_words_mixed();
+// resident 256-thread blocks of k_transport per CU with lds_bytes of dynamic LDS (0: query failed)
+int transport_blocks_per_cu_fp64(unsigned lds_bytes);
+int transport_blocks_per_cu_mixed(unsigned lds_bytes);
 hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);

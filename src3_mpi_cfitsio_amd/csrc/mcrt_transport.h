@@ -140,7 +140,9 @@ __device__ __forceinline__ void adj_peel_event(Ctx<T>& C, const Pkt<T>& p, int s
 // Peel-off queue slots are reserved kSlotChunk at a time per lane, so an
 // interaction usually writes its record without waiting on an atomic.  Slots
 // a lane reserved but did not fill are marked kSlotHole at the end of the
-// round; the peel kernel skips them.
+// round; the peel kernel skips them.  The queue of a round is shared by all
+// lanes: a lane reserves its next chunk before an event that needs a slot,
+// and once the reservations pass the capacity the round is over for it.
 constexpr unsigned kSlotChunk = 8;
 constexpr uint32_t kSlotHole = 0xFFFFFFFFu;
 
@@ -166,6 +168,19 @@ __device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
     C.slot_end = C.slot_next + kSlotChunk;
   }
   return C.slot_next++;
+}
+
+// Make sure this lane holds a free slot for its next event's record; false:
+// the round's queue is full (the lane stops for this round).  Every lane of
+// the calling branch reserves together (one atomic per wavefront).
+template <class T>
+__device__ __forceinline__ bool hold_slot(Ctx<T>& C) {
+  if (C.q.capacity == 0 || C.slot_next < C.slot_end) return true;
+  const unsigned base = reserve_slots(C.q.count, kSlotChunk);
+  if (base >= C.q.capacity) { C.slot_next = C.slot_end = 0u; return false; }
+  C.slot_next = base;
+  C.slot_end = (base + kSlotChunk < C.q.capacity) ? base + kSlotChunk : C.q.capacity;
+  return true;
 }
 
 // initp_common.f90:1-31 (limb = 0)
@@ -929,13 +944,13 @@ __device__ __forceinline__ unsigned long long fetch_packets(unsigned long long* 
 // (or when nobody walks): the long interaction code then runs with most
 // lanes active instead of one lane at a time, and the walk (the bulk of
 // the work) runs with all walking lanes in step.
-// Per round a lane runs at most rp.events events (emissions + interactions),
-// each queueing at most one peel-off record, so a queue of
-// lanes * (events + kSlotChunk - 1) records never overflows.  A packet still
-// waiting for an interaction when its lane's events are spent is parked in
-// HBM for the next round; long-lived packets (hundreds of thousands of
-// interactions in optically thick cells) thereby run beside fresh packets
-// instead of holding the whole launch.
+// Every event (emission or interaction) queues one peel-off record, and the
+// round's queue is shared: a lane runs events (at most rp.events) for as
+// long as it can reserve queue slots, so a long-lived packet (hundreds of
+// thousands of interactions in optically thick cells) advances at its own
+// pace through the whole round instead of a fixed share of events.  A
+// packet still waiting for an interaction when the queue is full (or its
+// events are spent) is parked in HBM for the next round.
 enum : int { L_IDLE = 0, L_WALK = 1, L_READY = 2 };
 
 template <class T>
@@ -957,6 +972,7 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
   }
   uint32_t budget = rp.events;
   bool dry = false;                 // the packet counter ran out for this lane
+  bool full = false;                // the round's peel-off queue is full
   for (;;) {
     if (st == L_WALK) {
       const int r = walk_step<T>(C, L);
@@ -967,18 +983,24 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
       }
     }
     const unsigned long long walking = __ballot(st == L_WALK);
-    const unsigned long long ready = __ballot(st == L_READY && budget > 0);
-    const unsigned long long idle = __ballot(st == L_IDLE && budget > 0 && !dry);
+    const unsigned long long ready = __ballot(st == L_READY && budget > 0 && !full);
+    const unsigned long long idle = __ballot(st == L_IDLE && budget > 0 && !dry && !full);
     if ((walking | ready | idle) == 0ull) break;
     if (ready && (walking == 0ull || (unsigned)__popcll(ready) >= rp.regroup)) {
-      if (st == L_READY && budget > 0) {
-        interact<T>(C, L);
-        budget--;
-        st = L_WALK;
+      if (st == L_READY && budget > 0 && !full) {
+        full = !hold_slot(C);
+        if (!full) {
+          interact<T>(C, L);
+          budget--;
+          st = L_WALK;
+        }
       }
     }
     if (idle && (walking == 0ull || (unsigned)__popcll(idle) >= rp.regroup)) {
-      if (st == L_IDLE && budget > 0 && !dry) {
+      if (st == L_IDLE && budget > 0 && !dry && !full) {
+        full = !hold_slot(C);
+      }
+      if (st == L_IDLE && budget > 0 && !dry && !full) {
         const unsigned long long k = fetch_packets(&rp.ctl->next_packet);
         if (k >= rp.npackets) {
           dry = true;
