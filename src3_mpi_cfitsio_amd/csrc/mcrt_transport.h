@@ -509,6 +509,287 @@ __device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w, Rng& rng, do
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Peel-off walk by ray parameter (mixed path).  A peel walk is a straight ray
+// x(t) = o + t u from the first face it crosses.  Every wall of the grid is a
+// quadric in t: a shell |x|^2 = R^2 has roots -b +- sqrt(R^2 - h^2) (b = o.u,
+// h = |o x u|), a cone x^2 + y^2 = tan^2(theta) z^2 a quadratic, a phi plane
+// one root.  The walk keeps, for the current cell, the ray parameter at which
+// it leaves through each of the three axes; a crossing changes one cell index,
+// so only that axis is solved again.  The wall just crossed is recognized by
+// its root (the one nearest the current parameter) instead of the reference's
+// on-wall tie rule, and no per-step position is formed.  The FP64 path keeps
+// the reference's per-step solvers from the current position (peel_step).
+// The first crossing of every walk is the reference's (peel_ray_first), so
+// walks that start on a wall resolve exactly as in peeloff.
+// ---------------------------------------------------------------------------
+template <class T>
+constexpr bool kRayPeel = std::is_same<T, float>::value;
+
+constexpr double kRayNone = 1.0e300;
+
+template <class T>
+struct PeelRay {
+  G ox, oy, oz;        // origin: where the first crossing left the walk
+  G b, h2;             // o.u and |o x u|^2
+  G t;                 // current ray parameter
+  G tr;                // exit through the radial walls
+  G tlo, tlo2;         // next two crossings of the lower polar wall (thetarr[it-1]) beyond t
+  G thi, thi2;         // the same for the upper polar wall (thetarr[it])
+  G tp;                // exit through the azimuthal walls
+  int dr, dp;          // cell-index step of the radial / azimuthal exit
+  bool started;
+};
+
+// shell exit of cell ir at parameter t: the inner wall when the ray still
+// descends and reaches it, else the outer wall (possibly the one just entered,
+// at its second root)
+template <class T>
+__device__ __forceinline__ void ray_radial(const DeviceModel& m, const PeelRay<T>& Rc, int ir, G& tr, int& dr) {
+  const G sc = -Rc.b;
+  const G R2lo = m.r2arr[ir - 1], R2hi = m.r2arr[ir];
+  if (Rc.t < sc && R2lo > Rc.h2) {
+    const G q = sc - sqrt(R2lo - Rc.h2);
+    if (q > Rc.t) { tr = q; dr = -1; return; }
+  }
+  const G d = R2hi - Rc.h2;
+  tr = sc + sqrt(d > 0.0 ? d : 0.0);
+  dr = 1;
+}
+
+// roots (ascending) of the cone tan2 (or the equator plane, tan2 < 0) along the ray
+template <class T>
+__device__ __forceinline__ int ray_cone_roots(const PeelRay<T>& R, G ux, G uy, G uz, G tan2, G& r1, G& r2) {
+  if (tan2 < 0.0) {                       // equator: z = 0
+    if (uz == 0.0) return 0;
+    r1 = -R.oz / uz;
+    return 1;
+  }
+  if (tan2 == 0.0) return 0;              // pole
+  const G A = ux * ux + uy * uy - tan2 * uz * uz;
+  const G B = R.ox * ux + R.oy * uy - tan2 * R.oz * uz;
+  const G Cc = R.ox * R.ox + R.oy * R.oy - tan2 * R.oz * R.oz;
+  if (fabs(A) > 1.0e-10) {
+    const G disc = B * B - A * Cc;
+    if (!(disc > 0.0)) return 0;
+    const G q = -(B + copysign(sqrt(disc), B));
+    const G a = q / A, c = (q != 0.0) ? Cc / q : a;
+    r1 = fmin(a, c);
+    r2 = fmax(a, c);
+    return 2;
+  }
+  if (fabs(B) > 0.5e-10) { r1 = -Cc / (2.0 * B); return 1; }
+  return 0;
+}
+
+// next two crossings beyond t of one polar wall with roots r1 <= r2; when the
+// ray has just entered the cell through this wall, the root nearest t is that
+// crossing and is left out
+__device__ __forceinline__ void ray_wall_next(G t, bool entered, int n, G r1, G r2, G& nx, G& nx2) {
+  nx = kRayNone;
+  nx2 = kRayNone;
+  if (n == 1) {
+    if (!entered && r1 > t) nx = r1;
+    return;
+  }
+  if (n == 2) {
+    if (entered) {
+      const G far = (fabs(r1 - t) < fabs(r2 - t)) ? r2 : r1;
+      if (far > t) nx = far;
+    } else if (r1 > t) {
+      nx = r1;
+      nx2 = r2;
+    } else if (r2 > t) {
+      nx = r2;
+    }
+  }
+}
+
+// azimuthal exit of cell ip (each plane is crossed once: the wall entered
+// through is skipped); came: 0 none, 1 the lower wall, 2 the upper wall
+template <class T>
+__device__ __forceinline__ void ray_azimuth(const DeviceModel& m, const PeelRay<T>& R, G ux, G uy, int ip, int came,
+                                            G& tp, int& dp) {
+  tp = kRayNone;
+  dp = 0;
+  if (m.npg <= 2 || m.ntg <= 2) return;
+  if (came != 1) {
+    const G A = m.aarr[ip - 1], B = m.barr[ip - 1];
+    const G den = A * ux + B * uy;
+    if (den != 0.0) {
+      const G r = -(A * R.ox + B * R.oy) / den;
+      if (r > R.t && r < tp) { tp = r; dp = -1; }
+    }
+  }
+  if (came != 2) {
+    const G A = m.aarr[ip], B = m.barr[ip];
+    const G den = A * ux + B * uy;
+    if (den != 0.0) {
+      const G r = -(A * R.ox + B * R.oy) / den;
+      if (r > R.t && r < tp) { tp = r; dp = 1; }
+    }
+  }
+}
+
+// Set up the ray at the point where the first crossing (face iface) left the
+// walk, in the cell it entered.
+template <class T>
+__device__ __forceinline__ void ray_start(const DeviceModel& m, PeelWalk<T>& w, PeelRay<T>& R, int iface) {
+  const Pkt<T>& p = w.p;
+  R.ox = p.xp; R.oy = p.yp; R.oz = p.zp;
+  R.b = p.xp * w.ux + p.yp * w.uy + p.zp * w.uz;
+  const G cx = p.yp * w.uz - p.zp * w.uy, cy = p.zp * w.ux - p.xp * w.uz, cz = p.xp * w.uy - p.yp * w.ux;
+  R.h2 = cx * cx + cy * cy + cz * cz;
+  R.t = 0.0;
+  // a "-" face (1, 3, 5) enters the next cell through its upper wall, a "+" face through its lower wall
+  ray_radial<T>(m, R, p.ir, R.tr, R.dr);
+  R.tlo = R.tlo2 = R.thi = R.thi2 = kRayNone;
+  if (m.ntg > 2) {
+    G r1 = 0.0, r2 = 0.0;
+    int n = ray_cone_roots<T>(R, w.ux, w.uy, w.uz, m.tan2arr[p.it - 1], r1, r2);
+    ray_wall_next(R.t, iface == 4, n, r1, r2, R.tlo, R.tlo2);
+    n = ray_cone_roots<T>(R, w.ux, w.uy, w.uz, m.tan2arr[p.it], r1, r2);
+    ray_wall_next(R.t, iface == 3, n, r1, r2, R.thi, R.thi2);
+  }
+  ray_azimuth<T>(m, R, w.ux, w.uy, p.ip, (iface == 5) ? 2 : ((iface == 6) ? 1 : 0), R.tp, R.dp);
+  R.started = true;
+}
+
+// the walk's first crossing: peel_step's reference solvers from the record
+// position (on-wall tie rule included), keeping the face it crossed
+template <class T>
+__device__ __forceinline__ int peel_ray_first(Ctx<T>& C, PeelWalk<T>& w, PeelRay<T>& R, Rng& rng) {
+  const DeviceModel& m = C.m;
+  Pkt<T>& p = w.p;
+  G dt;
+  int iface;
+  bool found = next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface);
+  if (!found) {
+    if (testgrid<T>(m, p, rng)) C.cnt.add(C_RESETS);
+    found = next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface);
+  }
+  if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  C.cnt.add(C_PEELSEG);
+  size_t c = cell_index(m, p.ir, p.it, p.ip);
+  int iop;
+  T rho;
+  T opac = cell_opac<T>(m, p, c, iop, rho);
+  w.tsum = w.tsum + opac * (T)dt;
+  if (w.tsum > (T)m.taumax) return 2;
+  cross_face(m, p.ir, p.it, p.ip, iface);
+  p.xp = p.xp + dt * w.ux;
+  p.yp = p.yp + dt * w.uy;
+  p.zp = p.zp + dt * w.uz;
+  if (p.ir == m.nrg) return 1;
+  if (p.ir == 0) return 2;
+  if constexpr (kRayPeel<T>) ray_start<T>(m, w, R, iface);
+  return 0;
+}
+
+// One crossing of the ray walk: 0 goes on, 1 left the grid toward the
+// observer (w.p holds the exit point), 2 ended (tau > taumax, the star, or
+// the segment guard).  Whatever axis a lane crosses, it then solves one new
+// wall: one square root and two divisions, computed by every lane alike (a
+// branch per axis would make a wavefront run all three solvers).
+template <class T>
+__device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<T>& R) {
+  const DeviceModel& m = C.m;
+  Pkt<T>& p = w.p;
+  if (++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  C.cnt.add(C_PEELSEG);
+  // 0 radial, 1 lower polar wall, 2 upper polar wall, 3 azimuthal
+  int axis = 0;
+  G tn = R.tr;
+  if (R.tlo < tn) { tn = R.tlo; axis = 1; }
+  if (R.thi < tn) { tn = R.thi; axis = 2; }
+  if (R.tp < tn) { tn = R.tp; axis = 3; }
+  const G dt = tn > R.t ? tn - R.t : 0.0;
+  size_t c = cell_index(m, p.ir, p.it, p.ip);
+  int iop;
+  T rho;
+  const T opac = cell_opac<T>(m, p, c, iop, rho);
+  w.tsum = w.tsum + opac * (T)dt;
+  if (w.tsum > (T)m.taumax) return 2;
+  R.t = tn;
+  // the crossing, and the coefficients of the one wall to solve next
+  const G sc = -R.b;
+  G sqarg = 0.0, n1 = 1.0, d1 = 1.0, n2 = 1.0, d2 = 1.0, A = 0.0, B = 0.0, Cq = 0.0, R2o = 0.0;
+  int kind = 3;          // 0 shell, 1 cone, 2 plane, 3 nothing to solve
+  bool inward = false;
+  if (axis == 0) {
+    p.ir += R.dr;
+    if (p.ir == m.nrg || p.ir == 0) {
+      p.xp = R.ox + tn * w.ux;
+      p.yp = R.oy + tn * w.uy;
+      p.zp = R.oz + tn * w.uz;
+      return p.ir == 0 ? 2 : 1;
+    }
+    const G R2lo = m.r2arr[p.ir - 1];
+    R2o = m.r2arr[p.ir];
+    inward = tn < sc && R2lo > R.h2;
+    sqarg = (inward ? R2lo : R2o) - R.h2;
+    kind = 0;
+  } else if (axis == 3) {
+    p.ip += R.dp;
+    if (p.ip == 0) p.ip = m.npg - 1;
+    if (p.ip == m.npg) p.ip = 1;
+    const int j = (R.dp < 0) ? p.ip - 1 : p.ip;
+    const G a = m.aarr[j], bb = m.barr[j];
+    n1 = -(a * R.ox + bb * R.oy);
+    d1 = a * w.ux + bb * w.uy;
+    kind = (d1 != 0.0) ? 2 : 3;
+  } else {
+    int j;
+    if (axis == 1) { p.it -= 1; R.thi = R.tlo2; R.thi2 = kRayNone; j = p.it - 1; }
+    else { p.it += 1; R.tlo = R.thi2; R.tlo2 = kRayNone; j = p.it; }
+    const G t2 = m.tan2arr[j];
+    if (t2 < 0.0) {                      // equator plane
+      n1 = -R.oz;
+      d1 = w.uz;
+      kind = (d1 != 0.0) ? 2 : 3;
+    } else if (t2 > 0.0) {
+      A = w.ux * w.ux + w.uy * w.uy - t2 * w.uz * w.uz;
+      B = R.ox * w.ux + R.oy * w.uy - t2 * R.oz * w.uz;
+      Cq = R.ox * R.ox + R.oy * R.oy - t2 * R.oz * R.oz;
+      if (fabs(A) > 1.0e-10) { sqarg = B * B - A * Cq; kind = (sqarg > 0.0) ? 1 : 3; }
+      else if (fabs(B) > 0.5e-10) { n1 = -Cq; d1 = 2.0 * B; kind = 2; }
+    }
+  }
+  // the shared solve
+  const G sq = sqrt(sqarg > 0.0 ? sqarg : 0.0);
+  if (kind == 1) {
+    const G q = -(B + copysign(sq, B));
+    n1 = q; d1 = A;
+    n2 = Cq; d2 = (q != 0.0) ? q : A;
+  }
+  const G x1 = n1 / d1, x2 = n2 / d2;
+  if (axis == 0) {
+    G root = inward ? sc - sq : sc + sq;
+    int dr = inward ? -1 : 1;
+    if (inward && !(root > tn)) {        // rounding at a tangent: leave through the outer wall
+      const G d = R2o - R.h2;
+      root = sc + sqrt(d > 0.0 ? d : 0.0);
+      dr = 1;
+    }
+    R.tr = root;
+    R.dr = dr;
+  } else if (axis == 3) {
+    R.tp = (kind == 2 && x1 > tn) ? x1 : kRayNone;
+  } else {
+    G nx = kRayNone, nx2 = kRayNone;
+    if (kind == 1) {
+      const G r1 = fmin(x1, x2), r2 = fmax(x1, x2);
+      if (r1 > tn) { nx = r1; nx2 = r2; }
+      else if (r2 > tn) nx = r2;
+    } else if (kind == 2) {
+      if (x1 > tn) nx = x1;
+    }
+    if (axis == 1) { R.tlo = nx; R.tlo2 = nx2; }
+    else { R.thi = nx; R.thi2 = nx2; }
+  }
+  return 0;
+}
+
 // :192-240.  The Stokes vector a walk brings to the observer (exponential
 // weight, then stokespeel for scattered light) and where it lands.
 // q: the packet at the event; s0: its (weighted) Stokes I; (xe, ye, ze):
@@ -1098,6 +1379,8 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
   enum : int { P_IDLE = 0, P_WALK = 1, P_DONE = 2 };
   int st = P_IDLE;
   PeelWalk<T> w;
+  PeelRay<T> ray;
+  ray.started = false;
   for (;;) {
     while (st == P_IDLE && item < total) {
       const uint64_t ri = item / npeel;
@@ -1111,6 +1394,7 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
           if (peel_begin<T>(m, q, k, w)) {
             // testgrid re-placements only: one stream per (run, round, walk)
             rng_init(C.rng, rkey, ((uint64_t)rp.round << 40) + item);
+            ray.started = false;
             st = P_WALK;
             break;
           }
@@ -1119,7 +1403,9 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
       item += stride;
     }
     if (st == P_WALK) {
-      const int r = peel_step<T>(C, w, C.rng);
+      int r;
+      if constexpr (kRayPeel<T>) r = ray.started ? peel_ray_step<T>(C, w, ray) : peel_ray_first<T>(C, w, ray, C.rng);
+      else r = peel_step<T>(C, w, C.rng);
       if (r == 1) st = P_DONE;
       else if (r == 2) { st = P_IDLE; item += stride; }
     }
