@@ -369,6 +369,12 @@ __device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
 struct LocalCounts {
   unsigned long long* s;    // C_NCOUNTERS LDS slots of this workgroup
   __device__ __forceinline__ void add(int i, unsigned long long v = 1ull) const { atomicAdd(&s[i], v); }
+  // +1 from every active lane, as one atomic per wavefront: the per-segment
+  // counters would otherwise serialize 64 same-address LDS atomics per step
+  __device__ __forceinline__ void add_lanes(int i) const {
+    const unsigned long long mask = __ballot(1);
+    if (__lane_id() == __ffsll((long long)mask) - 1) atomicAdd(&s[i], (unsigned long long)__popcll(mask));
+  }
 };
 
 // Direction of flight from (cost, sint, cosp, sinp).  FP64 path: the

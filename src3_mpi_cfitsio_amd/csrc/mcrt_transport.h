@@ -358,7 +358,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
       const T ds = (L.tau - L.tsum) / opac;
       if ((G)ds < 0.5 * wall_clearance<T>(m, p)) {
         ++L.nseg;
-        C.cnt.add(C_SEGMENTS);
+        C.cnt.add_lanes(C_SEGMENTS);
         if (iop >= 0) tally_abs<kComb>(C, L, c, (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
         if constexpr (kAdj) { if (iop >= 0) adj_segment<T>(C, c, pick(p.kapd, iop), (G)ds, rho, true); }
         p.xp = p.xp + (G)ds * p.ux;
@@ -377,7 +377,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
     if (!next_face<T>(m, p, p.ux, p.uy, p.uz, dt, iface)) return -1;
   }
   if (++L.nseg > m.max_segments) return -1;
-  C.cnt.add(C_SEGMENTS);
+  C.cnt.add_lanes(C_SEGMENTS);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
   T rho;
@@ -488,7 +488,7 @@ __device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w, Rng& rng, do
     }
   }
   if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
-  C.cnt.add(C_PEELSEG);
+  C.cnt.add_lanes(C_PEELSEG);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
   T rho;
@@ -669,7 +669,7 @@ __device__ __forceinline__ int peel_ray_first(Ctx<T>& C, PeelWalk<T>& w, PeelRay
     found = next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface);
   }
   if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
-  C.cnt.add(C_PEELSEG);
+  C.cnt.add_lanes(C_PEELSEG);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
   T rho;
@@ -696,7 +696,7 @@ __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<
   const DeviceModel& m = C.m;
   Pkt<T>& p = w.p;
   if (++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
-  C.cnt.add(C_PEELSEG);
+  C.cnt.add_lanes(C_PEELSEG);
   // 0 radial, 1 lower polar wall, 2 upper polar wall, 3 azimuthal
   int axis = 0;
   G tn = R.tr;
@@ -704,11 +704,12 @@ __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<
   if (R.thi < tn) { tn = R.thi; axis = 2; }
   if (R.tp < tn) { tn = R.tp; axis = 3; }
   const G dt = tn > R.t ? tn - R.t : 0.0;
-  size_t c = cell_index(m, p.ir, p.it, p.ip);
-  int iop;
-  T rho;
-  const T opac = cell_opac<T>(m, p, c, iop, rho);
-  w.tsum = w.tsum + opac * (T)dt;
+  {
+    int iop;
+    T rho;
+    const T opac = cell_opac<T>(m, p, cell_index(m, p.ir, p.it, p.ip), iop, rho);
+    w.tsum = w.tsum + opac * (T)dt;
+  }
   if (w.tsum > (T)m.taumax) return 2;
   R.t = tn;
   // the crossing, and the coefficients of the one wall to solve next
@@ -944,7 +945,7 @@ __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
   T g = o.g, pl = o.pl;
   T peak = cs_peak<T>(g), bmu, ri1, rprob, pp[4];
   for (;;) {
-    C.cnt.add(C_HTOT);
+    C.cnt.add_lanes(C_HTOT);
     bmu = one - two * uniform<T>(C.rng);
     T cosb2 = bmu * bmu;
     dustmat<T>(g, pl, bmu, cosb2, pp);
@@ -961,7 +962,7 @@ __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
     if (rprob > peak) peak = rprob;
     T xran = uniform<T>(C.rng);
     if (peak * xran > rprob) continue;
-    C.cnt.add(C_HIT);
+    C.cnt.add_lanes(C_HIT);
     break;
   }
   T dir[5] = {p.cost, p.sint, p.phi, p.cosp, p.sinp};
@@ -993,6 +994,22 @@ __device__ __forceinline__ int flight_end(Ctx<T>& C, Life<T>& L) {
   return 0;
 }
 
+// A peel-off walk from p crosses no wall before wall_clearance, so its
+// optical depth exceeds taumax inside the first segment once opac times the
+// clearance does (with a margin for rounding): the walk ends there with
+// nothing for the observer (peeloff_3d_2_test.f90:175), and its record is
+// not queued.  Deep inside optically thick cells, where the long-lived
+// packets spend their interactions, that is every record.
+template <class T>
+__device__ __forceinline__ bool peel_hidden(const DeviceModel& m, Life<T>& L) {
+  const Pkt<T>& p = L.p;
+  if (p.on_wall) return false;
+  int iop;
+  T rho;
+  const T opac = cell_opac_l<T>(m, L, cell_index(m, p.ir, p.it, p.ip), iop, rho);
+  return (double)opac * wall_clearance<T>(m, p) > 1.001 * m.taumax;
+}
+
 // propagate_5_fuv.f90:62-198: one interaction (scatter, or absorb and
 // re-emit), its peel-off record, and the start of the next flight.
 template <class T, bool kAdj = false>
@@ -1007,7 +1024,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   L.nint++;
   if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
     p.iflag = 1; L.iscat++; L.iint++;
-    if (m.ipeel == 1) peel_event<T, kAdj>(C, p, L.idust2, iop);
+    if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
     stokes<T>(C, p, iop);
   } else {
     atomicAdd(&t.dbig[c], 1);
@@ -1027,7 +1044,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     opacset<T>(m, p.nub, p.op);
     p.wave = (T)1.2398 / p.nub;
     set_kapd(p, m, 1);
-    if (m.ipeel == 1) peel_event<T, kAdj>(C, p, L.idust2, iop);
+    if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
     L.iint++;
   }
   T tau = RT<T>::tau_of(uniform<T>(C.rng));
