@@ -409,9 +409,10 @@ CONFIGS: Dict[str, dict] = {
     "c1": dict(nr=200, nt=180, nphi=64, clump_sigma=0.5, clump_seed=1001, pl=0.5,
                npeel=10, nx=256, bands=((1.1, 1.4), (2.0, 2.4), (3.2, 4.0)),
                npackets=100_000_000, niter=5),
-    # adjoint model
+    # adjoint model; the observer sits 0.05 deg off 60 deg, which is a theta
+    # wall of this grid (a peel angle on a wall is refused, gridset:695-701)
     "c3": dict(nr=150, nt=120, nphi=32, clump_sigma=0.3, clump_seed=1003, pl=0.5,
-               npeel=1, thete_deg=[60.0], nx=128, bands=((2.0, 2.4), (3.2, 4.0)),
+               npeel=1, thete_deg=[60.05], nx=128, bands=((2.0, 2.4), (3.2, 4.0)),
                npackets=10_000_000, niter=1),
     # large optically thick Lucy run
     "c4": dict(nr=400, nt=360, nphi=128, mdisk=0.1, pl=0.5, npeel=10, nx=512,
