@@ -59,6 +59,26 @@ def kernel_bytes(c, st, rec_bytes, lane_bytes):
     return transport, peel
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_peel_k_transport_c1_1e8_mixed_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch over the
+    timed step, each scaled by what the counter reads for a copy kernel of
+    known size): its line `traffic_bytes_per_launch <kernel> <bytes>`.
+    None when the summary holds no figure for that kernel."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
+
+
 def sed_error(a, b):
     """Flux-weighted relative L1 of the total SED over (frequency, inclination)
     bins holding >= 100 packets in the reference run b."""
@@ -228,7 +248,7 @@ def main():
                    "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; FP64 geometry and dtauabs)",
                    "parallelism": "1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
                      "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
                      "kernel_ms_per_launch": kms / max(nl, 1.0),
                      "algorithmic_bytes_per_launch": kB / max(nl, 1.0),
