@@ -43,7 +43,7 @@ def log(*a):
 
 
 def kernel_bytes(c, st, rec_bytes, lane_bytes):
-    """Algorithmic HBM bytes of the two kernels of a run (see DESIGN.md).
+    """Algorithmic HBM bytes of the two kernels of a run.
 
     transport: per face crossing the FP32 density (4 B) + species (1 B) +
     FP64 dtauabs read-modify-write (16 B); per escaped packet its SED row,
@@ -109,6 +109,105 @@ def cpu_baseline(m, tdust, seconds, threads):
     return n / dt, n, dt
 
 
+def run_batch_workload(args):
+    """The 512-model grid of axisymmetric disks (uneven grids, SEDs only) in
+    one persistent launch: two warm-up Lucy iterations, the autotuner on a
+    sample, then a timed run of --packets per model (default 10^6) in mixed
+    precision, and FP64 on a quarter of that for the ratio and the SED
+    error on identical counters."""
+    from src3_mpi_cfitsio_amd import model as M
+    from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, BatchEngine
+    t0 = time.perf_counter()
+    b = M.batch_models(512, seed=1002)
+    log(f"512 models, {sum(x.ncell for x in b.models)} cells, built in {time.perf_counter()-t0:.1f}s")
+    e = BatchEngine(b)
+    for k in range(2):
+        e.run_batch(100_000, seed=500 + k, precision=MCRT_MIXED)
+        e.batch_dusttemp(update_tdust=True)
+        log(f"warm-up Lucy iteration {k}: {e.last_run_ms()/1e3:.2f}s")
+    best, table = e.autotune(packets_per_model=2000, seed=3, precision=MCRT_MIXED)
+    log(f"autotuner: {len(table)} shapes, best (block, waves, regroup, combine) = {best}")
+    e.set_batch_launch(graph=1)
+    n = args.packets or 1_000_000
+    times = []
+    for s in range(args.warmup + args.steps):
+        e.synchronize()
+        t1 = time.perf_counter()
+        e.run_batch(n, seed=900 + s, precision=MCRT_MIXED)
+        e.synchronize()
+        dt = time.perf_counter() - t1
+        log(f"step {s}: {512*n} packets in {dt:.2f}s (kernel {e.last_run_ms()/1e3:.2f}s)")
+        if s >= args.warmup:
+            times.append(dt)
+    ms = 1e3 * float(np.mean(times))
+    value = 512 * n / (ms / 1e3)
+    extra = {}
+    if not args.no_fp64:
+        nc = max(1, n // 4)
+        res = {}
+        for prec in (MCRT_FP64, MCRT_MIXED):
+            t1 = time.perf_counter()
+            e.run_batch(nc, seed=9999, precision=prec)
+            e.synchronize()
+            dt = time.perf_counter() - t1
+            tal = e.batch_tallies()[0]
+            res[prec] = (dt, np.sum([t["si"][0, 0].astype(np.float64) for t in tal], axis=0),
+                         np.concatenate([t["si"][0, 0].astype(np.float64).ravel() for t in tal]))
+            log(f"{'fp64' if prec == MCRT_FP64 else 'mixed'} comparison: {512*nc} packets in {dt:.2f}s")
+        a, f = res[MCRT_MIXED][2], res[MCRT_FP64][2]
+        extra = dict(fp64_value=512 * nc / res[MCRT_FP64][0],
+                     mixed_over_fp64=res[MCRT_FP64][0] / res[MCRT_MIXED][0],
+                     sed_err_vs_fp64=float(np.sum(np.abs(a - f)) / max(np.sum(np.abs(f)), 1e-300)),
+                     fp64_comparison_packets=512 * nc)
+    e.close()
+    out = {"metric": "photon packets/s, batched model grid (one persistent launch)", "value": value, "unit": "packets/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "c2: 512 axisymmetric disk+envelope models, nr 50-400, ntheta 40-200, SEDs at 10 "
+                                  "inclinations, one Lucy iteration per step", "packets_per_model": n,
+                      "launch": {"block": best[0], "waves_per_simd": best[1], "regroup": best[2], "combine": best[3]},
+                      "parallelism": "1 GPU"}}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def run_adjoint_workload(args):
+    """Image gradient by path replay on the 150x120x32 model: --packets
+    (default 10^7) in mixed precision after two FP64 Lucy iterations, and
+    the FP64 gradient on the same counters for the comparison."""
+    from src3_mpi_cfitsio_amd import model as M
+    from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine
+    m = M.config("c3")
+    e = Engine(m)
+    for k in range(2):
+        e.zero_tallies()
+        e.run_packets(2_000_000, seed=300 + k, precision=MCRT_FP64)
+        e.dusttemp(2_000_000, update_tdust=True)
+    log("model heated")
+    W = np.random.default_rng(1003).uniform(0.0, 1.0, size=(m.nband, m.nx, m.nx, m.npeel)).astype(np.float32)
+    n = args.packets or 10_000_000
+    t0 = time.perf_counter()
+    l32, g32 = e.adjoint_image(W, n, seed=21, precision=MCRT_MIXED, grad=True)
+    dt = time.perf_counter() - t0
+    log(f"mixed adjoint: {n} packets in {dt:.2f}s")
+    extra = {}
+    if not args.no_fp64:
+        t0 = time.perf_counter()
+        l64, g64 = e.adjoint_image(W, n, seed=21, precision=MCRT_FP64, grad=True)
+        dt64 = time.perf_counter() - t0
+        log(f"fp64 adjoint: {n} packets in {dt64:.2f}s")
+        extra = dict(fp64_value=n / dt64, mixed_over_fp64=dt64 / dt, loss_rel_err_vs_fp64=abs(l32 - l64) / abs(l64),
+                     grad_rel_l2_vs_fp64=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)))
+    e.close()
+    out = {"metric": "photon packets/s, path-replay image gradient (forward + replay)", "value": n / dt,
+           "unit": "packets/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "c3: 150x120x32 cells, 128x128 images at 2.2 and 3.6 um, one inclination, "
+                                  "d(sum W image)/d(rho) per cell", "packets": n, "parallelism": "1 GPU"}}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes on the GPU")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"],
+                    help="c1: the headline single-model metric; c2: the 512-model batch; c3: the image gradient")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -132,9 +233,14 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the path shards by packet ranges with one tally all-reduce per Lucy iteration; "
-              "see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by packet ranges with one tally all-reduce per Lucy iteration)",
+              file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c2":
+        return run_batch_workload(args)
+    if args.workload == "c3":
+        return run_adjoint_workload(args)
 
     from src3_mpi_cfitsio_amd import model as M
     from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine
