@@ -71,6 +71,23 @@ int mcrt_set_dust(mcrt_ctx *ctx, int32_t nopac, int32_t nlam,
                   int32_t nT, int32_t nnu, const double *lTint, const double *Tint,
                   const double *nuint, const double *kapint, const double *kappap);
 
+/* setemiss (dust_5.f90:507-699) on the GPU: the Planck-mean emissivity
+ * tables that mcrt_set_dust takes, computed from the dust tables with the
+ * reference's qsimp integration per (species, temperature, interval).
+ * Inputs as in mcrt_set_dust (nudust/kappad/adust[iop*nlam + k]);
+ * nlambda[iop] = table entries used per species (NULL: all nlam);
+ * smooth[iop] != 0 smooths kappap as for species above ndg (needs nT >= 21;
+ * NULL: none).  Outputs (host): kapint[(iop*nnu + j)*nT + i], kappap and
+ * kappar [iop*nT + i] (kappar = the inverse Rosseland sum, as setemiss
+ * leaves it).  nfail (optional) = qsimp calls that did not converge (the
+ * reference's counta + countc); ms (optional) = kernel time.  Replaces the
+ * call `setemiss()` in dustinit (dust_5.f90:499) and its MPI split. */
+int mcrt_setemiss(mcrt_ctx *ctx, int32_t nopac, int32_t nlam, const int32_t *nlambda,
+                  const double *nudust, const double *kappad, const double *adust,
+                  int32_t nT, const double *Tint, int32_t nnu, const double *nuint,
+                  const int32_t *smooth, double *kapint, double *kappap, double *kappar,
+                  int64_t *nfail, float *ms);
+
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);
 

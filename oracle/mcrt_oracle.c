@@ -1300,3 +1300,177 @@ int32_t orc_trace_ray(const orc_model *m, double pos[3], const double u[3], int3
     *tau_out = tsum;
     return n;
 }
+
+/* ======================================================================
+ * setemiss: Planck-mean emissivity tables (dust_5.f90:507-699)
+ * ====================================================================== */
+
+/* mathsubs.f90:79-113: exp(x)-1 with the Abramowitz & Stegun 4.2.45
+ * polynomial for |x| <= 0.6931472 */
+double orc_expm1_as(double x)
+{
+    static const double c[7] = {0.9999999995, 0.4999999206, 0.1666653019, 0.0416573475,
+                                0.0083013598, 0.0013298820, 0.0001413161};
+    if (fabs(x) > 0.6931472) return exp(x) - 1.0;
+    double s = 0.0;
+    for (int i = 6; i >= 0; --i) s = x * (c[i] + s);
+    return s;
+}
+
+/* mathsubs.f90:4-22 */
+double orc_bnu(double nu, double T)
+{
+    const double x = nu / T;
+    if (x == 0.0) return 0.0;
+    return 0.1539897 * x * x * x / orc_expm1_as(x);
+}
+
+/* mathsubs.f90:29-47 */
+double orc_dbnu(double nu, double T)
+{
+    const double x = nu / T;
+    if (x == 0.0) return 0.0;
+    return -0.1539897 * x * x * x * x / (4.0 * orc_expm1_as(x) * orc_expm1_as(-x));
+}
+
+/* module kappa_B_int (dust_5.f90:1-7) for one interval */
+typedef struct { int kind; double k0, k1, nu0, dnu, teff; } orc_seg;
+
+/* kappaB (dust_5.f90:782-794) for kind 0, chidB (:814-826) for kind 1 */
+static double seg_f(const orc_seg *s, double f)
+{
+    const double nu = s->nu0 + f * s->dnu;
+    if (s->kind == 0) return ((1.0 - f) * s->k0 + f * s->k1) * orc_bnu(nu, s->teff);
+    return orc_dbnu(nu, s->teff) / ((1.0 - f) * s->k0 + f * s->k1);
+}
+
+/* trapzd (nr_2.f:233-254): the n-th refinement of the trapezoid sum in *s */
+static void seg_trapzd(const orc_seg *sg, double a, double b, double *s, int n)
+{
+    if (n == 1) {
+        *s = 0.5 * (b - a) * (seg_f(sg, a) + seg_f(sg, b));
+        return;
+    }
+    const int it = 1 << (n - 2);
+    const double tnm = (double)it;
+    const double del = (b - a) / tnm;
+    double x = a + 0.5 * del, sum = 0.0;
+    for (int j = 1; j <= it; ++j) {
+        sum = sum + seg_f(sg, x);
+        x = x + del;
+    }
+    *s = 0.5 * (*s + (b - a) * sum / tnm);
+}
+
+/* qsimp (nr_2.f:209-231), EPS = 1e-5, JMAX = 20, over [0,1].  Returns the
+ * reference's `converg` flag, which is set when the loop did NOT converge. */
+int32_t orc_qsimp_seg(int32_t kind, double k0, double k1, double nu0, double dnu, double teff, double *out)
+{
+    const orc_seg sg = {kind, k0, k1, nu0, dnu, teff};
+    double ost = -1.0e30, os = -1.0e30, st = 0.0, s = 0.0;
+    for (int j = 1; j <= 20; ++j) {
+        seg_trapzd(&sg, 0.0, 1.0, &st, j);
+        s = (4.0 * st - ost) / 3.0;
+        if (fabs(s - os) < 1.0e-5 * fabs(os)) { *out = s; return 0; }
+        os = s;
+        ost = st;
+    }
+    *out = s;
+    return 1;
+}
+
+/* opacset's kappa and albedo of one species at nu (dust_5.f90:21-55); nud
+ * descending, nl entries */
+static void opac_ka(const double *nud, const double *kap, const double *alb, int nl, double nu,
+                    double *kappa, double *albedo)
+{
+    if (nu >= nud[0]) { *kappa = kap[0]; *albedo = alb[0]; return; }
+    if (nu <= nud[nl - 1]) { *kappa = kap[nl - 1]; *albedo = alb[nl - 1]; return; }
+    int i0 = 1, i1 = nl;                                   /* 1-based, as the reference */
+    while (i1 - i0 > 1) {
+        const int i = (i1 + i0) / 2;
+        if (nu <= nud[i - 1]) i0 = i; else i1 = i;
+    }
+    const double f = (nu - nud[i1 - 1]) / (nud[i0 - 1] - nud[i1 - 1]);
+    const double fm1 = 1.0 - f;
+    *kappa = fm1 * kap[i1 - 1] + f * kap[i0 - 1];
+    *albedo = fm1 * alb[i1 - 1] + f * alb[i0 - 1];
+}
+
+/* setemiss (dust_5.f90:507-699) for every species; tables [id*nlam + k]
+ * (nlambda[id] of them used, all nlam when nlambda is NULL); outputs
+ * kapint[(id*nnu + j)*nT + i] = kapint(i,j,id), kappap/kappar[id*nT + i].
+ * smooth[id] != 0 applies the two 21-point passes the reference gives the
+ * species above ndg (:612-653).  Returns the number of qsimp calls that did
+ * not converge (the reference's counta + countc), or -1 when a smoothed
+ * species has fewer than 21 temperatures. */
+int64_t orc_setemiss(int32_t nopac, int32_t nlam, const int32_t *nlambda, const double *nudust,
+                     const double *kappad, const double *adust, int32_t nT, const double *Tint,
+                     int32_t nnu, const double *nuint, const int32_t *smooth,
+                     double *kapint, double *kappap, double *kappar)
+{
+    int64_t nfail = 0;
+    if (smooth)
+        for (int id = 0; id < nopac; ++id)
+            if (smooth[id] && nT < 21) return -1;    /* the smoothing windows need 21 temperatures */
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)nT);
+    for (int id = 0; id < nopac; ++id) {
+        const int nl = nlambda ? nlambda[id] : nlam;
+        const double *nud = nudust + (size_t)id * nlam;
+        const double *kap = kappad + (size_t)id * nlam;
+        const double *alb = adust + (size_t)id * nlam;
+        double *K = kapint + (size_t)id * nnu * nT;
+        for (int i = 0; i < nT; ++i) {
+            const double teff = Tint[i];
+            double nu1 = 0.0, kappa, albedo;
+            opac_ka(nud, kap, alb, nl, nu1, &kappa, &albedo);
+            double chi1 = kappa, kap1 = (1.0 - albedo) * kappa, kr = 0.0;
+            for (int j = 0; j < nnu; ++j) {
+                const double nu0 = nu1;
+                nu1 = nuint[j] * teff;
+                const double dnu = nu1 - nu0;
+                const double dx = dnu / teff;
+                const double chi0 = chi1, kap0 = kap1;
+                opac_ka(nud, kap, alb, nl, nu1, &kappa, &albedo);
+                chi1 = kappa;
+                kap1 = (1.0 - albedo) * kappa;
+                double sk = 0.0, sc = 0.0;
+                if (nu0 / teff < 50.0) {
+                    nfail += orc_qsimp_seg(0, kap0, kap1, nu0, dnu, teff, &sk);
+                    nfail += orc_qsimp_seg(1, chi0, chi1, nu0, dnu, teff, &sc);
+                }
+                double *k = K + (size_t)j * nT + i;
+                if (j == 0) {
+                    *k = dx * sk;
+                    kr = dx * sc;
+                } else {
+                    *k = dx * sk + K[(size_t)(j - 1) * nT + i];
+                    kr = dx * sc + kr;
+                }
+            }
+            kappar[(size_t)id * nT + i] = 1.0 / kr;
+            const double tot = K[(size_t)(nnu - 1) * nT + i];
+            kappap[(size_t)id * nT + i] = tot;
+            for (int j = 0; j < nnu - 1; ++j) K[(size_t)j * nT + i] = K[(size_t)j * nT + i] / tot;
+            K[(size_t)(nnu - 1) * nT + i] = 1.0;
+        }
+        if (smooth && smooth[id]) {
+            double *kp = kappap + (size_t)id * nT;
+            const int ns = 10;
+            for (int pass = 0; pass < 2; ++pass) {
+                for (int i = 1; i <= nT; ++i) {          /* 1-based i, as the reference */
+                    int lo, hi;
+                    if (i > ns && i <= nT - ns) { lo = i - ns; hi = i + ns; }
+                    else if (i <= ns) { lo = 1; hi = 2 * i + 1; }
+                    else { lo = 2 * i - nT; hi = nT; }
+                    double acc = 0.0;
+                    for (int ii = lo; ii <= hi; ++ii) acc = acc + kp[ii - 1];
+                    tmp[i - 1] = acc / (double)(hi - lo + 1);
+                }
+                memcpy(kp, tmp, sizeof(double) * (size_t)nT);
+            }
+        }
+    }
+    free(tmp);
+    return nfail;
+}

@@ -140,6 +140,18 @@ int32_t orc_run_packets(const orc_model *m, const double *tdust, orc_tallies *t,
                         uint64_t first_packet, int64_t npackets, uint64_t seed,
                         int32_t rng_mode, int32_t nthreads, double *trace);
 
+/* ---- emissivity tables (setemiss, dust_5.f90:507-699) ------------------- */
+double orc_expm1_as(double x);                 /* mathsubs.f90:79-113 */
+double orc_bnu(double nu, double T);           /* mathsubs.f90:4-22 */
+double orc_dbnu(double nu, double T);          /* mathsubs.f90:29-47 */
+/* one interval: qsimp (nr_2.f:209-231) of kappaB (kind 0) or chidB (kind 1);
+ * returns 1 when qsimp did not converge */
+int32_t orc_qsimp_seg(int32_t kind, double k0, double k1, double nu0, double dnu, double teff, double *out);
+int64_t orc_setemiss(int32_t nopac, int32_t nlam, const int32_t *nlambda, const double *nudust,
+                     const double *kappad, const double *adust, int32_t nT, const double *Tint,
+                     int32_t nnu, const double *nuint, const int32_t *smooth,
+                     double *kapint, double *kappap, double *kappar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -104,6 +104,17 @@ def oracle():
         L.orc_run_packets.argtypes = [C.POINTER(OrcModel), _dp, C.POINTER(OrcTallies), C.c_uint64,
                                       C.c_int64, C.c_uint64, C.c_int32, C.c_int32, _dp]
         L.orc_run_packets.restype = C.c_int32
+        for f in (L.orc_expm1_as, ):
+            f.argtypes = [C.c_double]
+            f.restype = C.c_double
+        for f in (L.orc_bnu, L.orc_dbnu):
+            f.argtypes = [C.c_double, C.c_double]
+            f.restype = C.c_double
+        L.orc_qsimp_seg.argtypes = [C.c_int32] + [C.c_double] * 5 + [_dp]
+        L.orc_qsimp_seg.restype = C.c_int32
+        L.orc_setemiss.argtypes = [C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32, _dp, _ip,
+                                   _dp, _dp, _dp]
+        L.orc_setemiss.restype = C.c_int64
         _orc = L
     return _orc
 
@@ -131,6 +142,12 @@ def ref():
         L.ref_bbfreq.restype = C.c_double
         L.ref_bnu.argtypes = [C.c_double, C.c_double]
         L.ref_bnu.restype = C.c_double
+        if hasattr(L, "ref_qsimp_seg"):
+            L.ref_dbnu.argtypes = [C.c_double, C.c_double]
+            L.ref_dbnu.restype = C.c_double
+            L.ref_expm1.argtypes = [C.c_double]
+            L.ref_expm1.restype = C.c_double
+            L.ref_qsimp_seg.argtypes = [C.c_int] + [C.c_double] * 5 + [_dp, C.POINTER(C.c_int)]
         L.ref_open_log()
         _ref = L
     return _ref
@@ -244,3 +261,25 @@ def trace_ray(m, pos, u, ijk, on_wall, tau_stop, max_steps=4096, om=None, stream
                         C.byref(tau), C.byref(st), C.c_uint64(int(stream)))
     return dict(n=n, faces=faces[:max(n, 0)], cells=cells[:max(n, 0)], pos=p, ijk=ij, tau=tau.value,
                 status=st.value)
+
+
+def setemiss(nudust, kappad, adust, Tint, nuint, smooth=None, nlambda=None):
+    """setemiss (dust_5.f90:507-699) for tables of shape (nopac, nlam):
+    returns kapint (nopac, nnu, nT), kappap (nopac, nT), kappar (nopac, nT)
+    and the number of qsimp calls that did not converge."""
+    L = oracle()
+    nud, kap, alb = (np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in (nudust, kappad, adust))
+    nopac, nlam = nud.shape
+    T = np.ascontiguousarray(Tint, dtype=np.float64)
+    nu = np.ascontiguousarray(nuint, dtype=np.float64)
+    nT, nnu = len(T), len(nu)
+    kapint = np.zeros((nopac, nnu, nT))
+    kappap = np.zeros((nopac, nT))
+    kappar = np.zeros((nopac, nT))
+    sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.int32)
+    nl = None if nlambda is None else np.ascontiguousarray(nlambda, dtype=np.int32)
+    nfail = L.orc_setemiss(nopac, nlam, None if nl is None else _p(nl, C.c_int32), _p(nud, C.c_double),
+                           _p(kap, C.c_double), _p(alb, C.c_double), nT, _p(T, C.c_double), nnu, _p(nu, C.c_double),
+                           None if sm is None else _p(sm, C.c_int32), _p(kapint, C.c_double),
+                           _p(kappap, C.c_double), _p(kappar, C.c_double))
+    return kapint, kappap, kappar, int(nfail)

@@ -1,7 +1,8 @@
 ! ref_bindings.f90 -- C-callable wrappers around the reference's own
 ! MPI-free leaf routines (closest_wall.f90, radface.f90, thetaface.f90,
 ! phiface.f90, findangle3.f90, check.f90, nr_2.f:locate, random.f90:bbfreq/ran,
-! ecuyer_cote_mod.f90 via user_set_generator.f90, mathsubs.f90:Bnu).
+! ecuyer_cote_mod.f90 via user_set_generator.f90, mathsubs.f90:Bnu/dBnu/expm1,
+! nr_2.f:qsimp).
 !
 ! TEST INFRASTRUCTURE ONLY: linked with the reference sources (compiled
 ! unmodified from where they lie) into oracle/_ref/libref.so, which the tests
@@ -13,6 +14,8 @@ module ref_bindings
   use random, only : ran, bbfreq
   use user_set_generator, only : user_set_all
   implicit none
+  real(8), save :: s_k0, s_k1, s_nu0, s_dnu, s_teff   ! integrand state of ref_qsimp_seg
+  external qsimp
 contains
 
   subroutine ref_open_log() bind(C, name='ref_open_log')
@@ -125,5 +128,57 @@ contains
     a = nu; b = t
     ref_bnu = Bnu(a, b)
   end function ref_bnu
+
+  real(c_double) function ref_dbnu(nu, t) bind(C, name='ref_dbnu')
+    real(c_double), value :: nu, t
+    real(8) :: dBnu, a, b
+    external dBnu
+    a = nu; b = t
+    ref_dbnu = dBnu(a, b)
+  end function ref_dbnu
+
+  real(c_double) function ref_expm1(x) bind(C, name='ref_expm1')
+    real(c_double), value :: x
+    real(8) :: expm1, a
+    external expm1
+    a = x
+    ref_expm1 = expm1(a)
+  end function ref_expm1
+
+  ! One interval of setemiss (dust_5.f90:560-575): the reference's qsimp
+  ! (nr_2.f:209-231) over f in [0,1] of kappaB (kind 0) or chidB (kind 1).
+  ! The two integrands are dust_5.f90:782-794 and :814-826 restated (that
+  ! file needs mpif.h); Bnu, dBnu, expm1 and qsimp are the reference's.
+  subroutine ref_qsimp_seg(kind, k0, k1, nu0, dnu, teff, s, converg) bind(C, name='ref_qsimp_seg')
+    integer(c_int), value :: kind
+    real(c_double), value :: k0, k1, nu0, dnu, teff
+    real(c_double), intent(out) :: s
+    integer(c_int), intent(out) :: converg
+    real(8) :: a, b, r
+    logical :: cv
+    s_k0 = k0; s_k1 = k1; s_nu0 = nu0; s_dnu = dnu; s_teff = teff
+    a = 0.d0; b = 1.d0
+    if (kind == 0) then
+       call qsimp(seg_kappab, a, b, r, cv)
+    else
+       call qsimp(seg_chidb, a, b, r, cv)
+    end if
+    s = r
+    converg = merge(1, 0, cv)
+  end subroutine ref_qsimp_seg
+
+  real(8) function seg_kappab(f)
+    real(8) :: f, nu, Bnu
+    external Bnu
+    nu = s_nu0 + f * s_dnu
+    seg_kappab = ((1.d0 - f) * s_k0 + f * s_k1) * Bnu(nu, s_teff)
+  end function seg_kappab
+
+  real(8) function seg_chidb(f)
+    real(8) :: f, nu, dBnu
+    external dBnu
+    nu = s_nu0 + f * s_dnu
+    seg_chidb = dBnu(nu, s_teff) / ((1.d0 - f) * s_k0 + f * s_k1)
+  end function seg_chidb
 
 end module ref_bindings

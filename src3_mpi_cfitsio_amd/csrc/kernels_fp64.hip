@@ -4,8 +4,18 @@
 #define MCRT_FP64_ONLY_KERNELS 1
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
+#include "mcrt_emiss.h"
 
 namespace mcrt {
+
+hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s) {
+  const size_t nseg = (size_t)a.nopac * a.nnu * a.nT;
+  hipLaunchKernelGGL(k_emiss_seg, dim3((unsigned)((nseg + 255) / 256)), dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_emiss_scan, dim3((unsigned)a.nopac), dim3(256), 2 * (size_t)a.nT * sizeof(double), s, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
   hipLaunchKernelGGL(k_transport<double>, dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);

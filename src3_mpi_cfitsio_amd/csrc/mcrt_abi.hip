@@ -3,6 +3,7 @@
 #include "../../include/mcrt.h"
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
+#include "mcrt_emiss.h"
 
 #include <algorithm>
 #include <cmath>
@@ -398,6 +399,59 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   c->has_dust = true;
+  return MCRT_OK;
+}
+
+int mcrt_setemiss(mcrt_ctx* c, int32_t nopac, int32_t nlam, const int32_t* nlambda, const double* nudust,
+                  const double* kappad, const double* adust, int32_t nT, const double* Tint, int32_t nnu,
+                  const double* nuint, const int32_t* smooth, double* kapint, double* kappap, double* kappar,
+                  int64_t* nfail, float* ms) {
+  if (!c) return MCRT_E_ARG;
+  if (nopac < 1 || nlam < 2 || nT < 2 || nnu < 2) return fail(c, MCRT_E_ARG, "mcrt_setemiss: nopac >= 1, nlam, nT, nnu >= 2");
+  if (nT > 4096) return fail(c, MCRT_E_ARG, "mcrt_setemiss: nT must be <= 4096 (kappap smoothing in LDS)");
+  if (!nudust || !kappad || !adust || !Tint || !nuint || !kapint || !kappap || !kappar)
+    return fail(c, MCRT_E_ARG, "mcrt_setemiss: null array");
+  std::vector<int> nl(nopac, nlam), sm(nopac, 0);
+  for (int id = 0; id < nopac; ++id) {
+    if (nlambda) nl[id] = nlambda[id];
+    if (smooth) sm[id] = smooth[id] ? 1 : 0;
+    if (nl[id] < 2 || nl[id] > nlam) return fail(c, MCRT_E_ARG, "mcrt_setemiss: nlambda must be 2..nlam");
+    if (sm[id] && nT < 21) return fail(c, MCRT_E_ARG, "mcrt_setemiss: smoothed species need nT >= 21");
+  }
+  for (int i = 0; i < nT; ++i)
+    if (!(Tint[i] > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_setemiss: Tint must be > 0");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const size_t nd = (size_t)nopac * nlam, nseg = (size_t)nopac * nnu * nT, nrow = (size_t)nopac * nT;
+  const size_t sizes[] = {nopac * sizeof(int), nopac * sizeof(int), nd * 8, nd * 8, nd * 8, (size_t)nT * 8,
+                          (size_t)nnu * 8, 2 * nseg * 8, nseg * 8, nrow * 8, nrow * 8, 8};
+  const int nbuf = (int)(sizeof(sizes) / sizeof(sizes[0]));
+  void* p[12] = {nullptr};
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < nbuf && e == hipSuccess; ++k) e = hipMalloc(&p[k], sizes[k]);
+  float t_ms = 0.0f;
+  unsigned long long fails = 0;
+  if (e == hipSuccess) {
+    const void* src[] = {nl.data(), sm.data(), nudust, kappad, adust, Tint, nuint};
+    for (int k = 0; k < 7 && e == hipSuccess; ++k) e = hipMemcpyAsync(p[k], src[k], sizes[k], hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(p[11], 0, 8, c->stream);
+    EmissArgs a{nopac, nlam, nT, nnu, (const int*)p[0], (const double*)p[2], (const double*)p[3], (const double*)p[4],
+                (const double*)p[5], (const double*)p[6], (const int*)p[1], (double*)p[7], (double*)p[8], (double*)p[9],
+                (double*)p[10], (unsigned long long*)p[11]};
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_setemiss(a, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(kapint, p[8], sizes[8], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(kappap, p[9], sizes[9], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(kappar, p[10], sizes[10], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&fails, p[11], 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, c->ev0, c->ev1);
+  }
+  for (int k = 0; k < nbuf; ++k)
+    if (p[k]) (void)hipFree(p[k]);
+  if (e != hipSuccess) return hipfail(c, e, "mcrt_setemiss");
+  if (nfail) *nfail = (int64_t)fails;
+  if (ms) *ms = t_ms;
   return MCRT_OK;
 }
 

@@ -6,6 +6,7 @@
 namespace mcrt {
 
 struct RunParams;
+struct EmissArgs;
 struct TraceArgs;
 struct AdjParams;
 
@@ -47,6 +48,8 @@ hipError_t batch_occupancy_mixed(const BatchLaunch& L, int* blocks_per_cu);
 // Lucy temperatures of every model (npk: packets per model; max_cells: cells of the largest model)
 hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* npk, int mode, double* tdave, size_t max_cells,
                                hipStream_t s);
+// setemiss tables: k_emiss_seg then k_emiss_scan (FP64, contraction off)
+hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s);
 hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);

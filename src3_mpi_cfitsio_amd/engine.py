@@ -336,6 +336,42 @@ class BatchEngine(Engine):
         return tuple(int(x) for x in best), table[:min(nrows.value, 64)]
 
 
+def setemiss(nudust, kappad, adust, Tint, nuint, smooth=None, nlambda=None, device: int = 0) -> Dict[str, object]:
+    """setemiss (dust_5.f90:507-699) on the GPU: the Planck-mean tables of
+    dust tables of shape (nopac, nlam) (nudust descending, eV) at the
+    temperatures Tint (K/11605) and the x = h nu/kT grid nuint.  Returns
+    kapint (nopac, nnu, nT) normalized CDFs, kappap and kappar (nopac, nT),
+    nfail (qsimp calls that did not converge) and the kernel time in ms."""
+    lib = _lib.load()
+    nud, kap, alb = (np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in (nudust, kappad, adust))
+    nopac, nlam = nud.shape
+    if kap.shape != nud.shape or alb.shape != nud.shape:
+        raise ValueError("nudust, kappad and adust must have the same shape")
+    T, nu = _f64(Tint), _f64(nuint)
+    nT, nnu = len(T), len(nu)
+    out = dict(kapint=np.zeros((nopac, nnu, nT)), kappap=np.zeros((nopac, nT)), kappar=np.zeros((nopac, nT)))
+    sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.int32)
+    nl = None if nlambda is None else np.ascontiguousarray(nlambda, dtype=np.int32)
+    nfail, ms = C.c_int64(), C.c_float()
+    h = C.c_void_p()
+    rc = lib.mcrt_create(device, C.byref(h))
+    if rc != 0:
+        raise MCRTError(f"mcrt_create failed with code {rc} (no usable GPU?)")
+    try:
+        rc = lib.mcrt_setemiss(h, nopac, nlam, None if nl is None else ptr(nl, C.c_int32), ptr(nud, C.c_double),
+                               ptr(kap, C.c_double), ptr(alb, C.c_double), nT, ptr(T, C.c_double), nnu,
+                               ptr(nu, C.c_double), None if sm is None else ptr(sm, C.c_int32),
+                               ptr(out["kapint"], C.c_double), ptr(out["kappap"], C.c_double),
+                               ptr(out["kappar"], C.c_double), C.byref(nfail), C.byref(ms))
+        if rc != 0:
+            msg = lib.mcrt_last_error(h)
+            raise MCRTError(f"libmcrt error {rc}: {msg.decode() if msg else ''}")
+    finally:
+        lib.mcrt_destroy(h)
+    out["nfail"], out["ms"] = int(nfail.value), float(ms.value)
+    return out
+
+
 def diskflux(tal: Dict[str, np.ndarray], model: Model, npackets: int, ltot: Optional[float] = None):
     """flux_test.f90:19-80: normalize SEDs by np/nmu/nph/(ltot/lsol) and
     peeled SEDs and images by np/(ltot/lsol); turn the squared sums into
@@ -369,5 +405,5 @@ def diskflux(tal: Dict[str, np.ndarray], model: Model, npackets: int, ltot: Opti
     return out
 
 
-__all__ = ["Engine", "MCRTError", "diskflux", "MCRT_FP64", "MCRT_MIXED", "MCRT_T_REFERENCE",
+__all__ = ["Engine", "MCRTError", "diskflux", "setemiss", "MCRT_FP64", "MCRT_MIXED", "MCRT_T_REFERENCE",
            "MCRT_T_PERCELL", "COUNTERS", "SED_NAMES", "PSED_NAMES", "IMAGE_NAMES"]

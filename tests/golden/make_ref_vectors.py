@@ -131,6 +131,29 @@ def main():
     out["chk_in"], out["chk_out"] = cv, np.array([R.ref_check(float(v)) for v in cv])
     xs = np.linspace(0.0, 40.0, 161)
     out["bnu_x"], out["bnu"] = xs, np.array([R.ref_bnu(float(v), 1.0) for v in xs])
+    # --- setemiss pieces: mathsubs.f90 expm1/dBnu and nr_2.f qsimp over
+    # one interval of kappaB / chidB (dust_5.f90:560-575), including the
+    # first interval (nu0 = 0) and intervals near the x = 32 end
+    ex = np.concatenate([np.linspace(-3.0, 3.0, 241), [0.6931472, -0.6931472, 0.69314721, 1e-12, -1e-9]])
+    out["em1_x"], out["em1"] = ex, np.array([R.ref_expm1(float(v)) for v in ex])
+    dx_ = np.concatenate([np.linspace(0.0, 40.0, 161)[1:], [-0.5, -2.0]])
+    out["dbnu_x"], out["dbnu"] = dx_, np.array([R.ref_dbnu(float(v), 1.0) for v in dx_])
+    nseg = 400
+    seg = np.zeros((nseg, 6))
+    seg[:, 0] = np.arange(nseg) % 2                                   # kind
+    seg[:, 1:3] = rng.uniform(0.01, 5.0, (nseg, 2))                   # k0, k1
+    T = 10.0 ** rng.uniform(-5.0, 1.5, nseg)
+    x0 = rng.uniform(0.0, 31.9, nseg)
+    x0[:20] = 0.0
+    x0[20:40] = 31.9375
+    dxs = rng.uniform(1e-3, 0.07, nseg)
+    seg[:, 3], seg[:, 4], seg[:, 5] = x0 * T, dxs * T, T              # nu0, dnu, teff
+    res, flag = np.zeros(nseg), np.zeros(nseg, np.int32)
+    for k in range(nseg):
+        r, cv = C.c_double(), C.c_int()
+        R.ref_qsimp_seg(int(seg[k, 0]), *[float(v) for v in seg[k, 1:]], C.byref(r), C.byref(cv))
+        res[k], flag[k] = r.value, cv.value
+    out["qs_in"], out["qs_out"], out["qs_flag"] = seg, res, flag
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
