@@ -118,8 +118,9 @@ def run_batch_workload(args):
     from src3_mpi_cfitsio_amd import model as M
     from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, BatchEngine
     t0 = time.perf_counter()
-    b = M.batch_models(512, seed=1002)
-    log(f"512 models, {sum(x.ncell for x in b.models)} cells, built in {time.perf_counter()-t0:.1f}s")
+    b = M.batch_models(512, seed=1002, device=True)
+    log(f"512 models, {sum(x.ncell for x in b.models)} cells, built in {time.perf_counter()-t0:.1f}s "
+        f"(cell arrays on the GPU)")
     e = BatchEngine(b)
     for k in range(2):
         e.run_batch(100_000, seed=500 + k, precision=MCRT_MIXED)
@@ -208,6 +209,66 @@ def run_adjoint_workload(args):
     print(json.dumps(out), flush=True)
 
 
+EMISS_OPS_PER_EVAL = 38     # FP64 operations per integrand evaluation, counted from the source (exp, / as one)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def run_setemiss_workload(args):
+    """setemiss (the Planck-mean emissivity tables, qsimp per interval) for
+    --species dust species (default 187: a 384 MB kapint table) at nT = 500
+    temperatures and nnu = 513 frequency intervals, against the C oracle on
+    one host core over the same species."""
+    from src3_mpi_cfitsio_amd import model as M
+    from src3_mpi_cfitsio_amd.engine import setemiss
+    nsp = args.species
+    rng = np.random.default_rng(1004)
+    sp = [M.synthetic_dust(nlam=100, kappa0=rng.uniform(50, 400), slope=rng.uniform(0.5, 2.0),
+                           albedo=(rng.uniform(0.3, 0.7), rng.uniform(0.0, 0.1))) for _ in range(nsp)]
+    nud, kap, alb = (np.array([d[k] for d in sp]) for k in ("nudust", "kappad", "adust"))
+    em = M.emission_tables(sp[0], nT=500, nnu=513, nodes=2)
+    T, nu = em["Tint"], em["nuint"]
+    smooth = np.zeros(nsp, np.int32)
+    smooth[nsp // 2:] = 1                                  # the species above ndg are smoothed
+    setemiss(nud[:2], kap[:2], alb[:2], T, nu, smooth=smooth[:2])      # load the kernels
+    walls, kms = [], []
+    for s in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        r = setemiss(nud, kap, alb, T, nu, smooth=smooth)
+        dt = time.perf_counter() - t0
+        log(f"step {s}: {nsp} species in {dt*1e3:.1f} ms (kernels {r['ms']:.2f} ms), "
+            f"{r['nevals']/(nsp*500*513):.2f} evaluations per interval, {r['nfail']} qsimp non-converged")
+        if s >= args.warmup:
+            walls.append(dt)
+            kms.append(r["ms"])
+    kms_mean = float(np.mean(kms))
+    ops = r["nevals"] * EMISS_OPS_PER_EVAL
+    achieved = ops / (kms_mean / 1e3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py
+        t0 = time.perf_counter()
+        oracle_py.setemiss(nud, kap, alb, T, nu, smooth=smooth)
+        dt = time.perf_counter() - t0
+        cpu = {"value": nsp / dt, "unit": "species tables/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 C oracle (qsimp per interval, as dust_5.f90:507-699), all {nsp} species in "
+                         f"{dt:.1f}s on one host thread"}
+        log(f"cpu baseline: {nsp} species in {dt:.2f}s")
+    out = {"metric": "setemiss species tables/s (500 temperatures x 513 intervals, qsimp per interval)",
+           "value": nsp / (kms_mean / 1e3), "unit": "species tables/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": kms_mean, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"setemiss: {nsp} dust species x 500 temperatures x 513 intervals, "
+                                  f"{nsp - nsp // 2} smoothed", "wall_ms_per_step_with_copies": 1e3 * float(np.mean(walls)),
+                      "parallelism": "1 GPU"},
+           "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                        "kernel": "k_emiss_seg + k_emiss_scan", "evaluations": r["nevals"],
+                        "ops_per_evaluation": EMISS_OPS_PER_EVAL},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,8 +280,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes on the GPU")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"],
-                    help="c1: the headline single-model metric; c2: the 512-model batch; c3: the image gradient")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "setemiss"],
+                    help="c1: the headline single-model metric; c2: the 512-model batch; c3: the image gradient; "
+                         "setemiss: the emissivity tables")
+    ap.add_argument("--species", type=int, default=187, help="dust species of the setemiss workload")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -241,6 +304,8 @@ def main():
         return run_batch_workload(args)
     if args.workload == "c3":
         return run_adjoint_workload(args)
+    if args.workload == "setemiss":
+        return run_setemiss_workload(args)
 
     from src3_mpi_cfitsio_amd import model as M
     from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine

@@ -80,13 +80,34 @@ int mcrt_set_dust(mcrt_ctx *ctx, int32_t nopac, int32_t nlam,
  * NULL: none).  Outputs (host): kapint[(iop*nnu + j)*nT + i], kappap and
  * kappar [iop*nT + i] (kappar = the inverse Rosseland sum, as setemiss
  * leaves it).  nfail (optional) = qsimp calls that did not converge (the
- * reference's counta + countc); ms (optional) = kernel time.  Replaces the
+ * reference's counta + countc); nevals (optional) = integrand evaluations;
+ * ms (optional) = kernel time.  Replaces the
  * call `setemiss()` in dustinit (dust_5.f90:499) and its MPI split. */
 int mcrt_setemiss(mcrt_ctx *ctx, int32_t nopac, int32_t nlam, const int32_t *nlambda,
                   const double *nudust, const double *kappad, const double *adust,
                   int32_t nT, const double *Tint, int32_t nnu, const double *nuint,
                   const int32_t *smooth, double *kapint, double *kappap, double *kappar,
-                  int64_t *nfail, float *ms);
+                  int64_t *nfail, int64_t *nevals, float *ms);
+
+/* Cell arrays of nmodels synthetic models in one launch: gridset's cell
+ * loop (gridset_5_model13new.f90:1105-1180) with cell volumes, the flared
+ * disk of the synthetic workloads normalized to its mass, the analytic
+ * Ulrich envelope of densenv (densenv.f90:30-45, 69-73, 111-113; zerod.f90)
+ * at the cell centres, an optional clumping factor per cell, and the
+ * species (1 disk-dominated, 3 envelope, 0 empty).  Walls of model k
+ * (nrg[k], ntg[k], npg[k] of them, R* and radians) follow those of model
+ * k-1 in rarr/thetarr/phiarr; cells likewise, each model in the cell order
+ * of mcrt_set_grid.  params[k*12 + i]: rmincgs (R* in cm), autors (R* per
+ * AU), disk Rin and Rout (AU), h0 (AU at 1 AU), flaring exponent, radial
+ * exponent, envelope Rc (R*), rhoe0 (g/cm^3), envelope inner radius (R*),
+ * disk mass (g), ambient density floor (g/cm^3).  clump may be NULL.
+ * nflag = cells where zerod found no single streamline root (density NaN).
+ * No counterpart in the reference as a whole: it replaces the grid set-up
+ * of analytic models, one model per program run. */
+int mcrt_build_models(mcrt_ctx *ctx, int32_t nmodels, const int32_t *nrg, const int32_t *ntg,
+                      const int32_t *npg, const double *rarr, const double *thetarr,
+                      const double *phiarr, const double *params, const double *clump,
+                      double *rho, uint8_t *species, double *vcell, int64_t *nflag, float *ms);
 
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);

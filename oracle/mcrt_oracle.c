@@ -1474,3 +1474,76 @@ int64_t orc_setemiss(int32_t nopac, int32_t nlam, const int32_t *nlambda, const 
     free(tmp);
     return nfail;
 }
+
+/* ======================================================================
+ * Model building: gridset's cell loop with the analytic Ulrich envelope
+ * ====================================================================== */
+
+/* zerod.f90:3-69: the root in (0,1] of xmu0^3 + (z-1) xmu0 - z xmu by the
+ * trigonometric / Cardano forms.  Returns 0 when *xmu0 was set, 1 in the
+ * reference's "2 roots" case and 2 when no root is positive (both leave
+ * *xmu0 as it was). */
+int32_t orc_zerod(double xmu, double z, double *xmu0)
+{
+    const double pi = 3.1415926535897932384626433;
+    const double a1 = 0.0, a2 = z - 1.0, a3 = -z * xmu;
+    const double q = (a1 * a1 - 3.0 * a2) / 9.0;
+    const double r = (2.0 * (a1 * a1 * a1) - 9.0 * a1 * a2 + 27.0 * a3) / 54.0;
+    const double q3 = q * q * q;
+    if (q3 - r * r >= 0.0) {
+        const double qh = sqrt(q);
+        const double denom = sqrt(q3);
+        const double theta = acos(r / denom);
+        const double x1 = -2.0 * qh * cos(theta / 3.0) - a1 / 3.0;
+        const double x2 = -2.0 * qh * cos((theta + 2.0 * pi) / 3.0) - a1 / 3.0;
+        const double x3 = -2.0 * qh * cos((theta + 4.0 * pi) / 3.0) - a1 / 3.0;
+        if ((x1 > 0.0 && x2 > 0.0) || (x1 > 0.0 && x3 > 0.0) || (x2 > 0.0 && x3 > 0.0)) return 1;
+        int set = 0;
+        if (x1 > 0.0) { *xmu0 = x1; set = 1; }
+        if (x2 > 0.0) { *xmu0 = x2; set = 1; }
+        if (x3 > 0.0) { *xmu0 = x3; set = 1; }
+        return set ? 0 : 2;
+    }
+    const double factor = sqrt(r * r - q3) + fabs(r);
+    const double factor3 = pow(factor, 1.0 / 3.0);
+    *xmu0 = -1.0 * copysign(1.0, r) * (factor3 + q / factor3) - a1 / 3.0;
+    return 0;
+}
+
+/* densenv.f90:30-45, 69-73, 111-113 without holes, gaps or bubbles
+ * (ihole = igape = ibub = 0): the Ulrich (1976) infall density at radius
+ * rad (R*) and polar cos/sin of a cell centre, rc the centrifugal radius,
+ * rhoe0 the density scale, zero inside rmine and beyond rmax, floored at
+ * rhoamb.  *flag = zerod's return (NaN density when it is not 0). */
+double orc_densenv(double rad, double costin, double sina, double rc, double rhoe0, double rmine, double rmax,
+                   double rhoamb, int32_t *flag)
+{
+    double dens = 0.0;
+    *flag = 0;
+    const double cosa = fabs(costin);
+    const double zp = rad * cosa;
+    (void)sina;
+    const double xmu = zp / rad;
+    if (rad > rmine) {
+        const double rx = rad / rc;
+        double xmu0 = NAN;
+        *flag = orc_zerod(xmu, rx, &xmu0);
+        if (*flag) xmu0 = NAN;
+        const double factor = 1.0 / ((xmu / xmu0) + 2.0 * (xmu0 * xmu0) / rx);
+        dens = rhoe0 * pow(rx, -1.5) * factor / sqrt(1.0 + xmu / xmu0);
+    }
+    if (rad > rmine && dens < rhoamb) dens = rhoamb;
+    if (rad >= rmax) dens = 0.0;
+    if (rad <= rmine) dens = 0.0;
+    return dens;
+}
+
+/* gridset_5_model13new.f90:1108,1127,1146-1148: the volume of cell
+ * (ir,it,ip) in cm^3, walls in R*, rmincgs = R* in cm */
+double orc_cell_volume(double r1, double r2, double t1, double t2, double p1, double p2, double rmincgs)
+{
+    const double dr3 = pow(r2, 3.0) - pow(r1, 3.0);
+    const double dcost = cos(t2) - cos(t1);
+    const double dp = p2 - p1;
+    return -(dr3 * dcost * dp / 3.0 * pow(rmincgs, 3.0));
+}

@@ -152,6 +152,12 @@ int64_t orc_setemiss(int32_t nopac, int32_t nlam, const int32_t *nlambda, const 
                      int32_t nnu, const double *nuint, const int32_t *smooth,
                      double *kapint, double *kappap, double *kappar);
 
+/* ---- model building (gridset cell loop, densenv, zerod) ------------------ */
+int32_t orc_zerod(double xmu, double z, double *xmu0);               /* zerod.f90 */
+double orc_densenv(double rad, double costin, double sina, double rc, double rhoe0, double rmine, double rmax,
+                   double rhoamb, int32_t *flag);                     /* densenv.f90 core */
+double orc_cell_volume(double r1, double r2, double t1, double t2, double p1, double p2, double rmincgs);
+
 #ifdef __cplusplus
 }
 #endif

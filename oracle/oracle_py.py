@@ -115,6 +115,12 @@ def oracle():
         L.orc_setemiss.argtypes = [C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32, _dp, _ip,
                                    _dp, _dp, _dp]
         L.orc_setemiss.restype = C.c_int64
+        L.orc_zerod.argtypes = [C.c_double, C.c_double, _dp]
+        L.orc_zerod.restype = C.c_int32
+        L.orc_densenv.argtypes = [C.c_double] * 8 + [_ip]
+        L.orc_densenv.restype = C.c_double
+        L.orc_cell_volume.argtypes = [C.c_double] * 7
+        L.orc_cell_volume.restype = C.c_double
         _orc = L
     return _orc
 
@@ -148,6 +154,8 @@ def ref():
             L.ref_expm1.argtypes = [C.c_double]
             L.ref_expm1.restype = C.c_double
             L.ref_qsimp_seg.argtypes = [C.c_int] + [C.c_double] * 5 + [_dp, C.POINTER(C.c_int)]
+        if hasattr(L, "ref_zerod"):
+            L.ref_zerod.argtypes = [C.c_double, C.c_double, _dp, C.POINTER(C.c_int)]
         L.ref_open_log()
         _ref = L
     return _ref

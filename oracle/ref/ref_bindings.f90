@@ -2,7 +2,7 @@
 ! MPI-free leaf routines (closest_wall.f90, radface.f90, thetaface.f90,
 ! phiface.f90, findangle3.f90, check.f90, nr_2.f:locate, random.f90:bbfreq/ran,
 ! ecuyer_cote_mod.f90 via user_set_generator.f90, mathsubs.f90:Bnu/dBnu/expm1,
-! nr_2.f:qsimp).
+! nr_2.f:qsimp, zerod.f90).
 !
 ! TEST INFRASTRUCTURE ONLY: linked with the reference sources (compiled
 ! unmodified from where they lie) into oracle/_ref/libref.so, which the tests
@@ -180,5 +180,19 @@ contains
     nu = s_nu0 + f * s_dnu
     seg_chidb = dBnu(nu, s_teff) / ((1.d0 - f) * s_k0 + f * s_k1)
   end function seg_chidb
+
+  ! zerod.f90: the Ulrich streamline cubic.  xmu0 is written only when the
+  ! routine finds its root; iflag is 1 in its "2 roots" case.
+  subroutine ref_zerod(xmu, z, xmu0, iflag) bind(C, name='ref_zerod')
+    real(c_double), value :: xmu, z
+    real(c_double), intent(inout) :: xmu0
+    integer(c_int), intent(out) :: iflag
+    real(8) :: a, b, r
+    integer :: fl
+    a = xmu; b = z; r = xmu0; fl = 0
+    call zerod(a, b, r, fl)
+    xmu0 = r
+    iflag = fl
+  end subroutine ref_zerod
 
 end module ref_bindings

@@ -5,8 +5,20 @@
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
 #include "mcrt_emiss.h"
+#include "mcrt_build.h"
 
 namespace mcrt {
+
+hipError_t launch_build(const BuildArgs& a, hipStream_t s) {
+  const dim3 grid(a.nblk, (unsigned)a.nmodels);
+  hipLaunchKernelGGL(k_build_cells, grid, dim3(256), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_build_mass, dim3((unsigned)a.nmodels), dim3(256), 0, s, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_build_finish, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s) {
   const size_t nseg = (size_t)a.nopac * a.nnu * a.nT;

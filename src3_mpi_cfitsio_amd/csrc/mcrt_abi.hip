@@ -4,6 +4,7 @@
 #include "mcrt_kernels.h"
 #include "mcrt_launch.h"
 #include "mcrt_emiss.h"
+#include "mcrt_build.h"
 
 #include <algorithm>
 #include <cmath>
@@ -405,7 +406,7 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
 int mcrt_setemiss(mcrt_ctx* c, int32_t nopac, int32_t nlam, const int32_t* nlambda, const double* nudust,
                   const double* kappad, const double* adust, int32_t nT, const double* Tint, int32_t nnu,
                   const double* nuint, const int32_t* smooth, double* kapint, double* kappap, double* kappar,
-                  int64_t* nfail, float* ms) {
+                  int64_t* nfail, int64_t* nevals, float* ms) {
   if (!c) return MCRT_E_ARG;
   if (nopac < 1 || nlam < 2 || nT < 2 || nnu < 2) return fail(c, MCRT_E_ARG, "mcrt_setemiss: nopac >= 1, nlam, nT, nnu >= 2");
   if (nT > 4096) return fail(c, MCRT_E_ARG, "mcrt_setemiss: nT must be <= 4096 (kappap smoothing in LDS)");
@@ -423,34 +424,97 @@ int mcrt_setemiss(mcrt_ctx* c, int32_t nopac, int32_t nlam, const int32_t* nlamb
   HIPCHECK(c, hipSetDevice(c->device));
   const size_t nd = (size_t)nopac * nlam, nseg = (size_t)nopac * nnu * nT, nrow = (size_t)nopac * nT;
   const size_t sizes[] = {nopac * sizeof(int), nopac * sizeof(int), nd * 8, nd * 8, nd * 8, (size_t)nT * 8,
-                          (size_t)nnu * 8, 2 * nseg * 8, nseg * 8, nrow * 8, nrow * 8, 8};
+                          (size_t)nnu * 8, 2 * nseg * 8, nseg * 8, nrow * 8, nrow * 8, 16};
   const int nbuf = (int)(sizeof(sizes) / sizeof(sizes[0]));
   void* p[12] = {nullptr};
   hipError_t e = hipSuccess;
   for (int k = 0; k < nbuf && e == hipSuccess; ++k) e = hipMalloc(&p[k], sizes[k]);
   float t_ms = 0.0f;
-  unsigned long long fails = 0;
+  unsigned long long counts[2] = {0, 0};      // non-converged qsimp calls, integrand evaluations
   if (e == hipSuccess) {
     const void* src[] = {nl.data(), sm.data(), nudust, kappad, adust, Tint, nuint};
     for (int k = 0; k < 7 && e == hipSuccess; ++k) e = hipMemcpyAsync(p[k], src[k], sizes[k], hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(p[11], 0, 8, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(p[11], 0, 16, c->stream);
     EmissArgs a{nopac, nlam, nT, nnu, (const int*)p[0], (const double*)p[2], (const double*)p[3], (const double*)p[4],
                 (const double*)p[5], (const double*)p[6], (const int*)p[1], (double*)p[7], (double*)p[8], (double*)p[9],
-                (double*)p[10], (unsigned long long*)p[11]};
+                (double*)p[10], (unsigned long long*)p[11], (unsigned long long*)p[11] + 1};
     if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
     if (e == hipSuccess) e = launch_setemiss(a, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(kapint, p[8], sizes[8], hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(kappap, p[9], sizes[9], hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(kappar, p[10], sizes[10], hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(&fails, p[11], 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(counts, p[11], 16, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, c->ev0, c->ev1);
   }
   for (int k = 0; k < nbuf; ++k)
     if (p[k]) (void)hipFree(p[k]);
   if (e != hipSuccess) return hipfail(c, e, "mcrt_setemiss");
-  if (nfail) *nfail = (int64_t)fails;
+  if (nfail) *nfail = (int64_t)counts[0];
+  if (nevals) *nevals = (int64_t)counts[1];
+  if (ms) *ms = t_ms;
+  return MCRT_OK;
+}
+
+int mcrt_build_models(mcrt_ctx* c, int32_t nmodels, const int32_t* nrg, const int32_t* ntg, const int32_t* npg,
+                      const double* rarr, const double* thetarr, const double* phiarr, const double* params,
+                      const double* clump, double* rho, uint8_t* species, double* vcell, int64_t* nflag, float* ms) {
+  if (!c) return MCRT_E_ARG;
+  if (nmodels < 1 || nmodels > 65535) return fail(c, MCRT_E_ARG, "mcrt_build_models: nmodels must be 1..65535");
+  if (!nrg || !ntg || !npg || !rarr || !thetarr || !phiarr || !params || !rho || !species || !vcell)
+    return fail(c, MCRT_E_ARG, "mcrt_build_models: null array");
+  std::vector<int> nr(nmodels), nt(nmodels), np(nmodels);
+  std::vector<long long> roff(nmodels), toff(nmodels), poff(nmodels), coff(nmodels);
+  long long nrw = 0, ntw = 0, npw = 0, ncell = 0, maxc = 0;
+  for (int k = 0; k < nmodels; ++k) {
+    if (nrg[k] < 2 || ntg[k] < 2 || npg[k] < 2) return fail(c, MCRT_E_ARG, "mcrt_build_models: nrg, ntg, npg must be >= 2");
+    nr[k] = nrg[k] - 1; nt[k] = ntg[k] - 1; np[k] = npg[k] - 1;
+    roff[k] = nrw; toff[k] = ntw; poff[k] = npw; coff[k] = ncell;
+    nrw += nrg[k]; ntw += ntg[k]; npw += npg[k];
+    const long long n = (long long)nr[k] * nt[k] * np[k];
+    ncell += n;
+    maxc = std::max(maxc, n);
+  }
+  const unsigned nblk = (unsigned)((maxc + 255) / 256);
+  if ((long long)nblk * 256 < maxc || nblk > 0x7fffffffu) return fail(c, MCRT_E_ARG, "mcrt_build_models: model too large");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const size_t isz = (size_t)nmodels * sizeof(int), lsz = (size_t)nmodels * sizeof(long long);
+  const size_t sizes[] = {isz, isz, isz, lsz, lsz, lsz, lsz, (size_t)nrw * 8, (size_t)ntw * 8, (size_t)npw * 8,
+                          (size_t)nmodels * kBuildPar * 8, clump ? (size_t)ncell * 8 : 8, (size_t)ncell * 8,
+                          (size_t)ncell * 8, (size_t)ncell * 8, (size_t)ncell, (size_t)nmodels * nblk * 8,
+                          (size_t)nmodels * 8, 8};
+  const int nbuf = (int)(sizeof(sizes) / sizeof(sizes[0]));
+  void* p[19] = {nullptr};
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < nbuf && e == hipSuccess; ++k) e = hipMalloc(&p[k], sizes[k]);
+  float t_ms = 0.0f;
+  unsigned long long flags = 0;
+  if (e == hipSuccess) {
+    const void* src[] = {nr.data(), nt.data(), np.data(), roff.data(), toff.data(), poff.data(), coff.data(),
+                         rarr, thetarr, phiarr, params, clump};
+    for (int k = 0; k < 12 && e == hipSuccess; ++k)
+      if (src[k]) e = hipMemcpyAsync(p[k], src[k], sizes[k], hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(p[18], 0, 8, c->stream);
+    BuildArgs a{nmodels, nblk, (const int*)p[0], (const int*)p[1], (const int*)p[2], (const long long*)p[3],
+                (const long long*)p[4], (const long long*)p[5], (const long long*)p[6], (const double*)p[7],
+                (const double*)p[8], (const double*)p[9], (const double*)p[10], clump ? (const double*)p[11] : nullptr,
+                (double*)p[12], (double*)p[13], (double*)p[14], (uint8_t*)p[15], (double*)p[16], (double*)p[17],
+                (unsigned long long*)p[18]};
+    if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (e == hipSuccess) e = launch_build(a, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rho, p[12], sizes[12], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(vcell, p[13], sizes[13], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(species, p[15], sizes[15], hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flags, p[18], 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t_ms, c->ev0, c->ev1);
+  }
+  for (int k = 0; k < nbuf; ++k)
+    if (p[k]) (void)hipFree(p[k]);
+  if (e != hipSuccess) return hipfail(c, e, "mcrt_build_models");
+  if (nflag) *nflag = (int64_t)flags;
   if (ms) *ms = t_ms;
   return MCRT_OK;
 }

@@ -35,6 +35,7 @@ struct EmissArgs {
   double* seg;                               // [2][nopac*nnu*nT]: kappaB, chidB interval integrals
   double *kapint, *kappap, *kappar;          // [(id*nnu + j)*nT + i], [id*nT + i], [id*nT + i]
   unsigned long long* nfail;                 // qsimp calls that did not converge
+  unsigned long long* nevals;                // integrand evaluations (null: not counted)
 };
 
 #ifdef MCRT_FP64_ONLY_KERNELS
@@ -73,11 +74,12 @@ __device__ __forceinline__ double em_f(const EmSeg& s, double f) {
 }
 
 // qsimp + trapzd over [0,1]; returns true when the loop did not converge
-// (the reference's `converg`)
+// (the reference's `converg`).  *levels = trapzd refinements done.
 template <int kKind>
-__device__ bool em_qsimp(const EmSeg& s, double* out) {
+__device__ bool em_qsimp(const EmSeg& s, double* out, int* levels) {
   double ost = -1.0e30, os = -1.0e30, st = 0.0, r = 0.0;
   for (int n = 1; n <= 20; ++n) {
+    *levels = n;
     if (n == 1) {
       st = 0.5 * (1.0 - 0.0) * (em_f<kKind>(s, 0.0) + em_f<kKind>(s, 1.0));
     } else {
@@ -123,6 +125,7 @@ __global__ void __launch_bounds__(256) k_emiss_seg(EmissArgs a) {
   const size_t nseg = (size_t)a.nopac * a.nnu * a.nT;
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned fails = 0;
+  unsigned long long evals = 0;
   if (g < nseg) {
     const int i = (int)(g % (size_t)a.nT);
     const size_t r = g / (size_t)a.nT;
@@ -141,13 +144,19 @@ __global__ void __launch_bounds__(256) k_emiss_seg(EmissArgs a) {
     em_opac(nud, kap, alb, nl, nu1, c1, al1);
     double sk = 0.0, sc = 0.0;
     if (nu0 / teff < 50.0) {
-      fails += em_qsimp<0>(EmSeg{(1.0 - al0) * c0, (1.0 - al1) * c1, nu0, dnu, teff}, &sk) ? 1u : 0u;
-      fails += em_qsimp<1>(EmSeg{c0, c1, nu0, dnu, teff}, &sc) ? 1u : 0u;
+      int lk = 0, lc = 0;
+      fails += em_qsimp<0>(EmSeg{(1.0 - al0) * c0, (1.0 - al1) * c1, nu0, dnu, teff}, &sk, &lk) ? 1u : 0u;
+      fails += em_qsimp<1>(EmSeg{c0, c1, nu0, dnu, teff}, &sc, &lc) ? 1u : 0u;
+      evals = (1ull << (lk - 1)) + (1ull << (lc - 1)) + 2ull;    // 2^(n-1) + 1 per qsimp of n levels
     }
     a.seg[g] = sk;
     a.seg[nseg + g] = sc;
   }
   if (fails) atomicAdd(a.nfail, (unsigned long long)fails);
+  if (a.nevals) {
+    for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
+    if (__lane_id() == 0) atomicAdd(a.nevals, evals);
+  }
 }
 
 // one block per species; dynamic LDS: 2 nT doubles.  Smoothed species need

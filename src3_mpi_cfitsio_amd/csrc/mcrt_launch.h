@@ -7,6 +7,7 @@ namespace mcrt {
 
 struct RunParams;
 struct EmissArgs;
+struct BuildArgs;
 struct TraceArgs;
 struct AdjParams;
 
@@ -50,6 +51,8 @@ hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* n
                                hipStream_t s);
 // setemiss tables: k_emiss_seg then k_emiss_scan (FP64, contraction off)
 hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s);
+// model cell arrays: k_build_cells, k_build_mass, k_build_finish (FP64, contraction off)
+hipError_t launch_build(const BuildArgs& a, hipStream_t s);
 hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);

@@ -341,7 +341,8 @@ def setemiss(nudust, kappad, adust, Tint, nuint, smooth=None, nlambda=None, devi
     dust tables of shape (nopac, nlam) (nudust descending, eV) at the
     temperatures Tint (K/11605) and the x = h nu/kT grid nuint.  Returns
     kapint (nopac, nnu, nT) normalized CDFs, kappap and kappar (nopac, nT),
-    nfail (qsimp calls that did not converge) and the kernel time in ms."""
+    nfail (qsimp calls that did not converge), nevals (integrand
+    evaluations) and the kernel time in ms."""
     lib = _lib.load()
     nud, kap, alb = (np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64) for a in (nudust, kappad, adust))
     nopac, nlam = nud.shape
@@ -352,7 +353,7 @@ def setemiss(nudust, kappad, adust, Tint, nuint, smooth=None, nlambda=None, devi
     out = dict(kapint=np.zeros((nopac, nnu, nT)), kappap=np.zeros((nopac, nT)), kappar=np.zeros((nopac, nT)))
     sm = None if smooth is None else np.ascontiguousarray(smooth, dtype=np.int32)
     nl = None if nlambda is None else np.ascontiguousarray(nlambda, dtype=np.int32)
-    nfail, ms = C.c_int64(), C.c_float()
+    nfail, nevals, ms = C.c_int64(), C.c_int64(), C.c_float()
     h = C.c_void_p()
     rc = lib.mcrt_create(device, C.byref(h))
     if rc != 0:
@@ -362,14 +363,59 @@ def setemiss(nudust, kappad, adust, Tint, nuint, smooth=None, nlambda=None, devi
                                ptr(kap, C.c_double), ptr(alb, C.c_double), nT, ptr(T, C.c_double), nnu,
                                ptr(nu, C.c_double), None if sm is None else ptr(sm, C.c_int32),
                                ptr(out["kapint"], C.c_double), ptr(out["kappap"], C.c_double),
-                               ptr(out["kappar"], C.c_double), C.byref(nfail), C.byref(ms))
+                               ptr(out["kappar"], C.c_double), C.byref(nfail), C.byref(nevals), C.byref(ms))
         if rc != 0:
             msg = lib.mcrt_last_error(h)
             raise MCRTError(f"libmcrt error {rc}: {msg.decode() if msg else ''}")
     finally:
         lib.mcrt_destroy(h)
-    out["nfail"], out["ms"] = int(nfail.value), float(ms.value)
+    out["nfail"], out["nevals"], out["ms"] = int(nfail.value), int(nevals.value), float(ms.value)
     return out
+
+
+def build_models(walls, params, clumps=None, device: int = 0):
+    """Cell arrays of many models in one GPU launch (mcrt_build_models):
+    walls = [(rarr, thetarr, phiarr), ...] (R*, radians), params = array
+    (nmodels, 12) in the order mcrt_build_models documents, clumps = per
+    model a clumping factor of the cell shape, or None.  Returns
+    ([(rho, species, vcell), ...] of shape (np-1, nt-1, nr-1), kernel ms)."""
+    lib = _lib.load()
+    n = len(walls)
+    P = np.ascontiguousarray(params, dtype=np.float64).reshape(n, 12)
+    dims = np.array([[len(w[0]), len(w[1]), len(w[2])] for w in walls], np.int32)
+    nrg, ntg, npg = (np.ascontiguousarray(dims[:, i]) for i in range(3))
+    R, T, F = (np.ascontiguousarray(np.concatenate([_f64(w[i]) for w in walls])) for i in range(3))
+    shapes = [(int(d[2]) - 1, int(d[1]) - 1, int(d[0]) - 1) for d in dims]
+    sizes = [a * b * c for a, b, c in shapes]
+    ncell = int(sum(sizes))
+    clump = None
+    if clumps is not None:
+        clump = np.ascontiguousarray(np.concatenate([_f64(np.broadcast_to(1.0 if c is None else c, sh)).ravel()
+                                                     for c, sh in zip(clumps, shapes)]))
+    rho, vcell = np.zeros(ncell), np.zeros(ncell)
+    species = np.zeros(ncell, np.uint8)
+    nflag, ms = C.c_int64(), C.c_float()
+    h = C.c_void_p()
+    rc = lib.mcrt_create(device, C.byref(h))
+    if rc != 0:
+        raise MCRTError(f"mcrt_create failed with code {rc} (no usable GPU?)")
+    try:
+        rc = lib.mcrt_build_models(h, n, ptr(nrg, C.c_int32), ptr(ntg, C.c_int32), ptr(npg, C.c_int32),
+                                   ptr(R, C.c_double), ptr(T, C.c_double), ptr(F, C.c_double), ptr(P, C.c_double),
+                                   None if clump is None else ptr(clump, C.c_double), ptr(rho, C.c_double),
+                                   ptr(species, C.c_uint8), ptr(vcell, C.c_double), C.byref(nflag), C.byref(ms))
+        if rc != 0:
+            msg = lib.mcrt_last_error(h)
+            raise MCRTError(f"libmcrt error {rc}: {msg.decode() if msg else ''}")
+    finally:
+        lib.mcrt_destroy(h)
+    if nflag.value:
+        raise MCRTError(f"mcrt_build_models: zerod found no single streamline root in {nflag.value} cells")
+    out, o = [], 0
+    for sh, sz in zip(shapes, sizes):
+        out.append((rho[o:o + sz].reshape(sh), species[o:o + sz].reshape(sh), vcell[o:o + sz].reshape(sh)))
+        o += sz
+    return out, float(ms.value)
 
 
 def diskflux(tal: Dict[str, np.ndarray], model: Model, npackets: int, ltot: Optional[float] = None):
@@ -405,5 +451,5 @@ def diskflux(tal: Dict[str, np.ndarray], model: Model, npackets: int, ltot: Opti
     return out
 
 
-__all__ = ["Engine", "MCRTError", "diskflux", "setemiss", "MCRT_FP64", "MCRT_MIXED", "MCRT_T_REFERENCE",
+__all__ = ["Engine", "MCRTError", "diskflux", "setemiss", "build_models", "MCRT_FP64", "MCRT_MIXED", "MCRT_T_REFERENCE",
            "MCRT_T_PERCELL", "COUNTERS", "SED_NAMES", "PSED_NAMES", "IMAGE_NAMES"]

@@ -189,23 +189,65 @@ def cell_volumes(rarr, thetarr, phiarr, rstar_rsun):
 # ---------------------------------------------------------------------------
 # densities
 # ---------------------------------------------------------------------------
+def zerod(xmu, z):
+    """zerod.f90:3-69 elementwise: the root in (0,1] of
+    xmu0^3 + (z-1) xmu0 - z xmu = 0 by the trigonometric form (three real
+    roots; the last positive one) or Cardano's (one).  Returns (xmu0, flag):
+    flag 1 where the reference reports two positive roots, 2 where none is
+    positive; xmu0 is NaN there (the reference leaves it unset)."""
+    xmu = np.asarray(xmu, dtype=np.float64)
+    z = np.asarray(z, dtype=np.float64)
+    a1 = 0.0
+    a2 = z - 1.0
+    a3 = -z * xmu
+    q = (a1 * a1 - 3.0 * a2) / 9.0
+    r = (2.0 * (a1 * a1 * a1) - 9.0 * a1 * a2 + 27.0 * a3) / 54.0
+    q3 = q * q * q
+    three = (q3 - r * r) >= 0.0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        qh = np.sqrt(q)
+        theta = np.arccos(r / np.sqrt(q3))
+        x1 = -2.0 * qh * np.cos(theta / 3.0) - a1 / 3.0
+        x2 = -2.0 * qh * np.cos((theta + 2.0 * math.pi) / 3.0) - a1 / 3.0
+        x3 = -2.0 * qh * np.cos((theta + 4.0 * math.pi) / 3.0) - a1 / 3.0
+        factor = np.sqrt(r * r - q3) + np.abs(r)
+        factor3 = factor ** (1.0 / 3.0)
+        xc = -1.0 * np.copysign(1.0, r) * (factor3 + q / factor3) - a1 / 3.0
+    p1, p2, p3 = x1 > 0.0, x2 > 0.0, x3 > 0.0
+    two = (p1 & p2) | (p1 & p3) | (p2 & p3)
+    none = ~(p1 | p2 | p3)
+    xt = np.where(p3, x3, np.where(p2, x2, x1))
+    flag = np.where(three & two, 1, np.where(three & none, 2, 0)).astype(np.int32)
+    xmu0 = np.where(three, xt, xc)
+    return np.where(flag == 0, xmu0, np.nan), flag
+
+
 def ulrich_mu0(mu: np.ndarray, rx: np.ndarray) -> np.ndarray:
-    """Streamline origin: root in [0,1] of mu0^3 + mu0 (rx-1) - mu rx = 0
-    (the cubic zerod.f90 solves), by safeguarded Newton iterations."""
-    mu = np.abs(mu)
-    m0 = np.where(rx < 1.0, np.maximum(mu, 0.5), mu)
-    lo = np.zeros_like(mu)
-    hi = np.ones_like(mu)
-    for _ in range(60):
-        f = m0 ** 3 + m0 * (rx - 1.0) - mu * rx
-        lo = np.where(f < 0, m0, lo)
-        hi = np.where(f >= 0, m0, hi)
-        df = 3 * m0 ** 2 + rx - 1.0
-        step = np.where(np.abs(df) > 1e-30, f / np.where(df == 0, 1.0, df), 0.0)
-        nm = m0 - step
-        bad = (nm <= lo) | (nm >= hi) | ~np.isfinite(nm)
-        m0 = np.where(bad, 0.5 * (lo + hi), nm)
-    return np.clip(m0, 1e-12, 1.0)
+    """Streamline origin mu0 of the Ulrich infall at |cos theta| = mu and
+    r/Rc = rx, as zerod.f90 solves it."""
+    return zerod(np.abs(mu), rx)[0]
+
+
+def densenv(rad, costin, rc, rhoe0, rmine, rmax, rhoamb=0.0):
+    """densenv.f90:30-45, 69-73, 111-113 elementwise (no holes, gaps or
+    bubbles): Ulrich (1976) infall density at radius rad (R*) and polar
+    cosine costin, centrifugal radius rc (R*), density scale rhoe0, zero
+    inside rmine and beyond rmax, floored at rhoamb.  Returns (dens, flag)
+    with zerod's flag per point."""
+    rad = np.asarray(rad, dtype=np.float64)
+    cosa = np.abs(costin)
+    zp = rad * cosa
+    xmu = zp / rad
+    rx = rad / rc
+    xmu0, flag = zerod(xmu, rx)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        factor = 1.0 / ((xmu / xmu0) + 2.0 * (xmu0 * xmu0) / rx)
+        dens = rhoe0 * rx ** -1.5 * factor / np.sqrt(1.0 + xmu / xmu0)
+    out = rad > rmine
+    dens = np.where(out, dens, 0.0)
+    dens = np.where(out & (dens < rhoamb), rhoamb, dens)
+    dens = np.where((rad >= rmax) | (rad <= rmine), 0.0, dens)
+    return dens, np.where(out, flag, 0)
 
 
 def fractal_field(shape, sigma_ln: float, dimension: float, seed: int) -> np.ndarray:
@@ -227,6 +269,20 @@ def fractal_field(shape, sigma_ln: float, dimension: float, seed: int) -> np.nda
     return np.exp(sigma_ln * g - 0.5 * sigma_ln ** 2)
 
 
+def ulrich_rhoe0(mdot_msun_yr: float, mstar_msun: float, rc_au: float) -> float:
+    """Envelope density scale Mdot / (4 pi sqrt(G M Rc^3)), g/cm^3."""
+    return (mdot_msun_yr * MSUN / YEAR) / (4.0 * math.pi * math.sqrt(G * mstar_msun * MSUN * (rc_au * AU) ** 3))
+
+
+def density_params(rstar_rsun, mdisk_msun, rin_au, rout_au, h0_au=0.03, flare=1.25, alpha=2.25,
+                   mdot_msun_yr=1e-6, rc_au=100.0, mstar_msun=0.5, rhoamb=0.0) -> np.ndarray:
+    """The 12 per-model parameters of the GPU cell builder
+    (mcrt_build_models), from the arguments yso_density takes."""
+    autors = AU / (rstar_rsun * RSUN)
+    return np.array([rstar_rsun * RSUN, autors, rin_au, rout_au, h0_au, flare, alpha, rc_au * autors,
+                     ulrich_rhoe0(mdot_msun_yr, mstar_msun, rc_au), rin_au * autors, mdisk_msun * MSUN, rhoamb])
+
+
 def yso_density(rarr, thetarr, phiarr, vcell, rstar_rsun, mdisk_msun, rin_au, rout_au,
                 h0_au=0.03, flare=1.25, alpha=2.25, mdot_msun_yr=1e-6, rc_au=100.0,
                 mstar_msun=0.5, clump_sigma=0.0, clump_dim=2.6, clump_seed=0):
@@ -246,14 +302,12 @@ def yso_density(rarr, thetarr, phiarr, vcell, rstar_rsun, mdisk_msun, rin_au, ro
     mass = float(np.sum(rho_d * vcell))
     if mass > 0:
         rho_d *= mdisk_msun * MSUN / mass
-    # Ulrich (1976) infall: rho = rho0 (r/Rc)^-3/2 (1+mu/mu0)^-1/2 (mu/mu0 + 2 mu0^2 Rc/r)^-1
-    rx = r_au / rc_au
-    mu = np.abs(np.cos(tc)) * np.ones_like(rx)
-    rxb = rx * np.ones_like(mu)
-    mu0 = ulrich_mu0(mu, rxb)
-    rho0 = (mdot_msun_yr * MSUN / YEAR) / (4.0 * math.pi * math.sqrt(G * mstar_msun * MSUN * (rc_au * AU) ** 3))
-    rho_e = rho0 * rxb ** -1.5 / np.sqrt(1.0 + mu / mu0) / (mu / mu0 + 2.0 * mu0 ** 2 / rxb)
-    rho_e = np.where(r_au >= rin_au, rho_e, 0.0)
+    # Ulrich (1976) infall as densenv evaluates it at gridset's cell centres
+    # (gridset_5_model13new.f90:1107,1123-1125): radii in R*
+    rho_e, flag = densenv(rc, np.cos(tc), rc_au * autors, ulrich_rhoe0(mdot_msun_yr, mstar_msun, rc_au),
+                          rin_au * autors, float(rarr[-1]))
+    if np.any(flag):
+        raise ValueError("zerod found no single streamline root at a cell centre")
     rho_e = np.broadcast_to(rho_e, vcell.shape).copy()
     if clump_sigma > 0:
         f = fractal_field(vcell.shape, clump_sigma, clump_dim, clump_seed)
@@ -359,15 +413,29 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
           clump_sigma=0.0, clump_dim=2.6, clump_seed=1001, pl=0.0,
           npeel=10, nx=101, bands=((0.01, 5000.0),), nfreq=100, nmu=10,
           npackets=1_000_000, niter=5, nlam=100, nT=500, nnu=513,
-          thete_deg=None, phie_deg=0.0, flare=1.25, _tables=None) -> Model:
+          thete_deg=None, phie_deg=0.0, flare=1.25, _tables=None, device=False, _cells=None) -> Model:
+    """One model; device=True builds its cell arrays (volumes, densities,
+    species) on the GPU (engine.build_models) instead of in numpy."""
     rarr = radial_walls(nr, rin_au, rout_au, rstar)
     thetarr = theta_walls(nt) if nt > 1 else np.array([0.0, math.pi])
     costarr, tan2arr = theta_tables(thetarr)
     phiarr, aarr, barr = phi_walls(nphi)
-    vcell = cell_volumes(rarr, thetarr, phiarr, rstar)
-    rho, species = yso_density(rarr, thetarr, phiarr, vcell, rstar, mdisk, rin_au, rout_au,
-                               flare=flare, mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
-                               clump_dim=clump_dim, clump_seed=clump_seed)
+    if _cells is not None:
+        rho, species, vcell = _cells
+    elif device:
+        from .engine import build_models
+        clump = None
+        if clump_sigma > 0:
+            clump = fractal_field((len(phiarr) - 1, len(thetarr) - 1, len(rarr) - 1), clump_sigma, clump_dim,
+                                  clump_seed)
+        par = density_params(rstar, mdisk, rin_au, rout_au, flare=flare, mdot_msun_yr=mdot, rc_au=rc_au)
+        cells, _ = build_models([(rarr, thetarr, phiarr)], par[None, :], None if clump is None else [clump])
+        rho, species, vcell = cells[0]
+    else:
+        vcell = cell_volumes(rarr, thetarr, phiarr, rstar)
+        rho, species = yso_density(rarr, thetarr, phiarr, vcell, rstar, mdisk, rin_au, rout_au,
+                                   flare=flare, mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
+                                   clump_dim=clump_dim, clump_seed=clump_seed)
     if _tables is None:          # (dust, emission tables) reused across models of a batch
         dust = synthetic_dust(nlam=nlam, pl=pl)
         em = emission_tables(dust, nT=nT, nnu=nnu)
@@ -454,27 +522,41 @@ class Batch:
 
 
 def batch_models(n: int = 512, seed: int = 1002, nr_range=(50, 400), nt_range=(40, 200), npeel_sed: int = 10,
-                 nfreq: int = 100, npackets: int = 1_000_000, nlam: int = 100, nT: int = 500, nnu: int = 513) -> Batch:
+                 nfreq: int = 100, npackets: int = 1_000_000, nlam: int = 100, nT: int = 500, nnu: int = 513,
+                 device: bool = False) -> Batch:
     """Seeded grid of axisymmetric disk + envelope models with uneven grids:
     nr uniform in nr_range, n_theta uniform even in nt_range, Mdisk
     log-uniform 1e-4..1e-1 Msun, flaring exponent 1.0..1.3, Rin 0.1..1 AU,
     envelope infall rate log-uniform 1e-7..1e-5 Msun/yr.  Same star and dust
     as the 100x90 disk; SEDs binned at npeel_sed inclinations (cos theta
-    bins of the escaping packets), no peel-off images."""
+    bins of the escaping packets), no peel-off images.  device=True builds
+    every model's cell arrays in one GPU launch."""
     rng = np.random.default_rng(seed)
-    tables = None
-    models, params = [], []
+    params = []
     for k in range(n):
         nr = int(rng.integers(nr_range[0], nr_range[1] + 1))
         nt = int(rng.integers(nt_range[0] // 2, nt_range[1] // 2 + 1)) * 2
-        p = dict(nr=nr, nt=nt, mdisk=float(10 ** rng.uniform(-4.0, -1.0)), flare=float(rng.uniform(1.0, 1.3)),
-                 rin_au=float(rng.uniform(0.1, 1.0)), mdot=float(10 ** rng.uniform(-7.0, -5.0)))
-        m = build(f"batch{k}", nr=nr, nt=nt, nphi=1, rin_au=p["rin_au"], mdisk=p["mdisk"], mdot=p["mdot"],
-                  flare=p["flare"], npeel=0, nx=1, bands=(), nfreq=nfreq, nmu=npeel_sed, npackets=npackets, niter=1,
-                  nlam=nlam, nT=nT, nnu=nnu, _tables=tables)
+        params.append(dict(nr=nr, nt=nt, mdisk=float(10 ** rng.uniform(-4.0, -1.0)),
+                           flare=float(rng.uniform(1.0, 1.3)), rin_au=float(rng.uniform(0.1, 1.0)),
+                           mdot=float(10 ** rng.uniform(-7.0, -5.0))))
+    rstar, rout_au = 2.0, 1000.0                 # build()'s defaults, shared by every model of the grid
+    cells = [None] * n
+    if device:                                   # every model's cell arrays in one GPU launch
+        from .engine import build_models
+        walls = [(radial_walls(p["nr"], p["rin_au"], rout_au, rstar), theta_walls(p["nt"]), phi_walls(1)[0])
+                 for p in params]
+        par = np.array([density_params(rstar, p["mdisk"], p["rin_au"], rout_au, flare=p["flare"],
+                                       mdot_msun_yr=p["mdot"]) for p in params])
+        cells, _ = build_models(walls, par)
+    tables = None
+    models = []
+    for k, p in enumerate(params):
+        m = build(f"batch{k}", nr=p["nr"], nt=p["nt"], nphi=1, rin_au=p["rin_au"], rout_au=rout_au, rstar=rstar,
+                  mdisk=p["mdisk"], mdot=p["mdot"], flare=p["flare"], npeel=0, nx=1, bands=(), nfreq=nfreq,
+                  nmu=npeel_sed, npackets=npackets, niter=1, nlam=nlam, nT=nT, nnu=nnu, _tables=tables,
+                  _cells=cells[k])
         tables = m.tables
         models.append(m)
-        params.append(p)
     return Batch(shared=models[0], models=models, params=params)
 
 

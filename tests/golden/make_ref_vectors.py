@@ -154,6 +154,17 @@ def main():
         R.ref_qsimp_seg(int(seg[k, 0]), *[float(v) for v in seg[k, 1:]], C.byref(r), C.byref(cv))
         res[k], flag[k] = r.value, cv.value
     out["qs_in"], out["qs_out"], out["qs_flag"] = seg, res, flag
+    # --- zerod.f90: the Ulrich streamline cubic, both branches, the rx = 1
+    # boundary and the equator (mu = 0); xmu0 starts at -7 so that a case
+    # that leaves it unset shows
+    zmu = np.concatenate([rng.uniform(0.0, 1.0, 400), [0.0, 0.0, 1.0, 1e-12, 0.5, 0.999999]])
+    zz = np.concatenate([10.0 ** rng.uniform(-4.0, 4.0, 400), [0.5, 2.0, 1.0, 1.0, 1.0 - 1e-9, 1e-6]])
+    zout, zfl = np.zeros(len(zz)), np.zeros(len(zz), np.int32)
+    for k in range(len(zz)):
+        r, cv = C.c_double(-7.0), C.c_int()
+        R.ref_zerod(float(zmu[k]), float(zz[k]), C.byref(r), C.byref(cv))
+        zout[k], zfl[k] = r.value, cv.value
+    out["zd_mu"], out["zd_z"], out["zd_out"], out["zd_flag"] = zmu, zz, zout, zfl
     path = os.path.join(HERE, "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")

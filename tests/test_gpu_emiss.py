@@ -51,8 +51,13 @@ def test_setemiss_matches_oracle(oracle_lib, nT, nnu, nopac):
     g = setemiss(nud, kap, alb, em["Tint"], em["nuint"], smooth=smooth, nlambda=nl)
     kapint, kappap, kappar, nfail = oracle_lib.setemiss(nud, kap, alb, em["Tint"], em["nuint"], smooth=smooth,
                                                         nlambda=nl)
-    print(f"kernel {g['ms']:.3f} ms, qsimp non-converged: gpu {g['nfail']} oracle {nfail}", flush=True)
+    nseg = nopac * nnu * nT
+    print(f"kernel {g['ms']:.3f} ms, {g['nevals'] / nseg:.2f} evaluations per interval, "
+          f"qsimp non-converged: gpu {g['nfail']} oracle {nfail}", flush=True)
     assert g["nfail"] == nfail
+    # qsimp stops at the third refinement at the earliest (5 evaluations) and
+    # at the twentieth at the latest, for each of the two integrals
+    assert 10 * nseg <= g["nevals"] <= 2 * (2 ** 19 + 1) * nseg
     _compare(g["kapint"], kapint, "kapint")
     _compare(g["kappap"], kappap, "kappap")
     _compare(g["kappar"], kappar, "kappar")
