@@ -55,6 +55,18 @@ module mcrt_mod
       real(c_double), intent(in) :: nudust(*), kappad(*), adust(*), gdust(*), pldust(*), kappav(*)
       real(c_double), intent(in) :: lTint(*), Tint(*), nuint(*), kapint(*), kappap(*)
     end function
+    ! setemiss (dust_5.f90:507-699) on the GPU; arrays in dust_mod's layout
+    integer(c_int) function mcrt_setemiss(ctx, nopac, nlam, nlambda, nudust, kappad, adust, nT, Tint, nnu, &
+         nuint, smooth, kapint, kappap, kappar, nfail, nevals, ms) bind(C, name='mcrt_setemiss')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nopac, nlam, nT, nnu
+      integer(c_int32_t), intent(in) :: nlambda(*), smooth(*)
+      real(c_double), intent(in) :: nudust(*), kappad(*), adust(*), Tint(*), nuint(*)
+      real(c_double), intent(out) :: kapint(*), kappap(*), kappar(*)
+      integer(c_int64_t), intent(out) :: nfail, nevals
+      real(c_float), intent(out) :: ms
+    end function
     integer(c_int) function mcrt_set_source(ctx, rstar, tstar, rmin) bind(C, name='mcrt_set_source')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
