@@ -109,6 +109,17 @@ int mcrt_build_models(mcrt_ctx *ctx, int32_t nmodels, const int32_t *nrg, const 
                       const double *phiarr, const double *params, const double *clump,
                       double *rho, uint8_t *species, double *vcell, int64_t *nflag, float *ms);
 
+/* External illumination (isrf_frac > 0 in the reference): the last npout
+ * packets of every mcrt_run_packets call, npout = isrf_frac * npackets
+ * rounded as newdisktherm_5_fuv.f90:158-163 rounds it, are photons of the
+ * interstellar field (initpout.f90:18-57, newdisktherm_5_fuv.f90:416-455)
+ * entering at 0.999999 rmax, with frequencies from rep_isrf.f90 on the
+ * tables of isrf_mod (cdf_i, logcdf_i, logfreq_i of nisrf entries, as
+ * isrf_cdf.f90 leaves them; cdf from 0 to 1).  isrf_frac = 0 turns it off.
+ * Batched runs and the adjoint do not emit external photons. */
+int mcrt_set_isrf(mcrt_ctx *ctx, int32_t nisrf, const double *cdf, const double *logcdf,
+                  const double *logfreq, double isrf_frac);
+
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);
 

@@ -1131,13 +1131,73 @@ static int propagate(orc_run *R, orc_pkt *p, int *n_inter)
     return 1;
 }
 
-/* newdisktherm_5_fuv.f90:294-413: one stellar photon (ips=0, no spots) */
-static int one_packet(orc_run *R, double *trace)
+/* rep_isrf.f90:13-29: ISRF frequency (eV) from the tabulated CDF */
+static double rep_isrf(orc_run *R)
+{
+    const orc_model *m = R->m;
+    const double xran = ran(&R->rng);
+    int32_t ifreq = orc_locate(m->isrf_cdf, m->nisrf, xran);
+    if (ifreq == m->nisrf) ifreq = ifreq - 1;
+    if (ifreq == 0) ifreq = 1;
+    const double *lf = m->isrf_logfreq - 1, *lc = m->isrf_logcdf - 1;     /* 1-based */
+    const double logfreq = lf[ifreq] + (log10(xran) - lc[ifreq]) / (lc[ifreq + 1] - lc[ifreq]) * (lf[ifreq + 1] - lf[ifreq]);
+    return pow(10.0, logfreq);
+}
+
+/* newdisktherm_5_fuv.f90:416-455 with initpout.f90:18-57: a photon of the
+ * external radiation field entering at 0.999999 rmax, directed inward in
+ * the local frame, not peeled (the reference subtracts the sky) */
+static void emit_outside(orc_run *R, orc_pkt *p)
+{
+    const orc_model *m = R->m;
+    p->sip = 1.0; p->sqp = 0.0; p->sup = 0.0; p->svp = 0.0;
+    double xran = ran(&R->rng);
+    p->cosb = 1.0 - 2.0 * xran;
+    p->sinb = sqrt(1.0 - p->cosb * p->cosb);
+    xran = ran(&R->rng);
+    p->cost = -xran;
+    p->sint = sqrt(1.0 - p->cost * p->cost);
+    xran = ran(&R->rng);
+    p->phi = R2P * xran;
+    p->cosp = cos(p->phi);
+    p->sinp = sin(p->phi);
+    xran = ran(&R->rng);
+    p->lp = R2P * xran;
+    initp_transform(p);
+    p->nub = rep_isrf(R);
+    p->on_wall = 1;
+    p->i_orig = 5;
+    opacset(m, p->nub, p->op);
+    p->wave = 1.2398 / p->nub;
+    set_kapd(p, m, 0);
+    const double coslp = cos(p->lp), sinlp = sin(p->lp);
+    const double rmax = m->rarr[m->nrg - 1];
+    const double shrink = (double)0.999999f;        /* a default-real literal in the reference */
+    p->zp = p->cosb * rmax * shrink;
+    const double rp = p->sinb * rmax * shrink;
+    p->xp = rp * coslp;
+    p->yp = rp * sinlp;
+    p->rsq = p->zp * p->zp + rp * rp;
+    p->rtot = sqrt(p->rsq);
+    p->ux = p->sint * p->cosp; p->uy = p->sint * p->sinp; p->uz = p->cost;
+    p->iflag = 0;
+    p->ir = m->nrg - 1;
+    p->it = orc_locate(m->thetarr, m->ntg, acos(p->cosb));
+    p->ip = orc_locate(m->phiarr, m->npg, p->lp);
+}
+
+/* newdisktherm_5_fuv.f90:294-455: one photon, stellar (ips=0, no spots)
+ * or, for the last packets of a run, from the external field */
+static int one_packet(orc_run *R, double *trace, int64_t i_run)
 {
     const orc_model *m = R->m;
     orc_pkt p;
     memset(&p, 0, sizeof(p));
     p.on_wall = 0;
+    if (m->isrf_from >= 0 && i_run >= m->isrf_from) {
+        emit_outside(R, &p);
+        goto transport;
+    }
     /* initpspot.f90:27-160 */
     p.sip = 1.0; p.sqp = 0.0; p.sup = 0.0; p.svp = 0.0;
     for (;;) {
@@ -1174,6 +1234,7 @@ static int one_packet(orc_run *R, double *trace)
             if (cosnorm > 0.0) peeloff(R, &p, sipnew, pid, 0);
         }
     }
+transport:;
     int nint = 0;
     int code = propagate(R, &p, &nint);
     if (code == 1) R->t->counters[ORC_C_ESCAPED] += 1.0;
@@ -1203,7 +1264,7 @@ int32_t orc_run_packets(const orc_model *m, const double *tdust, orc_tallies *t,
         for (int64_t i = 0; i < npackets; ++i) {
             R.rng.packet = first_packet + (uint64_t)i;
             R.rng.n = 0;
-            one_packet(&R, trace ? trace + 8 * i : NULL);
+            one_packet(&R, trace ? trace + 8 * i : NULL, i);
         }
         return 0;
     }
@@ -1229,7 +1290,7 @@ int32_t orc_run_packets(const orc_model *m, const double *tdust, orc_tallies *t,
         for (int64_t i = 0; i < npackets; ++i) {
             R.rng.packet = first_packet + (uint64_t)i;
             R.rng.n = 0;
-            one_packet(&R, trace ? trace + 8 * i : NULL);
+            one_packet(&R, trace ? trace + 8 * i : NULL, i);
         }
     }
     for (int k = 0; k < nthreads; ++k) {

@@ -64,6 +64,12 @@ typedef struct {
     double numin, lnurat;                       /* eV, ln(numax/numin) */
     int32_t iintmax;                            /* 10000, propagate_5_fuv.f90:32 */
     double taumax;                              /* 25, peeloff_3d_2_test.f90:52 */
+    /* external illumination (initpout.f90, rep_isrf.f90, isrf_mod.f90):
+     * packet i of a run (0-based) enters from outside when i >= isrf_from;
+     * isrf_from < 0 turns it off */
+    int32_t nisrf;
+    const double *isrf_cdf, *isrf_logcdf, *isrf_logfreq;   /* nisrf */
+    int64_t isrf_from;
 } orc_model;
 
 /* Tallies: every pointer is caller-owned, zeroed by the caller, accumulated here.

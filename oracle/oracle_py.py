@@ -39,6 +39,8 @@ class OrcModel(C.Structure):
         ("aperture2", _dp), ("u_mu", _dp), ("sini_mu", _dp),
         ("numin", C.c_double), ("lnurat", C.c_double),
         ("iintmax", C.c_int32), ("taumax", C.c_double),
+        ("nisrf", C.c_int32), ("isrf_cdf", _dp), ("isrf_logcdf", _dp), ("isrf_logfreq", _dp),
+        ("isrf_from", C.c_int64),
     ]
 
 
@@ -195,8 +197,26 @@ class OracleModel:
         s.nfreq, s.nmu, s.nph, s.nap = m.nfreq, m.nmu, m.nph, m.nap
         s.numin, s.lnurat = m.numin, m.lnurat
         s.iintmax, s.taumax = 10000, 25.0
+        s.isrf_from = -1
+        if getattr(m, "isrf", None) is not None and m.isrf_frac > 0.0:
+            for k in ("cdf", "logcdf", "logfreq"):
+                self.arrs["isrf_" + k] = f(m.isrf[k])
+                setattr(s, "isrf_" + k, _p(self.arrs["isrf_" + k], C.c_double))
+            s.nisrf = len(self.arrs["isrf_cdf"])
         self.s = s
         self.m = m
+
+    def set_run(self, npackets):
+        """The external-field packets of a run of npackets: the last ones
+        (newdisktherm_5_fuv.f90:158-187)."""
+        m = self.m
+        if getattr(m, "isrf", None) is not None and m.isrf_frac > 0.0:
+            npout = int(m.isrf_frac * float(npackets))
+            if m.isrf_frac * float(npackets) - npout > 0.5:
+                npout += 1
+            self.s.isrf_from = int(npackets) - npout
+        else:
+            self.s.isrf_from = -1
 
 
 def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_mode=0, trace=False):
@@ -217,6 +237,7 @@ def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_m
     t.sed, t.psed, t.image = _p(sed, C.c_float), _p(psed, C.c_float), _p(img, C.c_float)
     td = np.ascontiguousarray(m.tdust_initial() if tdust is None else tdust, dtype=np.float64)
     tr = np.zeros((npackets, 8)) if trace else None
+    om.set_run(npackets)
     rc = L.orc_run_packets(C.byref(om.s), _p(td, C.c_double), C.byref(t), int(first_packet), int(npackets),
                            C.c_uint64(int(seed)), int(rng_mode), int(nthreads),
                            _p(tr, C.c_double) if trace else None)

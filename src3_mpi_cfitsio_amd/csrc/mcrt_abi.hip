@@ -27,6 +27,10 @@ struct mcrt_ctx {
   DeviceModel dm{};
   DeviceTallies dt{};
   bool has_grid = false, has_medium = false, has_dust = false, has_source = false, has_observer = false, has_tdust = false;
+  // external illumination: fraction of a run's packets, device tables
+  double isrf_frac = 0.0;
+  int nisrf = 0;
+  const double *isrf_cdf = nullptr, *isrf_logcdf = nullptr, *isrf_logfreq = nullptr;
   size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
   int64_t trace_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -227,6 +231,8 @@ int mcrt_create(int device, mcrt_ctx** out) {
     return MCRT_E_HIP;
   }
   c->dm.max_segments = 1000000;
+  c->dm.nisrf = 0;                          // external illumination off (mcrt_set_isrf)
+  c->dm.isrf_from = ~0ull;
   c->dm.iintmax = 10000;
   c->dm.taumax = 25.0;
   std::vector<double> z = zeta_partial_sums();
@@ -519,6 +525,31 @@ int mcrt_build_models(mcrt_ctx* c, int32_t nmodels, const int32_t* nrg, const in
   return MCRT_OK;
 }
 
+int mcrt_set_isrf(mcrt_ctx* c, int32_t nisrf, const double* cdf, const double* logcdf, const double* logfreq,
+                  double isrf_frac) {
+  if (!c) return MCRT_E_ARG;
+  if (!(isrf_frac >= 0.0 && isrf_frac < 1.0)) return fail(c, MCRT_E_ARG, "mcrt_set_isrf: isrf_frac must be in [0, 1)");
+  if (isrf_frac == 0.0) {
+    c->isrf_frac = 0.0;
+    c->nisrf = 0;
+    return MCRT_OK;
+  }
+  if (nisrf < 2 || !cdf || !logcdf || !logfreq) return fail(c, MCRT_E_ARG, "mcrt_set_isrf: nisrf >= 2 tables needed");
+  for (int i = 1; i < nisrf; ++i)
+    if (!(cdf[i] >= cdf[i - 1])) return fail(c, MCRT_E_ARG, "mcrt_set_isrf: the CDF must be non-decreasing");
+  if (cdf[0] != 0.0 || cdf[nisrf - 1] != 1.0) return fail(c, MCRT_E_ARG, "mcrt_set_isrf: the CDF must run from 0 to 1");
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = 0;
+  rc |= upload(c, "isrf.cdf", cdf, (size_t)nisrf, &c->isrf_cdf);
+  rc |= upload(c, "isrf.logcdf", logcdf, (size_t)nisrf, &c->isrf_logcdf);
+  rc |= upload(c, "isrf.logfreq", logfreq, (size_t)nisrf, &c->isrf_logfreq);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->nisrf = nisrf;
+  c->isrf_frac = isrf_frac;
+  return MCRT_OK;
+}
+
 int mcrt_set_source(mcrt_ctx* c, double rstar, double tstar, double rmin) {
   if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
@@ -606,6 +637,18 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   DeviceTallies t = c->dt;
   t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
   const bool fp64 = (precision == MCRT_FP64);
+  // the run's external-field packets are its last npout
+  // (newdisktherm_5_fuv.f90:158-187 with no accretion photons)
+  DeviceModel dm = c->dm;
+  if (c->nisrf > 0 && c->isrf_frac > 0.0) {
+    int64_t npout = (int64_t)(c->isrf_frac * (double)npackets);
+    if (c->isrf_frac * (double)npackets - (double)npout > 0.5) npout += 1;
+    dm.nisrf = c->nisrf;
+    dm.isrf_cdf = c->isrf_cdf;
+    dm.isrf_logcdf = c->isrf_logcdf;
+    dm.isrf_logfreq = c->isrf_logfreq;
+    dm.isrf_from = (unsigned long long)(npackets - npout);
+  }
   // Resident lanes: as many 256-thread blocks per CU as the transport
   // kernel's registers and LDS allow (never more than the packets need).
   // Each round every lane runs events while it can reserve slots in the
@@ -677,11 +720,11 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     // q_count and live restart every round (a fill kernel, not a memset)
     HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
     HIPCHECK(c, hipEventRecord(c->evr[0], c->stream));
-    hipError_t e = fp64 ? launch_transport_fp64(c->dm, t, rp, c->stream) : launch_transport_mixed(c->dm, t, rp, c->stream);
+    hipError_t e = fp64 ? launch_transport_fp64(dm, t, rp, c->stream) : launch_transport_mixed(dm, t, rp, c->stream);
     if (e != hipSuccess) return hipfail(c, e, "transport kernel launch");
     HIPCHECK(c, hipEventRecord(c->evr[1], c->stream));
     if (queued) {
-      e = fp64 ? launch_peel_fp64(c->dm, t, rp, pgrid, c->stream) : launch_peel_mixed(c->dm, t, rp, pgrid, c->stream);
+      e = fp64 ? launch_peel_fp64(dm, t, rp, pgrid, c->stream) : launch_peel_mixed(dm, t, rp, pgrid, c->stream);
       if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
     }
     HIPCHECK(c, hipEventRecord(c->evr[2], c->stream));

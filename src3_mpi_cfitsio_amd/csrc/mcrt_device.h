@@ -72,6 +72,11 @@ struct DeviceModel {
   int iintmax;
   double taumax;
   long long max_segments;        // per walk; guards against zero-length cycling
+  // external illumination (initpout.f90, rep_isrf.f90): packet k of a run
+  // (0-based) enters from outside when nisrf > 0 and k >= isrf_from
+  int nisrf;
+  const double *isrf_cdf, *isrf_logcdf, *isrf_logfreq;
+  unsigned long long isrf_from;
 };
 
 struct DeviceTallies {

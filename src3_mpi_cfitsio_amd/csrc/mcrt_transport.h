@@ -1098,42 +1098,86 @@ __device__ __forceinline__ void prop_end(Ctx<T>& C, Life<T>& L) {
   if (p.iflag == 1) C.cnt.add(C_SCOUNT);
 }
 
+// rep_isrf.f90:13-29: ISRF frequency (eV) from the tabulated CDF.  FP64
+// follows the reference's log10/pow; the mixed path interpolates the same
+// table in FP32.
+template <class T>
+__device__ __forceinline__ T rep_isrf(Ctx<T>& C) {
+  const DeviceModel& m = C.m;
+  const T xran = uniform<T>(C.rng);
+  int ifreq = locate<double>(m.isrf_cdf, m.nisrf, (double)xran);
+  if (ifreq == m.nisrf) ifreq = ifreq - 1;
+  if (ifreq == 0) ifreq = 1;
+  const T lf0 = (T)m.isrf_logfreq[ifreq - 1], lf1 = (T)m.isrf_logfreq[ifreq];
+  const T lc0 = (T)m.isrf_logcdf[ifreq - 1], lc1 = (T)m.isrf_logcdf[ifreq];
+  if constexpr (std::is_same<T, double>::value) {
+    const double logfreq = lf0 + (log10(xran) - lc0) / (lc1 - lc0) * (lf1 - lf0);
+    return pow(10.0, logfreq);
+  } else {
+    const float ln10 = 2.302585093f;
+    const float logfreq = lf0 + (RT<T>::log_acc(xran) / ln10 - lc0) / (lc1 - lc0) * (lf1 - lf0);
+    return RT<T>::exp_(ln10 * logfreq);
+  }
+}
+
 // newdisktherm_5_fuv.f90:294-413: one stellar photon (ips = 0, no spots),
-// its direct peel, and the start of its first flight.
+// its direct peel, and the start of its first flight.  With external
+// illumination the last packets of a run are photons of the external
+// field instead (newdisktherm_5_fuv.f90:416-455, initpout.f90:18-57):
+// entering at 0.999999 rmax, directed inward in the local frame, ISRF
+// frequency, not peeled (the reference subtracts the sky).  Both share
+// the transform, opacity, position and cell lookup.
 template <class T, bool kAdj = false>
 __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
+  const bool outside = m.nisrf > 0 && L.pid >= m.isrf_from;
   p.on_wall = false;
   p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
-  for (;;) {
+  if (!outside) {
+    for (;;) {
+      p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
+      p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
+      if (p.cosb != (T)0 && RT<T>::fabs_(p.cosb) < (T)1) break;
+    }
+    p.lp = (T)kR2p * uniform<T>(C.rng);
+    initp_sample(C, p);
+  } else {
     p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
     p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
-    if (p.cosb != (T)0 && RT<T>::fabs_(p.cosb) < (T)1) break;
+    p.cost = -uniform<T>(C.rng);
+    p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
+    p.phi = (T)kR2p * uniform<T>(C.rng);
+    p.cosp = RT<T>::cos_(p.phi);
+    p.sinp = RT<T>::sin_(p.phi);
+    p.lp = (T)kR2p * uniform<T>(C.rng);
   }
-  p.lp = (T)kR2p * uniform<T>(C.rng);
-  initp_sample(C, p);
   initp_transform(p);
-  p.nub = bbfreq<T>(C.rng, (T)m.tstar, C.inczeta);
-  p.i_orig = 1;
+  p.nub = outside ? rep_isrf<T>(C) : bbfreq<T>(C.rng, (T)m.tstar, C.inczeta);
+  p.i_orig = outside ? 5 : 1;
   p.iflag = 0;
   p.exitflag = 0;
   opacset<T>(m, p.nub, p.op);
   p.wave = (T)1.2398 / p.nub;
   set_kapd(p, m, 0);
   T coslp = RT<T>::cos_(p.lp), sinlp = RT<T>::sin_(p.lp);
-  p.zp = (G)p.cosb * m.rmin;
-  G rp = (G)p.sinb * m.rmin;
+  const G rw = outside ? m.rarr[m.nrg - 1] : m.rmin;
+  p.zp = (G)p.cosb * rw;
+  G rp = (G)p.sinb * rw;
+  if (outside) {                                  // cosb*rmax*0.999999, a default-real literal
+    p.zp = p.zp * (G)0.999999f;
+    rp = rp * (G)0.999999f;
+  }
   p.xp = rp * (G)coslp;
   p.yp = rp * (G)sinlp;
   p.rsq = p.zp * p.zp + rp * rp;
   p.rtot = sqrt(p.rsq);
   set_dir<T>(p);
-  p.ir = 1;
+  p.ir = outside ? m.nrg - 1 : 1;
   p.on_wall = true;
   p.it = locate<double>(m.thetarr, m.ntg, (double)RT<T>::acos_(p.cosb));
   p.ip = locate<double>(m.phiarr, m.npg, (double)p.lp);
-  if (m.ipeel == 1) peel_direct<T, kAdj>(C, p, coslp, sinlp);
+  if (m.ipeel == 1 && !outside) peel_direct<T, kAdj>(C, p, coslp, sinlp);
   // propagate_5_fuv.f90:20-40: the first flight
   cell_cache_reset(L);
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;

@@ -105,7 +105,19 @@ class Engine:
             m.nband, m.nwmax, ptr(obs["nw"], C.c_int32), ptr(obs["bf"], d), ptr(obs["bp"], d),
             m.nfreq, m.nmu, m.nph, m.nap, ptr(obs["ap"], d), ptr(obs["mu"], d), m.numin, m.lnurat))
         self.set_tdust(m.tdust_initial())
+        self.set_isrf(m.isrf_frac, m.isrf)
         self._check(L.mcrt_zero_tallies(self.h))
+
+    def set_isrf(self, isrf_frac: float, tables=None):
+        """External illumination: the last isrf_frac of every run's packets
+        enter from outside with ISRF frequencies (tables as model.isrf_tables
+        gives them); 0 turns it off."""
+        if not isrf_frac:
+            self._check(self.lib.mcrt_set_isrf(self.h, 0, None, None, None, 0.0))
+            return
+        cdf, lc, lf = (_f64(tables[k]) for k in ("cdf", "logcdf", "logfreq"))
+        d = C.c_double
+        self._check(self.lib.mcrt_set_isrf(self.h, len(cdf), ptr(cdf, d), ptr(lc, d), ptr(lf, d), float(isrf_frac)))
 
     def set_density(self, rho):
         """Replace the cell densities (g/cm^3, cell_shape); species unchanged."""

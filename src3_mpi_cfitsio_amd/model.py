@@ -107,6 +107,10 @@ class Model:
     niter: int = 1
     tdust0: Optional[np.ndarray] = None
     tables: Optional[tuple] = dataclasses.field(default=None, repr=False, compare=False)
+    # external illumination (initpout.f90, rep_isrf.f90): the fraction of
+    # the packets that enter from outside, and the ISRF sampling tables
+    isrf_frac: float = 0.0
+    isrf: Optional[Dict[str, np.ndarray]] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def ncell(self) -> int:
@@ -389,6 +393,98 @@ def emission_tables(dust, nT=500, nnu=513, Tmin=0.1, Tmax=200000.0, nodes=8):
     kapint[:, -1] = 1.0
     return dict(nT=nT, nnu=nnu, lTint=lTint, Tint=Tint, nuint=nuint,
                 kapint=np.ascontiguousarray(kapint.T), kappap=kappap)
+
+
+# ---------------------------------------------------------------------------
+# external illumination
+# ---------------------------------------------------------------------------
+# Black (1987) interstellar radiation field as CLOUDY tabulates it
+# (isrf_cdf.f90:35-41): log10 nu [Hz] and log10 nu F_nu
+_ISRF_LNU = (6.00, 10.72, 11.00, 11.23, 11.47, 11.55, 11.85, 12.26, 12.54, 12.71, 13.10, 13.64, 14.14, 14.38,
+             14.63, 14.93, 15.08, 15.36, 15.43, 16.25, 17.09, 18.00, 23.00)
+_ISRF_NUFNU = (-16.708, -2.96, -2.47, -2.09, -2.11, -2.34, -3.66, -2.72, -2.45, -2.57, -3.85, -3.34, -2.30, -1.79,
+               -1.79, -2.34, -2.72, -2.55, -2.62, -5.68, -6.45, -6.30, -11.3)
+
+
+def _locate2(xx, n, x):
+    """nr_2.f:181-206 (1-based result, xx ascending or descending, first n used)."""
+    jl, ju = 0, n + 1
+    asc = xx[n - 1] > xx[0]
+    while ju - jl > 1:
+        jm = (ju + jl) // 2
+        if asc == (x > xx[jm - 1]):
+            jl = jm
+        else:
+            ju = jm
+    if jl == 0 and x == xx[0]:
+        jl = 1
+    return jl
+
+
+def isrf_tables(nisrf: int = 200) -> Dict[str, object]:
+    """The ISRF sampling tables of isrf_cdf.f90 without extinction (isrf_Av
+    = 0; the extinction law file the reference reads is not part of it):
+    the spectrum log-interpolated onto nisrf log-spaced wavelengths in
+    0.05..5000 um, a trapezoid CDF in frequency, then cdf (normalized),
+    logcdf = log10 cdf (logcdf(1) = -99) and logfreq = log10 of h nu in eV
+    (logfreq(1) = 0, as the reference leaves it).  isrf_int is the
+    unnormalized integral (erg/s/cm^2/sr)."""
+    c = 2.99792458e14
+    freqin = 10.0 ** np.array(_ISRF_LNU)
+    pdfin = 10.0 ** np.array(_ISRF_NUFNU) / freqin
+    logfreqin, logpdfin = np.log10(freqin), np.log10(pdfin)
+    nin = len(freqin)
+    lmin = c / (freqin[nin - 1] * 0.99999999)
+    lmax = c / (freqin[0] * 1.0000001)
+    if lmin < 0.05:
+        lmin = 0.05
+    if lmax > 5000.0:
+        lmax = 5000.0
+    freq = np.zeros(nisrf)
+    for i in range(1, nisrf + 1):
+        lam = lmin * (lmax / lmin) ** (float(i) / float(nisrf))
+        freq[nisrf - i] = c / lam
+    pdf = np.zeros(nisrf)
+    for i in range(nisrf):
+        j = _locate2(freqin, nin, freq[i])
+        logpdf = logpdfin[j - 1] + (math.log10(freq[i]) - logfreqin[j - 1]) / (logfreqin[j] - logfreqin[j - 1]) * \
+            (logpdfin[j] - logpdfin[j - 1])
+        pdf[i] = 10.0 ** logpdf * 1.0
+    cdf = np.zeros(nisrf)
+    for i in range(1, nisrf):
+        cdf[i] = cdf[i - 1] + 0.5 * (pdf[i] + pdf[i - 1]) * (freq[i] - freq[i - 1])
+    isrf_int = float(cdf[nisrf - 1])
+    logcdf = np.zeros(nisrf)
+    logfreq = np.zeros(nisrf)
+    logcdf[0] = -99.0
+    for i in range(1, nisrf):
+        cdf[i] = cdf[i] / cdf[nisrf - 1]
+        logcdf[i] = math.log10(cdf[i])
+        freq[i] = freq[i] * 1.2398 / c
+        logfreq[i] = math.log10(freq[i])
+    return dict(cdf=cdf, logcdf=logcdf, logfreq=logfreq, isrf_int=isrf_int, pdf=pdf)
+
+
+def isrf_luminosity(isrf_int: float, rmax_rstar: float, rstar_rsun: float, isrf_scl: float = 1.0) -> float:
+    """l_isrf = isrf_int 4 pi (rmax R*)^2 / 2 * isrf_scl (newtts.f90:155), erg/s."""
+    return isrf_int * 4.0 * math.pi * (rmax_rstar * RSUN * rstar_rsun) ** 2 / 2.0 * isrf_scl
+
+
+def isrf_split(npackets: int, isrf_frac: float) -> int:
+    """Packets of a run that are stellar; the rest (the last ones) enter from
+    outside.  newdisktherm_5_fuv.f90:158-187 with no accretion: npout =
+    nint-style rounding of isrf_frac * np, ns = np - npout."""
+    if isrf_frac <= 0.0:
+        return npackets
+    npout = int(isrf_frac * float(npackets))
+    if isrf_frac * float(npackets) - npout > 0.5:
+        npout += 1
+    return npackets - npout
+
+
+def with_isrf(m: Model, isrf_frac: float, nisrf: int = 200) -> Model:
+    """The model with external illumination carrying isrf_frac of the packets."""
+    return dataclasses.replace(m, isrf_frac=float(isrf_frac), isrf=isrf_tables(nisrf))
 
 
 # ---------------------------------------------------------------------------
