@@ -67,6 +67,14 @@ module mcrt_mod
       integer(c_int64_t), intent(out) :: nfail, nevals
       real(c_float), intent(out) :: ms
     end function
+    ! external illumination: isrf_mod's tables and isrf_frac
+    integer(c_int) function mcrt_set_isrf(ctx, nisrf, cdf, logcdf, logfreq, isrf_frac) bind(C, name='mcrt_set_isrf')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nisrf
+      real(c_double), intent(in) :: cdf(*), logcdf(*), logfreq(*)
+      real(c_double), value :: isrf_frac
+    end function
     integer(c_int) function mcrt_set_source(ctx, rstar, tstar, rmin) bind(C, name='mcrt_set_source')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
