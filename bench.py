@@ -15,8 +15,9 @@ The default step is the workload's full Lucy iteration (10^8 packets on the
 200x180x64 model). At that size the bulk of the packets sets the step time;
 the serial chains of the few longest packets (up to ~10^6 interactions, a
 few seconds) overlap with it except at the very end.  The FP64 comparison
-runs a quarter of the step's packets in both precisions on identical RNG
-counters and dust temperatures.
+runs the last timed step again in FP64, from the same starting dust
+temperature on the same RNG counters, timed the same way; the SED and
+temperature errors compare the two steps' results.
 
 python bench.py --copy-probe BYTES only copies BYTES of device memory with
 torch (one copy kernel): the known-size kernel that scales rocprofv3's
@@ -319,12 +320,13 @@ def main():
     # temperature warm-up: short Lucy iterations so the timed steps transport
     # packets through a heated disk (re-emission at the dust temperature)
     nw = 2_000_000
+    tcur = m.tdust_initial()
     for k in range(4):
         eng.zero_tallies()
         eng.run_packets(nw, seed=1000 + k, precision=MCRT_MIXED, first_packet=0)
         eng.synchronize()
         tw = eng.last_run_ms() / 1e3
-        eng.dusttemp(nw, update_tdust=True)
+        tcur = eng.dusttemp(nw, update_tdust=True)
         log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s)")
     # the workload's packets per Lucy iteration
     N = args.packets or int(m.npackets)
@@ -334,12 +336,13 @@ def main():
     step_ms, kern_ms, counters, stats = [], [], [], []
     first = 0
     for s in range(args.warmup + args.steps):
+        t_start, first_step = tcur, first         # the step's starting temperature and counters
         eng.synchronize()
         t0 = time.perf_counter()
         eng.zero_tallies()
         eng.run_packets(N, seed=seed, precision=MCRT_MIXED, first_packet=first)
         tal = eng.tallies()                        # post-reduction tallies to the host
-        eng.dusttemp(N, update_tdust=True)
+        tcur = eng.dusttemp(N, update_tdust=True)
         eng.synchronize()
         dt = (time.perf_counter() - t0) * 1e3
         km = eng.last_run_ms()
@@ -368,36 +371,38 @@ def main():
     log(f"kernels per step: transport {st['transport_ms']:.1f} ms ({b_tr/1e9:.2f} GB algorithmic), "
         f"peel {st['peel_ms']:.1f} ms ({b_pe/1e9:.2f} GB), rounds {st['rounds']:.0f}")
 
-    # FP64 path and the mixed path's error against it on identical counters
-    tdust_state = None
+    # FP64 path and the mixed path's error against it: the last timed step
+    # again in FP64, from the same temperature on the same counters, timed
+    # the same way (zero, run, read back, invert)
+    tdust_state = t_start
     extra = {}
     if not args.no_fp64:
-        Nc = min(N, max(1_000_000, N // 4))
-        tfix = eng.dusttemp(N, update_tdust=False)
-        eng.set_tdust(tfix)
-        res = {}
-        for prec in (MCRT_FP64, MCRT_MIXED):
-            eng.set_tdust(tfix)
-            eng.zero_tallies()
-            eng.synchronize()
-            t0 = time.perf_counter()
-            eng.run_packets(Nc, seed=seed + 1, precision=prec, first_packet=10 ** 11)
-            tal = eng.tallies()
-            T = eng.dusttemp(Nc, mode=1, update_tdust=False)
-            eng.synchronize()
-            dt = time.perf_counter() - t0
-            res[prec] = (tal, T, dt, eng.last_run_ms())
-            log(f"{'fp64' if prec == MCRT_FP64 else 'mixed'} comparison run: {Nc} packets in {dt:.2f}s "
-                f"({Nc/dt:.4g} packets/s)")
-        tdust_state = tfix
-        sed_e = sed_error(res[MCRT_MIXED][0], res[MCRT_FP64][0])
-        t_mean, t_p99 = t_error(res[MCRT_MIXED][1], res[MCRT_FP64][1], mass)
-        fp64_rate = Nc / res[MCRT_FP64][2]
-        extra = dict(fp64_value=fp64_rate, mixed_over_fp64=(Nc / res[MCRT_MIXED][2]) / fp64_rate,
+        t_mixed = eng.dusttemp(N, mode=1, update_tdust=False)     # per-cell T of the mixed step's tallies
+        log("fp64 comparison: the timed step again in FP64 (the engine logs each round to stderr)")
+        eng.set_tdust(t_start)
+        verbose_before = os.environ.get("MCRT_VERBOSE")
+        os.environ["MCRT_VERBOSE"] = "1"             # a progress line per round of this long run
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.zero_tallies()
+        eng.run_packets(N, seed=seed, precision=MCRT_FP64, first_packet=first_step)
+        tal64 = eng.tallies()
+        eng.dusttemp(N, update_tdust=False)
+        eng.synchronize()
+        dt64 = time.perf_counter() - t0
+        if verbose_before is None:
+            del os.environ["MCRT_VERBOSE"]
+        else:
+            os.environ["MCRT_VERBOSE"] = verbose_before
+        t64 = eng.dusttemp(N, mode=1, update_tdust=False)
+        log(f"fp64 comparison run: {N} packets in {dt64:.2f}s ({N/dt64:.4g} packets/s); "
+            f"mixed step {step_ms[-1]/1e3:.2f}s")
+        sed_e = sed_error(tal, tal64)
+        t_mean, t_p99 = t_error(t_mixed, t64, mass)
+        fp64_rate = N / dt64
+        extra = dict(fp64_value=fp64_rate, mixed_over_fp64=dt64 / (step_ms[-1] / 1e3),
                      sed_err_vs_fp64=sed_e, t_err_vs_fp64_mass_weighted=t_mean, t_err_vs_fp64_p99=t_p99,
-                     fp64_comparison_packets=Nc)
-    else:
-        tdust_state = eng.dusttemp(N, update_tdust=False)
+                     fp64_comparison_packets=N)
 
     cpu = None
     if not args.no_cpu:
