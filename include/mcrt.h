@@ -120,6 +120,18 @@ int mcrt_build_models(mcrt_ctx *ctx, int32_t nmodels, const int32_t *nrg, const 
 int mcrt_set_isrf(mcrt_ctx *ctx, int32_t nisrf, const double *cdf, const double *logcdf,
                   const double *logfreq, double isrf_frac);
 
+/* Stellar hot spots (initpspot.f90 with ispot = 1; spotset.f90): nspot
+ * (1, or 2 antipodal) caps covering the fraction fspot of the surface,
+ * centred at latitude spotlat and longitude spotlon (degrees), emitting at
+ * tshock (K/11605).  Every mcrt_run_packets call then starts with its
+ * hot-spot photons (iphot < int(accfrac2 np), newdisktherm_5_fuv.f90:172-177
+ * and :296-313: emitted inside a cap, half of them as X-rays); with
+ * fspot < 1 a stellar photon emitted inside a cap takes tshock, with
+ * fspot = 1 every stellar photon does.  nspot = 0 turns spots off.
+ * Batched runs and the adjoint do not use spots. */
+int mcrt_set_spots(mcrt_ctx *ctx, int32_t nspot, double fspot, double spotlat, double spotlon,
+                   double tshock, double accfrac2);
+
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);
 

@@ -1198,17 +1198,45 @@ static int one_packet(orc_run *R, double *trace, int64_t i_run)
         emit_outside(R, &p);
         goto transport;
     }
-    /* initpspot.f90:27-160 */
+    /* initpspot.f90:27-160; a hot-spot photon (newdisktherm_5_fuv.f90:296-313,
+     * ispotph = 1) redraws its point until it lies in a spot */
+    const int hot = m->ispot && i_run < m->hot_to;
+    int ispotph = hot;
     p.sip = 1.0; p.sqp = 0.0; p.sup = 0.0; p.svp = 0.0;
     for (;;) {
-        p.cosb = 1.0 - 2.0 * ran(&R->rng);
-        p.sinb = sqrt(1.0 - p.cosb * p.cosb);
-        if (p.cosb != 0.0 && fabs(p.cosb) < 1.0) break;
+        for (;;) {
+            p.cosb = 1.0 - 2.0 * ran(&R->rng);
+            p.sinb = sqrt(1.0 - p.cosb * p.cosb);
+            if (p.cosb != 0.0 && fabs(p.cosb) < 1.0) break;
+        }
+        p.lp = R2P * ran(&R->rng);
+        if (!m->ispot) break;
+        const double cphi = cos(p.lp), sphi = sin(p.lp), sthet = p.sinb, cthet = p.cosb;
+        const double dotsp1 = m->scspot1 * sthet * cphi + m->ssspot1 * sthet * sphi + m->cspot1 * cthet;
+        const double dotsp2 = m->scspot2 * sthet * cphi + m->ssspot2 * sthet * sphi + m->cspot2 * cthet;
+        if (m->nspot == 1) {
+            if (ispotph == 1 && dotsp1 < m->csprad) continue;
+        } else {
+            if (ispotph == 1 && dotsp1 < m->csprad && dotsp2 < m->csprad) continue;
+        }
+        if (m->fspot < 1.0) {
+            double dellon = (p.lp < PI) ? p.lp : R2P - p.lp;
+            double costhp = m->cspot1 * p.cosb + m->sspot1 * p.sinb * cos(dellon);
+            if (costhp > m->csprad) ispotph = 1;
+            if (m->nspot == 2) {
+                dellon = (p.lp < PI) ? PI - p.lp : p.lp - PI;
+                costhp = m->cspot2 * p.cosb + m->sspot2 * p.sinb * cos(dellon);
+                if (costhp > m->csprad) ispotph = 1;
+            }
+        }
+        break;
     }
-    p.lp = R2P * ran(&R->rng);
     initp_sample(R, &p);
     initp_transform(&p);
-    p.nub = bbfreq(&R->rng, m->tstar);
+    if (m->ispot && (m->fspot == 1.0 || ispotph == 1)) p.nub = bbfreq(&R->rng, m->tshock);
+    else p.nub = bbfreq(&R->rng, m->tstar);
+    if (hot && ran(&R->rng) < 0.5)          /* half the hot-spot photons are X-rays */
+        p.nub = 1.2398 / 0.06 + ran(&R->rng) * (1.2398 / 0.015 - 1.2398 / 0.06);
     p.i_orig = 1;
     opacset(m, p.nub, p.op);
     p.wave = 1.2398 / p.nub;

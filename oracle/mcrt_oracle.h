@@ -70,6 +70,12 @@ typedef struct {
     int32_t nisrf;
     const double *isrf_cdf, *isrf_logcdf, *isrf_logfreq;   /* nisrf */
     int64_t isrf_from;
+    /* stellar hot spots (initpspot.f90 with ispot = 1, spotset.f90): packet
+     * i of a run is a hot-spot photon when i < hot_to */
+    int32_t ispot, nspot;
+    double fspot, csprad, tshock;
+    double scspot1, ssspot1, cspot1, sspot1, scspot2, ssspot2, cspot2, sspot2;
+    int64_t hot_to;
 } orc_model;
 
 /* Tallies: every pointer is caller-owned, zeroed by the caller, accumulated here.

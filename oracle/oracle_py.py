@@ -41,6 +41,10 @@ class OrcModel(C.Structure):
         ("iintmax", C.c_int32), ("taumax", C.c_double),
         ("nisrf", C.c_int32), ("isrf_cdf", _dp), ("isrf_logcdf", _dp), ("isrf_logfreq", _dp),
         ("isrf_from", C.c_int64),
+        ("ispot", C.c_int32), ("nspot", C.c_int32), ("fspot", C.c_double), ("csprad", C.c_double),
+        ("tshock", C.c_double), ("scspot1", C.c_double), ("ssspot1", C.c_double), ("cspot1", C.c_double),
+        ("sspot1", C.c_double), ("scspot2", C.c_double), ("ssspot2", C.c_double), ("cspot2", C.c_double),
+        ("sspot2", C.c_double), ("hot_to", C.c_int64),
     ]
 
 
@@ -167,6 +171,20 @@ def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+def spot_geometry(nspot, fspot, spotlat, spotlon):
+    """gridset_5_model13new.f90:159-165 (spot radius in degrees from the
+    filling factor) and spotset.f90:9-27 (cap centres and cos radius)."""
+    import math
+    pi = 3.1415926535897932384626433
+    deg2rad, rad2deg = pi / 180.0, 180.0 / pi
+    thspot = (math.acos(1.0 - 2.0 * fspot) if nspot == 1 else math.acos(1.0 - fspot)) * rad2deg
+    th1, ph1 = (90.0 - spotlat) * deg2rad, spotlon * deg2rad
+    th2, ph2 = pi - th1, ph1 + pi
+    return dict(scspot1=math.sin(th1) * math.cos(ph1), ssspot1=math.sin(th1) * math.sin(ph1), cspot1=math.cos(th1),
+                sspot1=math.sin(th1), scspot2=math.sin(th2) * math.cos(ph2), ssspot2=math.sin(th2) * math.sin(ph2),
+                cspot2=math.cos(th2), sspot2=math.sin(th2), csprad=math.cos(thspot * deg2rad))
+
+
 class OracleModel:
     """Holds the numpy arrays alive and exposes the C struct."""
 
@@ -203,6 +221,12 @@ class OracleModel:
                 self.arrs["isrf_" + k] = f(m.isrf[k])
                 setattr(s, "isrf_" + k, _p(self.arrs["isrf_" + k], C.c_double))
             s.nisrf = len(self.arrs["isrf_cdf"])
+        sp = getattr(m, "spots", None)
+        if sp:
+            g = spot_geometry(sp["nspot"], sp["fspot"], sp["spotlat"], sp["spotlon"])
+            s.ispot, s.nspot, s.fspot, s.tshock = 1, sp["nspot"], sp["fspot"], sp["tshock"]
+            for k, v in g.items():
+                setattr(s, k, v)
         self.s = s
         self.m = m
 
@@ -217,6 +241,13 @@ class OracleModel:
             self.s.isrf_from = int(npackets) - npout
         else:
             self.s.isrf_from = -1
+        sp = getattr(m, "spots", None)
+        if sp:
+            # newdisktherm_5_fuv.f90:172-177, :296: iphot < int(accfrac2 np) (1-based)
+            npaccstar = int(sp["accfrac2"] * float(npackets))
+            self.s.hot_to = max(npaccstar - 1, 0)
+        else:
+            self.s.hot_to = 0
 
 
 def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_mode=0, trace=False):

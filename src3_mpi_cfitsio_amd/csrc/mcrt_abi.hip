@@ -31,6 +31,9 @@ struct mcrt_ctx {
   double isrf_frac = 0.0;
   int nisrf = 0;
   const double *isrf_cdf = nullptr, *isrf_logcdf = nullptr, *isrf_logfreq = nullptr;
+  // stellar hot spots (mcrt_set_spots): geometry in spot, hot-spot photon fraction
+  DeviceModel spot{};
+  double accfrac2 = 0.0;
   size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
   int64_t trace_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -550,6 +553,42 @@ int mcrt_set_isrf(mcrt_ctx* c, int32_t nisrf, const double* cdf, const double* l
   return MCRT_OK;
 }
 
+int mcrt_set_spots(mcrt_ctx* c, int32_t nspot, double fspot, double spotlat, double spotlon, double tshock,
+                   double accfrac2) {
+  if (!c) return MCRT_E_ARG;
+  if (nspot == 0) {
+    c->spot.ispot = 0;
+    c->accfrac2 = 0.0;
+    return MCRT_OK;
+  }
+  if (nspot != 1 && nspot != 2) return fail(c, MCRT_E_ARG, "mcrt_set_spots: nspot must be 0, 1 or 2");
+  if (!(fspot > 0.0 && fspot <= 1.0)) return fail(c, MCRT_E_ARG, "mcrt_set_spots: fspot must be in (0, 1]");
+  if (!(tshock > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_spots: tshock must be > 0");
+  if (!(accfrac2 >= 0.0 && accfrac2 < 1.0)) return fail(c, MCRT_E_ARG, "mcrt_set_spots: accfrac2 must be in [0, 1)");
+  const double pi = 3.1415926535897932384626433, deg2rad = pi / 180.0, rad2deg = 180.0 / pi;
+  // gridset_5_model13new.f90:159-165: the spot radius in degrees from the filling factor
+  const double thspot = (nspot == 1 ? std::acos(1.0 - 2.0 * fspot) : std::acos(1.0 - fspot)) * rad2deg;
+  // spotset.f90:9-27
+  const double th0sp1 = (90.0 - spotlat) * deg2rad, phi0sp1 = spotlon * deg2rad;
+  const double th0sp2 = pi - th0sp1, phi0sp2 = phi0sp1 + pi;
+  DeviceModel& d = c->spot;
+  d.ispot = 1;
+  d.nspot = nspot;
+  d.fspot = fspot;
+  d.tshock = tshock;
+  d.scspot1 = std::sin(th0sp1) * std::cos(phi0sp1);
+  d.ssspot1 = std::sin(th0sp1) * std::sin(phi0sp1);
+  d.cspot1 = std::cos(th0sp1);
+  d.sspot1 = std::sin(th0sp1);
+  d.scspot2 = std::sin(th0sp2) * std::cos(phi0sp2);
+  d.ssspot2 = std::sin(th0sp2) * std::sin(phi0sp2);
+  d.cspot2 = std::cos(th0sp2);
+  d.sspot2 = std::sin(th0sp2);
+  d.csprad = std::cos(thspot * deg2rad);
+  c->accfrac2 = accfrac2;
+  return MCRT_OK;
+}
+
 int mcrt_set_source(mcrt_ctx* c, double rstar, double tstar, double rmin) {
   if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
@@ -648,6 +687,16 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.isrf_logcdf = c->isrf_logcdf;
     dm.isrf_logfreq = c->isrf_logfreq;
     dm.isrf_from = (unsigned long long)(npackets - npout);
+  }
+  if (c->spot.ispot) {
+    const DeviceModel& sp = c->spot;
+    dm.ispot = 1; dm.nspot = sp.nspot; dm.fspot = sp.fspot; dm.csprad = sp.csprad; dm.tshock = sp.tshock;
+    dm.scspot1 = sp.scspot1; dm.ssspot1 = sp.ssspot1; dm.cspot1 = sp.cspot1; dm.sspot1 = sp.sspot1;
+    dm.scspot2 = sp.scspot2; dm.ssspot2 = sp.ssspot2; dm.cspot2 = sp.cspot2; dm.sspot2 = sp.sspot2;
+    // newdisktherm_5_fuv.f90:172-177, :296: npaccstar = int(accfrac2 np) and
+    // the photons iphot < npaccstar (1-based) are hot-spot photons
+    const int64_t npaccstar = (int64_t)(c->accfrac2 * (double)npackets);
+    dm.hot_to = (unsigned long long)(npaccstar > 1 ? npaccstar - 1 : 0);
   }
   // Resident lanes: as many 256-thread blocks per CU as the transport
   // kernel's registers and LDS allow (never more than the packets need).

@@ -77,6 +77,14 @@ struct DeviceModel {
   int nisrf;
   const double *isrf_cdf, *isrf_logcdf, *isrf_logfreq;
   unsigned long long isrf_from;
+  // stellar hot spots (initpspot.f90 with ispot = 1, spotset.f90): nspot
+  // caps of cos radius csprad centred on (scspot, ssspot, cspot) unit
+  // vectors; stellar photons emitted inside a cap (or all, fspot = 1) have
+  // the shock temperature; packets k < hot_to of a run are hot-spot photons
+  int ispot, nspot;
+  double fspot, csprad, tshock;
+  double scspot1, ssspot1, cspot1, sspot1, scspot2, ssspot2, cspot2, sspot2;
+  unsigned long long hot_to;
 };
 
 struct DeviceTallies {

@@ -1132,15 +1132,38 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
   const bool outside = m.nisrf > 0 && L.pid >= m.isrf_from;
+  // hot-spot photons (newdisktherm_5_fuv.f90:296-313) come first in a run
+  const bool hot = m.ispot != 0 && !outside && L.pid < m.hot_to;
+  bool spot = hot;
   p.on_wall = false;
   p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
   if (!outside) {
+    // initpspot.f90:27-125: a point on the star; a hot-spot photon redraws
+    // until the point lies in a spot, and with fspot < 1 a stellar photon
+    // emitted inside a spot takes the spot's temperature
     for (;;) {
-      p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
-      p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
-      if (p.cosb != (T)0 && RT<T>::fabs_(p.cosb) < (T)1) break;
+      for (;;) {
+        p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
+        p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
+        if (p.cosb != (T)0 && RT<T>::fabs_(p.cosb) < (T)1) break;
+      }
+      p.lp = (T)kR2p * uniform<T>(C.rng);
+      if (!hot) break;
+      const T cphi = RT<T>::cos_(p.lp), sphi = RT<T>::sin_(p.lp);
+      const T d1 = (T)m.scspot1 * p.sinb * cphi + (T)m.ssspot1 * p.sinb * sphi + (T)m.cspot1 * p.cosb;
+      const T d2 = (T)m.scspot2 * p.sinb * cphi + (T)m.ssspot2 * p.sinb * sphi + (T)m.cspot2 * p.cosb;
+      const bool out1 = d1 < (T)m.csprad;
+      if (m.nspot == 1 ? !out1 : !(out1 && d2 < (T)m.csprad)) break;
     }
-    p.lp = (T)kR2p * uniform<T>(C.rng);
+    if (m.ispot != 0 && m.fspot < 1.0) {
+      const T pi = (T)(0.5 * kR2p);
+      T dellon = (p.lp < pi) ? p.lp : (T)kR2p - p.lp;
+      if ((T)m.cspot1 * p.cosb + (T)m.sspot1 * p.sinb * RT<T>::cos_(dellon) > (T)m.csprad) spot = true;
+      if (m.nspot == 2) {
+        dellon = (p.lp < pi) ? pi - p.lp : p.lp - pi;
+        if ((T)m.cspot2 * p.cosb + (T)m.sspot2 * p.sinb * RT<T>::cos_(dellon) > (T)m.csprad) spot = true;
+      }
+    }
     initp_sample(C, p);
   } else {
     p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
@@ -1153,7 +1176,11 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
     p.lp = (T)kR2p * uniform<T>(C.rng);
   }
   initp_transform(p);
-  p.nub = outside ? rep_isrf<T>(C) : bbfreq<T>(C.rng, (T)m.tstar, C.inczeta);
+  const bool shock = m.ispot != 0 && (spot || m.fspot == 1.0);
+  p.nub = outside ? rep_isrf<T>(C) : bbfreq<T>(C.rng, (T)(shock ? m.tshock : m.tstar), C.inczeta);
+  // half the hot-spot photons are X-rays, uniform in h nu over 0.015..0.06 um
+  if (hot && uniform<T>(C.rng) < (T)0.5)
+    p.nub = (T)(1.2398 / 0.06) + uniform<T>(C.rng) * (T)(1.2398 / 0.015 - 1.2398 / 0.06);
   p.i_orig = outside ? 5 : 1;
   p.iflag = 0;
   p.exitflag = 0;

@@ -106,7 +106,17 @@ class Engine:
             m.nfreq, m.nmu, m.nph, m.nap, ptr(obs["ap"], d), ptr(obs["mu"], d), m.numin, m.lnurat))
         self.set_tdust(m.tdust_initial())
         self.set_isrf(m.isrf_frac, m.isrf)
+        self.set_spots(m.spots)
         self._check(L.mcrt_zero_tallies(self.h))
+
+    def set_spots(self, spots=None):
+        """Stellar hot spots (dict as model.with_spots makes it); None turns them off."""
+        if not spots:
+            self._check(self.lib.mcrt_set_spots(self.h, 0, 0.0, 0.0, 0.0, 0.0, 0.0))
+            return
+        self._check(self.lib.mcrt_set_spots(self.h, int(spots["nspot"]), float(spots["fspot"]),
+                                            float(spots["spotlat"]), float(spots["spotlon"]),
+                                            float(spots["tshock"]), float(spots["accfrac2"])))
 
     def set_isrf(self, isrf_frac: float, tables=None):
         """External illumination: the last isrf_frac of every run's packets

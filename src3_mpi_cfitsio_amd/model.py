@@ -111,6 +111,9 @@ class Model:
     # the packets that enter from outside, and the ISRF sampling tables
     isrf_frac: float = 0.0
     isrf: Optional[Dict[str, np.ndarray]] = dataclasses.field(default=None, repr=False, compare=False)
+    # stellar hot spots (initpspot.f90, spotset.f90): nspot, fspot,
+    # spotlat/spotlon (deg), tshock (K/11605), accfrac2 (hot-spot photons)
+    spots: Optional[Dict[str, float]] = None
 
     @property
     def ncell(self) -> int:
@@ -480,6 +483,15 @@ def isrf_split(npackets: int, isrf_frac: float) -> int:
     if isrf_frac * float(npackets) - npout > 0.5:
         npout += 1
     return npackets - npout
+
+
+def with_spots(m: Model, nspot: int = 1, fspot: float = 0.01, spotlat: float = 60.0, spotlon: float = 0.0,
+               tshock_k: float = 8000.0, accfrac2: float = 0.1) -> Model:
+    """The model with nspot accretion hot spots covering fspot of the star
+    at tshock_k, and accfrac2 of every run's packets emitted from them."""
+    return dataclasses.replace(m, spots=dict(nspot=int(nspot), fspot=float(fspot), spotlat=float(spotlat),
+                                             spotlon=float(spotlon), tshock=float(tshock_k) / TK,
+                                             accfrac2=float(accfrac2)))
 
 
 def with_isrf(m: Model, isrf_frac: float, nisrf: int = 200) -> Model:
