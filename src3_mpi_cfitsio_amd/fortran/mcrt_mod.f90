@@ -75,6 +75,14 @@ module mcrt_mod
       real(c_double), intent(in) :: cdf(*), logcdf(*), logfreq(*)
       real(c_double), value :: isrf_frac
     end function
+    ! stellar hot spots: spot_mod's parameters and the hot-spot photon fraction
+    integer(c_int) function mcrt_set_spots(ctx, nspot, fspot, spotlat, spotlon, tshock, accfrac2) &
+         bind(C, name='mcrt_set_spots')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nspot
+      real(c_double), value :: fspot, spotlat, spotlon, tshock, accfrac2
+    end function
     integer(c_int) function mcrt_set_source(ctx, rstar, tstar, rmin) bind(C, name='mcrt_set_source')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
