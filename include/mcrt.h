@@ -124,13 +124,31 @@ int mcrt_set_isrf(mcrt_ctx *ctx, int32_t nisrf, const double *cdf, const double 
  * (1, or 2 antipodal) caps covering the fraction fspot of the surface,
  * centred at latitude spotlat and longitude spotlon (degrees), emitting at
  * tshock (K/11605).  Every mcrt_run_packets call then starts with its
- * hot-spot photons (iphot < int(accfrac2 np), newdisktherm_5_fuv.f90:172-177
- * and :296-313: emitted inside a cap, half of them as X-rays); with
+ * hot-spot photons, after any accretion photons (the iphot < npacc +
+ * int(accfrac2 np) that are not accretion photons,
+ * newdisktherm_5_fuv.f90:172-177 and :296-313: emitted inside a cap, half
+ * of them as X-rays); with
  * fspot < 1 a stellar photon emitted inside a cap takes tshock, with
  * fspot = 1 every stellar photon does.  nspot = 0 turns spots off.
  * Batched runs and the adjoint do not use spots. */
 int mcrt_set_spots(mcrt_ctx *ctx, int32_t nspot, double fspot, double spotlat, double spotlon,
                    double tshock, double accfrac2);
+
+/* Disk-accretion photons (accfrac > 0 in the reference; initpacc_4.f90,
+ * newdisktherm_5_fuv.f90:164-170 and :242-283): the first
+ * int(accfrac * npackets) - 1 packets of every mcrt_run_packets call start
+ * inside the disk and come before the hot-spot photons.  ldint holds the
+ * cumulative accretion luminosity by radial cell and hdisk the scale height
+ * (R*) of each radial cell (grid_mod's Ldint and hdisk as
+ * diskprofile_model11.f90:158-227 leaves them: nrg - 1 values each, Ldint
+ * from 0 in the first cell to 1 in the last).  A photon is placed in the
+ * cell j with Ldint(j) <= xi < Ldint(j+1), at that cell's mid radius, a
+ * random azimuth and a Gaussian height of scale hdisk(j), redrawn until
+ * the cell holds dust; the dust absorbs it (nabs, dbig) and re-emits it at
+ * the cell's temperature, peeled with origin 2 (disk).  Set the grid first.
+ * accfrac = 0 turns it off.  Batched runs and the adjoint do not emit
+ * accretion photons. */
+int mcrt_set_accretion(mcrt_ctx *ctx, const double *ldint, const double *hdisk, double accfrac);
 
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);

@@ -1030,6 +1030,20 @@ static int select_dust(orc_run *R, const orc_pkt *p)
     return R->m->species[CELL(R->m, p->ir, p->it, p->ip)];
 }
 
+/* reemit.f90:17-34 (lucydustfreq_2_test.f90 for the frequency): isotropic,
+ * unpolarized, at the dust temperature tave */
+static void reemit(orc_run *R, orc_pkt *p, double tave, int iop)
+{
+    p->sip = 1.0; p->sqp = 0.0; p->sup = 0.0; p->svp = 0.0;
+    p->cost = 1.0 - 2.0 * ran(&R->rng);
+    p->sint = sqrt(1.0 - p->cost * p->cost);
+    p->phi = R2P * ran(&R->rng);
+    p->cosp = cos(p->phi);
+    p->sinp = sin(p->phi);
+    p->nub = lucydustfreq(R->m, &R->rng, tave, iop);
+    p->uy = p->sint * p->sinp; p->ux = p->sint * p->cosp; p->uz = p->cost;
+}
+
 /* propagate_5_fuv.f90:1-326.  Returns exit code: 1 escaped, 2 killed, -1 error. */
 static int propagate(orc_run *R, orc_pkt *p, int *n_inter)
 {
@@ -1057,15 +1071,7 @@ static int propagate(orc_run *R, orc_pkt *p, int *n_inter)
                 p->i_orig = i_orig_of(idust2);
                 t->nabs[c] += 1;
                 /* emit_common_final.f90:73-81 -> reemit.f90:17-34 */
-                double tave = R->tdust[c];
-                p->sip = 1.0; p->sqp = 0.0; p->sup = 0.0; p->svp = 0.0;
-                p->cost = 1.0 - 2.0 * ran(&R->rng);
-                p->sint = sqrt(1.0 - p->cost * p->cost);
-                p->phi = R2P * ran(&R->rng);
-                p->cosp = cos(p->phi);
-                p->sinp = sin(p->phi);
-                p->nub = lucydustfreq(m, &R->rng, tave, iop);
-                p->uy = p->sint * p->sinp; p->ux = p->sint * p->cosp; p->uz = p->cost;
+                reemit(R, p, R->tdust[c], iop);
                 p->iflag = 0;
                 opacset(m, p->nub, p->op);
                 p->wave = 1.2398 / p->nub;
@@ -1186,6 +1192,90 @@ static void emit_outside(orc_run *R, orc_pkt *p)
     p->ip = orc_locate(m->phiarr, m->npg, p->lp);
 }
 
+#define ORC_ACC_TRIES 65536
+
+/* initpacc_4.f90:25-100 with emit_common_2.f90:37-85 (no diffusion) and
+ * reemit.f90, then newdisktherm_5_fuv.f90:249-283: a disk-accretion photon.
+ * Radial cell j with Ldint(j) <= xi < Ldint(j+1) (locate), the point at
+ * ravearr(j) = (rarr(j) + rarr(j+1))/2, azimuth from rantheta
+ * (random.f90:23-42), height hdisk(j) gasdev (random.f90:119-138, its saved
+ * second deviate kept per packet: the reference keeps one across packets),
+ * drawn again at the same radius until the cell holds dust; a point beyond
+ * the outer wall is drawn again too (the reference would read past the
+ * grid), and after ORC_ACC_TRIES draws the packet is dropped (0).  The dust
+ * absorbs it and re-emits it at the cell temperature; peeled as an emission
+ * of origin 2. */
+static int emit_acc(orc_run *R, orc_pkt *p, int *idust2)
+{
+    const orc_model *m = R->m;
+    orc_tallies *t = R->t;
+    const double xi = ran(&R->rng);
+    int j = orc_locate(m->acc_ldint, m->nrg - 1, xi);
+    if (j < 1) j = 1;
+    if (j > m->nrg - 1) j = m->nrg - 1;
+    const double rp = 0.5 * (m->rarr[j - 1] + m->rarr[j]);
+    const double hd = m->acc_hdisk[j - 1];
+    double gset = 0.0;
+    int iset = 0, found = 0;
+    size_t c = 0;
+    for (int tries = 0; tries < ORC_ACC_TRIES; ++tries) {
+        double x, y, r2 = 2.0;
+        while (r2 > 1.0) {
+            x = ran(&R->rng); x = x + x - 1.0;
+            y = ran(&R->rng); y = y + y - 1.0;
+            r2 = x * x + y * y;
+        }
+        const double r = sqrt(r2), sp = y / r, cp = x / r;
+        double g;
+        if (!iset) {
+            double v1, v2, rsq;
+            for (;;) {
+                v1 = 2.0 * ran(&R->rng) - 1.0;
+                v2 = 2.0 * ran(&R->rng) - 1.0;
+                rsq = v1 * v1 + v2 * v2;
+                if (rsq < 1.0 && rsq > 0.0) break;
+            }
+            const double fac = sqrt(-2.0 * log(rsq) / rsq);
+            gset = v1 * fac;
+            g = v2 * fac;
+            iset = 1;
+        } else {
+            g = gset;
+            iset = 0;
+        }
+        p->xp = rp * cp;
+        p->yp = rp * sp;
+        p->zp = hd * g;
+        p->rsq = p->xp * p->xp + p->yp * p->yp + p->zp * p->zp;
+        p->rtot = sqrt(p->rsq);
+        p->ir = orc_locate(m->r2arr, m->nrg, p->rsq);
+        if (p->ir < 1 || p->ir > m->nrg - 1) continue;
+        int it = orc_locate(m->costarr, m->ntg, p->zp / p->rtot);
+        double lp = atan2(p->yp, p->xp);
+        if (lp < 0.0) lp = lp + R2P;
+        int ip = orc_locate(m->phiarr, m->npg, lp);
+        p->it = it < 1 ? 1 : (it > m->ntg - 1 ? m->ntg - 1 : it);
+        p->ip = ip < 1 ? 1 : (ip > m->npg - 1 ? m->npg - 1 : ip);
+        c = CELL(m, p->ir, p->it, p->ip);
+        if (m->rho[c] > 0.0) { found = 1; break; }
+    }
+    if (!found) return 0;
+    *idust2 = select_dust(R, p);
+    t->dbig[c] += 1;
+    t->nabs[c] += 1;
+    const int iop = m->species_opac[*idust2];
+    reemit(R, p, R->tdust[c], iop);
+    p->on_wall = 0;
+    p->iflag = 0;
+    p->i_orig = 2;
+    opacset(m, p->nub, p->op);
+    p->wave = 1.2398 / p->nub;
+    set_kapd(p, m, 0);
+    if (m->ipeel == 1)
+        for (int pid = 1; pid <= m->npeel; ++pid) peeloff(R, p, p->sip, pid, *idust2);
+    return 1;
+}
+
 /* newdisktherm_5_fuv.f90:294-455: one photon, stellar (ips=0, no spots)
  * or, for the last packets of a run, from the external field */
 static int one_packet(orc_run *R, double *trace, int64_t i_run)
@@ -1194,6 +1284,19 @@ static int one_packet(orc_run *R, double *trace, int64_t i_run)
     orc_pkt p;
     memset(&p, 0, sizeof(p));
     p.on_wall = 0;
+    if (i_run < m->acc_to) {
+        int idust2 = 0;
+        if (!emit_acc(R, &p, &idust2)) {
+            R->t->counters[ORC_C_ERRORS] += 1.0;
+            if (trace) {
+                memset(trace, 0, 8 * sizeof(double));
+                trace[3] = 1.0;
+                trace[7] = -1.0;
+            }
+            return -1;
+        }
+        goto transport;
+    }
     if (m->isrf_from >= 0 && i_run >= m->isrf_from) {
         emit_outside(R, &p);
         goto transport;

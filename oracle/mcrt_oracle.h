@@ -71,11 +71,16 @@ typedef struct {
     const double *isrf_cdf, *isrf_logcdf, *isrf_logfreq;   /* nisrf */
     int64_t isrf_from;
     /* stellar hot spots (initpspot.f90 with ispot = 1, spotset.f90): packet
-     * i of a run is a hot-spot photon when i < hot_to */
+     * i of a run is a hot-spot photon when acc_to <= i < hot_to */
     int32_t ispot, nspot;
     double fspot, csprad, tshock;
     double scspot1, ssspot1, cspot1, sspot1, scspot2, ssspot2, cspot2, sspot2;
     int64_t hot_to;
+    /* disk-accretion photons (initpacc_4.f90): packet i of a run is one when
+     * i < acc_to; acc_ldint / acc_hdisk: Ldint and hdisk of nrg - 1 radial
+     * cells (grid_mod) */
+    int64_t acc_to;
+    const double *acc_ldint, *acc_hdisk;
 } orc_model;
 
 /* Tallies: every pointer is caller-owned, zeroed by the caller, accumulated here.

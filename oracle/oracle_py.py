@@ -45,6 +45,7 @@ class OrcModel(C.Structure):
         ("tshock", C.c_double), ("scspot1", C.c_double), ("ssspot1", C.c_double), ("cspot1", C.c_double),
         ("sspot1", C.c_double), ("scspot2", C.c_double), ("ssspot2", C.c_double), ("cspot2", C.c_double),
         ("sspot2", C.c_double), ("hot_to", C.c_int64),
+        ("acc_to", C.c_int64), ("acc_ldint", C.POINTER(C.c_double)), ("acc_hdisk", C.POINTER(C.c_double)),
     ]
 
 
@@ -227,12 +228,18 @@ class OracleModel:
             s.ispot, s.nspot, s.fspot, s.tshock = 1, sp["nspot"], sp["fspot"], sp["tshock"]
             for k, v in g.items():
                 setattr(s, k, v)
+        acc = getattr(m, "accretion", None)
+        if acc and acc["accfrac"] > 0.0:
+            for k in ("ldint", "hdisk"):
+                self.arrs["acc_" + k] = f(acc[k])
+                setattr(s, "acc_" + k, _p(self.arrs["acc_" + k], C.c_double))
         self.s = s
         self.m = m
 
     def set_run(self, npackets):
-        """The external-field packets of a run of npackets: the last ones
-        (newdisktherm_5_fuv.f90:158-187)."""
+        """The split of a run of npackets (newdisktherm_5_fuv.f90:158-187):
+        accretion photons first, then hot-spot photons, the external-field
+        packets last."""
         m = self.m
         if getattr(m, "isrf", None) is not None and m.isrf_frac > 0.0:
             npout = int(m.isrf_frac * float(npackets))
@@ -241,11 +248,16 @@ class OracleModel:
             self.s.isrf_from = int(npackets) - npout
         else:
             self.s.isrf_from = -1
+        # newdisktherm_5_fuv.f90:169-170, :242: iphot < npacc = int(accfrac np)
+        # (1-based) are accretion photons
+        acc = getattr(m, "accretion", None)
+        npacc = int(acc["accfrac"] * float(npackets)) if acc and acc["accfrac"] > 0.0 else 0
+        self.s.acc_to = max(npacc - 1, 0)
         sp = getattr(m, "spots", None)
         if sp:
-            # newdisktherm_5_fuv.f90:172-177, :296: iphot < int(accfrac2 np) (1-based)
+            # :172-177, :296: then hot-spot photons while iphot < npacc + npaccstar
             npaccstar = int(sp["accfrac2"] * float(npackets))
-            self.s.hot_to = max(npaccstar - 1, 0)
+            self.s.hot_to = max(npacc + npaccstar - 1, 0)
         else:
             self.s.hot_to = 0
 

@@ -34,6 +34,14 @@ hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, c
   return hipGetLastError();
 }
 
+hipError_t launch_acc_place_fp64(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out,
+                                  hipStream_t s) {
+  const unsigned grid = (unsigned)((m.acc_to + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_acc_place<double>, dim3(grid), dim3(256), 0, s, m, seed, first_packet, out);
+  return hipGetLastError();
+}
+
 int lane_words_fp64() { return lane_words<double>(); }
 
 int transport_blocks_per_cu_fp64(unsigned lds_bytes) {

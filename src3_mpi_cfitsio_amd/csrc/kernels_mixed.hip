@@ -11,6 +11,14 @@ hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, 
   return hipGetLastError();
 }
 
+hipError_t launch_acc_place_mixed(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out,
+                                  hipStream_t s) {
+  const unsigned grid = (unsigned)((m.acc_to + 255) / 256);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_acc_place<float>, dim3(grid), dim3(256), 0, s, m, seed, first_packet, out);
+  return hipGetLastError();
+}
+
 int lane_words_mixed() { return lane_words<float>(); }
 
 int transport_blocks_per_cu_mixed(unsigned lds_bytes) {

@@ -34,6 +34,11 @@ struct mcrt_ctx {
   // stellar hot spots (mcrt_set_spots): geometry in spot, hot-spot photon fraction
   DeviceModel spot{};
   double accfrac2 = 0.0;
+  // disk-accretion photons (mcrt_set_accretion): fraction of a run's packets,
+  // device tables of acc_n = nrg - 1 radial cells
+  double accfrac = 0.0;
+  int acc_n = 0;
+  const double *acc_ldint = nullptr, *acc_hdisk = nullptr;
   size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
   int64_t trace_n = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -589,6 +594,36 @@ int mcrt_set_spots(mcrt_ctx* c, int32_t nspot, double fspot, double spotlat, dou
   return MCRT_OK;
 }
 
+int mcrt_set_accretion(mcrt_ctx* c, const double* ldint, const double* hdisk, double accfrac) {
+  if (!c) return MCRT_E_ARG;
+  if (!(accfrac >= 0.0 && accfrac < 1.0)) return fail(c, MCRT_E_ARG, "mcrt_set_accretion: accfrac must be in [0, 1)");
+  if (accfrac == 0.0) {
+    c->accfrac = 0.0;
+    return MCRT_OK;
+  }
+  if (!c->has_grid) return fail(c, MCRT_E_STATE, "mcrt_set_accretion: set the grid first");
+  if (!ldint || !hdisk) return fail(c, MCRT_E_ARG, "mcrt_set_accretion: null table");
+  const int n = c->dm.nrg - 1;
+  if (n < 2) return fail(c, MCRT_E_ARG, "mcrt_set_accretion: at least two radial cells needed");
+  // the cell j with Ldint(j) <= xi < Ldint(j+1) must exist for every xi in [0, 1)
+  if (ldint[0] != 0.0 || ldint[n - 1] != 1.0)
+    return fail(c, MCRT_E_ARG, "mcrt_set_accretion: Ldint must run from 0 (first cell) to 1 (last cell)");
+  for (int i = 1; i < n; ++i)
+    if (!(ldint[i] >= ldint[i - 1])) return fail(c, MCRT_E_ARG, "mcrt_set_accretion: Ldint must be non-decreasing");
+  for (int i = 0; i < n; ++i)
+    if (!(hdisk[i] >= 0.0) || !std::isfinite(hdisk[i]))
+      return fail(c, MCRT_E_ARG, "mcrt_set_accretion: hdisk must be finite and >= 0");
+  HIPCHECK(c, hipSetDevice(c->device));
+  int rc = 0;
+  rc |= upload(c, "acc.ldint", ldint, (size_t)n, &c->acc_ldint);
+  rc |= upload(c, "acc.hdisk", hdisk, (size_t)n, &c->acc_hdisk);
+  if (rc) return rc;
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->acc_n = n;
+  c->accfrac = accfrac;
+  return MCRT_OK;
+}
+
 int mcrt_set_source(mcrt_ctx* c, double rstar, double tstar, double rmin) {
   if (c) c->b_gen++;
   if (!c) return MCRT_E_ARG;
@@ -676,9 +711,20 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   DeviceTallies t = c->dt;
   t.trace = (c->trace_n > 0) ? (double*)c->buf["trace"] : nullptr;
   const bool fp64 = (precision == MCRT_FP64);
-  // the run's external-field packets are its last npout
-  // (newdisktherm_5_fuv.f90:158-187 with no accretion photons)
+  // newdisktherm_5_fuv.f90:158-187, :242, :296: of np packets the first
+  // npacc - 1 = int(accfrac np) - 1 are disk-accretion photons, the next
+  // npaccstar = int(accfrac2 np) hot-spot photons (npaccstar - 1 without
+  // accretion photons), the last npout = nint(isrf_frac np) come from the
+  // external field
   DeviceModel dm = c->dm;
+  int64_t npacc = 0;
+  if (c->accfrac > 0.0) {
+    if (c->acc_n != c->dm.nrg - 1) return fail(c, MCRT_E_STATE, "mcrt_run_packets: the accretion tables predate the grid");
+    npacc = (int64_t)(c->accfrac * (double)npackets);
+    dm.acc_to = (unsigned long long)(npacc > 1 ? npacc - 1 : 0);
+    dm.acc_ldint = c->acc_ldint;
+    dm.acc_hdisk = c->acc_hdisk;
+  }
   if (c->nisrf > 0 && c->isrf_frac > 0.0) {
     int64_t npout = (int64_t)(c->isrf_frac * (double)npackets);
     if (c->isrf_frac * (double)npackets - (double)npout > 0.5) npout += 1;
@@ -693,10 +739,10 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.ispot = 1; dm.nspot = sp.nspot; dm.fspot = sp.fspot; dm.csprad = sp.csprad; dm.tshock = sp.tshock;
     dm.scspot1 = sp.scspot1; dm.ssspot1 = sp.ssspot1; dm.cspot1 = sp.cspot1; dm.sspot1 = sp.sspot1;
     dm.scspot2 = sp.scspot2; dm.ssspot2 = sp.ssspot2; dm.cspot2 = sp.cspot2; dm.sspot2 = sp.sspot2;
-    // newdisktherm_5_fuv.f90:172-177, :296: npaccstar = int(accfrac2 np) and
-    // the photons iphot < npaccstar (1-based) are hot-spot photons
+    // the photons iphot < npacc + npaccstar (1-based) that are not accretion
+    // photons are hot-spot photons
     const int64_t npaccstar = (int64_t)(c->accfrac2 * (double)npackets);
-    dm.hot_to = (unsigned long long)(npaccstar > 1 ? npaccstar - 1 : 0);
+    dm.hot_to = (unsigned long long)(npacc + npaccstar > 1 ? npacc + npaccstar - 1 : 0);
   }
   // Resident lanes: as many 256-thread blocks per CU as the transport
   // kernel's registers and LDS allow (never more than the packets need).
@@ -739,6 +785,10 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     q.recs = p;
     q.capacity = (unsigned)capacity;
   }
+  if (dm.acc_to > 0) {
+    if ((rc = dev_alloc(c, "acc.start", (size_t)dm.acc_to * sizeof(AccStart), &p))) return rc;
+    dm.acc_start = (const AccStart*)p;
+  }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
   double st[MCRT_NSTATS] = {0};
   const char* venv = std::getenv("MCRT_VERBOSE");
@@ -749,6 +799,11 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   st[MCRT_STAT_LANE_BYTES] = 4.0 * words;
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
   HIPCHECK(c, launch_fill_i32((int*)ctl, sizeof(RunCtl) / 4, 0, c->stream));
+  if (dm.acc_to > 0) {
+    AccStart* acc = (AccStart*)c->buf["acc.start"];
+    HIPCHECK(c, fp64 ? launch_acc_place_fp64(dm, seed, (uint64_t)first_packet, acc, c->stream)
+                     : launch_acc_place_mixed(dm, seed, (uint64_t)first_packet, acc, c->stream));
+  }
   int64_t round = 0;
   for (;; ++round) {
     if (round >= ((int64_t)1 << 24)) return fail(c, MCRT_E_HIP, "mcrt_run_packets: packets did not finish");

@@ -100,7 +100,9 @@ struct AdjParams {
 template <class T>
 __device__ __forceinline__ void adj_run_packet(Ctx<T>& C, Life<T>& L, uint64_t seed, uint64_t pkt) {
   rng_init(C.rng, seed, pkt);
-  emit<T, true>(C, L);
+  const int e = emit<T, true>(C, L);
+  if (e == 0) return;                      // an accretion photon with no dust cell
+  if (e == 2) interact<T, true>(C, L);
   for (;;) {
     const int r = walk_step<T, true>(C, L);
     if (r == 0) continue;

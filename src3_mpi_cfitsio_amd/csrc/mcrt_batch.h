@@ -175,8 +175,9 @@ __global__ void __launch_bounds__(BS) k_batch(DeviceModel m0, DeviceTallies t0, 
           bind_model<T>(m, t, t0, bd, model);
           rng_init(C.rng, seed + (uint64_t)model, pk);
           L.pid = pk;
-          emit<T>(C, L);
-          st = L_WALK;
+          const int e = emit<T>(C, L);
+          if (e == 0) end_packet<T>(C, L, -1);
+          else st = (e == 1) ? L_WALK : L_READY;
         }
       }
     }

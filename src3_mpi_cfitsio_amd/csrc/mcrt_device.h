@@ -80,12 +80,29 @@ struct DeviceModel {
   // stellar hot spots (initpspot.f90 with ispot = 1, spotset.f90): nspot
   // caps of cos radius csprad centred on (scspot, ssspot, cspot) unit
   // vectors; stellar photons emitted inside a cap (or all, fspot = 1) have
-  // the shock temperature; packets k < hot_to of a run are hot-spot photons
+  // the shock temperature; packets acc_to <= k < hot_to of a run are
+  // hot-spot photons
   int ispot, nspot;
   double fspot, csprad, tshock;
   double scspot1, ssspot1, cspot1, sspot1, scspot2, ssspot2, cspot2, sspot2;
   unsigned long long hot_to;
+  // disk-accretion photons (initpacc_4.f90): packets k < acc_to of a run;
+  // acc_ldint[nrg-1] is the cumulative accretion luminosity by radial cell
+  // (0 first, 1 last), acc_hdisk[nrg-1] the disk scale height (R*)
+  unsigned long long acc_to;
+  const double *acc_ldint, *acc_hdisk;
+  const struct AccStart* acc_start;   // [acc_to], written by k_acc_place
 };
+
+// where k_acc_place put accretion photon k: position, cell, species and the
+// packet's next random draw (sp = -1: no cell with dust was found)
+struct AccStart {
+  double xp, yp, zp, rsq;
+  int32_t ir, it, ip, sp;
+  uint32_t n, pad;
+};
+
+constexpr int kAccTries = 1 << 16;   // position draws before an accretion photon is dropped
 
 struct DeviceTallies {
   double* dtauabs;
@@ -127,16 +144,22 @@ __device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet)
   r.n = 0;
 }
 
-// Continue a stream at draw n (the block holding draw n is regenerated when
-// n is not at a block boundary; rng_word regenerates it otherwise).
-__device__ __forceinline__ void rng_resume(Rng& r, uint64_t seed, uint64_t packet, uint32_t n) {
-  rng_init(r, seed, packet);
+// Move a stream (keys kept) to draw n: the block holding draw n is
+// regenerated when n is not at a block boundary; rng_word regenerates it
+// otherwise.
+__device__ __forceinline__ void rng_seek(Rng& r, uint32_t n) {
   r.n = n;
   if (n & 3u) {
     uint32_t o[4];
     philox(n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
     r.b0 = o[0]; r.b1 = o[1]; r.b2 = o[2]; r.b3 = o[3];
   }
+}
+
+// Continue a stream at draw n
+__device__ __forceinline__ void rng_resume(Rng& r, uint64_t seed, uint64_t packet, uint32_t n) {
+  rng_init(r, seed, packet);
+  rng_seek(r, n);
 }
 
 __device__ __forceinline__ uint32_t rng_word(Rng& r) {

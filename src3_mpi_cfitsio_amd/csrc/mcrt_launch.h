@@ -27,6 +27,9 @@ struct BatchLaunch {
 // rp.lanes must be a multiple of 256
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
+// k_acc_place over the run's accretion photons k < m.acc_to
+hipError_t launch_acc_place_fp64(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out, hipStream_t s);
+hipError_t launch_acc_place_mixed(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out, hipStream_t s);
 int lane_words_fp64();
 int lane_words_mixed();
 // resident 256-thread blocks of k_transport per CU with lds_bytes of dynamic LDS (0: query failed)

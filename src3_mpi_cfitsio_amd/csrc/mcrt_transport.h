@@ -1012,44 +1012,55 @@ __device__ __forceinline__ bool peel_hidden(const DeviceModel& m, Life<T>& L) {
 
 // propagate_5_fuv.f90:62-198: one interaction (scatter, or absorb and
 // re-emit), its peel-off record, and the start of the next flight.
+// reemit.f90:17-34 with lucydustfreq_2_test.f90:34-45: isotropic and
+// unpolarized, at the cell's dust temperature (a Planck frequency at or
+// above Tmax)
+template <class T>
+__device__ __forceinline__ void reemit(Ctx<T>& C, Pkt<T>& p, T tave, int iop) {
+  const DeviceModel& m = C.m;
+  p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+  p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
+  p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
+  p.phi = (T)kR2p * uniform<T>(C.rng);
+  p.cosp = RT<T>::cos_(p.phi);
+  p.sinp = RT<T>::sin_(p.phi);
+  if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
+  else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
+  set_dir<T>(p);
+}
+
 template <class T, bool kAdj = false>
 __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
-  const T R2P = (T)kR2p;
   Pkt<T>& p = L.p;
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   // idust2 is the species of this cell (flight_end), so its opacity table is cached
   int iop = (L.c_sp == (uint32_t)c) ? L.spop : m.species_opac[L.idust2];
-  L.nint++;
-  if (uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
+  // an accretion photon's absorption where it was emitted (emit; no albedo
+  // draw, initpacc_4.f90:75-100, newdisktherm_5_fuv.f90:247-283)
+  const bool start = (p.i_orig == 0);
+  if (!start) L.nint++;
+  if (!start && uniform<T>(C.rng) <= pick(p.op, iop).albedo) {
     p.iflag = 1; L.iscat++; L.iint++;
     if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
     stokes<T>(C, p, iop);
   } else {
     atomicAdd(&t.dbig[c], 1);
-    p.i_orig = i_orig_of(L.idust2);
+    p.i_orig = start ? 2 : i_orig_of(L.idust2);
     atomicAdd(&t.nabs[c], 1);
-    T tave = cell_tdust<T>(m, L, c);
-    p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
-    p.cost = (T)1 - (T)2 * uniform<T>(C.rng);
-    p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
-    p.phi = R2P * uniform<T>(C.rng);
-    p.cosp = RT<T>::cos_(p.phi);
-    p.sinp = RT<T>::sin_(p.phi);
-    if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
-    else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
-    set_dir<T>(p);
+    reemit<T>(C, p, cell_tdust<T>(m, L, c), iop);
     p.iflag = 0;
     opacset<T>(m, p.nub, p.op);
     p.wave = (T)1.2398 / p.nub;
-    set_kapd(p, m, 1);
+    set_kapd(p, m, start ? 0 : 1);
     if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
-    L.iint++;
+    if (!start) L.iint++;
   }
   T tau = RT<T>::tau_of(uniform<T>(C.rng));
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
   set_dir<T>(p);
+  if (start) L.first = true;
   flight_begin(L, tau);
 }
 
@@ -1120,23 +1131,130 @@ __device__ __forceinline__ T rep_isrf(Ctx<T>& C) {
   }
 }
 
+// initpacc_4.f90:25-100 with emit_common_2.f90:37-85 (a cell outside the
+// diffusion region) and reemit.f90: a disk-accretion photon.  The radial
+// cell follows the cumulative accretion luminosity (locate on Ldint: the
+// cell j with Ldint(j) <= xi < Ldint(j+1)); the point lies at that cell's
+// mid radius, a uniform azimuth (rantheta) and a Gaussian height of the
+// cell's scale height (gasdev, its second deviate kept for the packet's next
+// draw); a point in an empty cell is drawn again at the same radius (as is
+// one beyond the outer wall, whose cell the reference would read past the
+// grid).  The cell's dust absorbs the photon (dbig, nabs) and re-emits it
+// at the cell's temperature, peeled with origin 2: emit loads the placement
+// and interact does the rest (i_orig = 0 marks the photon).  After
+// kAccTries draws without a cell holding dust the packet is dropped as an
+// error (sp = -1).
+template <class T>
+__device__ __forceinline__ AccStart acc_place(const DeviceModel& m, Rng& rng) {
+  AccStart a{};
+  a.sp = -1;
+  const double xi = (double)uniform<T>(rng);
+  int j = locate<double>(m.acc_ldint, m.nrg - 1, xi);   // Ldint(1) = 0 and Ldint(nrg-1) = 1: 1 <= j <= nrg-2
+  j = j < 1 ? 1 : (j > m.nrg - 1 ? m.nrg - 1 : j);
+  const G rp = 0.5 * (m.rarr[j - 1] + m.rarr[j]);
+  const G hd = m.acc_hdisk[j - 1];
+  T gset = (T)0;
+  bool iset = false;
+  for (int tries = 0; tries < kAccTries; ++tries) {
+    T x, y, r2;
+    do {
+      x = uniform<T>(rng); x = x + x - (T)1;
+      y = uniform<T>(rng); y = y + y - (T)1;
+      r2 = x * x + y * y;
+    } while (r2 > (T)1);
+    const T r = RT<T>::sqrt_(r2);
+    const T sp = y / r, cp = x / r;
+    T g;
+    if (!iset) {
+      T v1, v2, rsq;
+      do {
+        v1 = (T)2 * uniform<T>(rng) - (T)1;
+        v2 = (T)2 * uniform<T>(rng) - (T)1;
+        rsq = v1 * v1 + v2 * v2;
+      } while (!(rsq < (T)1 && rsq > (T)0));
+      const T fac = RT<T>::sqrt_((T)-2 * RT<T>::log_acc(rsq) / rsq);
+      gset = v1 * fac;
+      g = v2 * fac;
+      iset = true;
+    } else {
+      g = gset;
+      iset = false;
+    }
+    a.xp = rp * (G)cp;
+    a.yp = rp * (G)sp;
+    a.zp = hd * (G)g;
+    a.rsq = a.xp * a.xp + a.yp * a.yp + a.zp * a.zp;
+    const G rtot = sqrt(a.rsq);
+    a.ir = locate<double>(m.r2arr, m.nrg, a.rsq);
+    if (a.ir < 1 || a.ir > m.nrg - 1) continue;
+    const int it = locate<double>(m.costarr, m.ntg, a.zp / rtot);
+    G lp = atan2(a.yp, a.xp);
+    if (lp < 0.0) lp = lp + kR2p;
+    const int ip = locate<double>(m.phiarr, m.npg, lp);
+    a.it = it < 1 ? 1 : (it > m.ntg - 1 ? m.ntg - 1 : it);
+    a.ip = ip < 1 ? 1 : (ip > m.npg - 1 ? m.npg - 1 : ip);
+    const size_t c = cell_index(m, a.ir, a.it, a.ip);
+    if (cell_rho<T>(m, c) > (T)0) {
+      (void)uniform<T>(rng);   // select_dust_2.f90:134 draw
+      a.sp = m.species[c];
+      break;
+    }
+  }
+  a.n = rng.n;
+  return a;
+}
+
+// k_acc_place: the placement of a run's accretion photons k < acc_to, each
+// on its own packet stream, before the transport kernel starts them.  It is
+// kept out of k_transport, whose occupancy its registers would halve.
+template <class T>
+__global__ void __launch_bounds__(256) k_acc_place(DeviceModel m, uint64_t seed, uint64_t first_packet, AccStart* out) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m.acc_to) return;
+  Rng rng;
+  rng_init(rng, seed, first_packet + k);
+  out[k] = acc_place<T>(m, rng);
+}
+
 // newdisktherm_5_fuv.f90:294-413: one stellar photon (ips = 0, no spots),
 // its direct peel, and the start of its first flight.  With external
 // illumination the last packets of a run are photons of the external
 // field instead (newdisktherm_5_fuv.f90:416-455, initpout.f90:18-57):
 // entering at 0.999999 rmax, directed inward in the local frame, ISRF
 // frequency, not peeled (the reference subtracts the sky).  Both share
-// the transform, opacity, position and cell lookup.
+// the transform, opacity, position and cell lookup.  Disk-accretion
+// photons (newdisktherm_5_fuv.f90:242-283) come first in a run, then the
+// hot-spot photons (:296-313).  An accretion photon, placed by k_acc_place,
+// waits for interact (its absorption where it was emitted): returns 2; 1
+// when the first flight has begun; 0 when the packet was dropped.
 template <class T, bool kAdj = false>
-__device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
+__device__ __forceinline__ int emit(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = L.p;
-  const bool outside = m.nisrf > 0 && L.pid >= m.isrf_from;
-  // hot-spot photons (newdisktherm_5_fuv.f90:296-313) come first in a run
-  const bool hot = m.ispot != 0 && !outside && L.pid < m.hot_to;
+  const bool acc = L.pid < m.acc_to;
+  const bool outside = !acc && m.nisrf > 0 && L.pid >= m.isrf_from;
+  const bool hot = m.ispot != 0 && !outside && !acc && L.pid < m.hot_to;
   bool spot = hot;
   p.on_wall = false;
   p.sip = (T)1; p.sqp = (T)0; p.sup = (T)0; p.svp = (T)0;
+  cell_cache_reset(L);
+  L.iscat = 0; L.iint = 0; L.nint = 0;
+  if (acc) {
+    const AccStart a = m.acc_start[L.pid];
+    if (a.sp < 0) {
+      p.nub = (T)0; p.cost = (T)0; p.phi = (T)0;
+      return 0;
+    }
+    p.xp = a.xp; p.yp = a.yp; p.zp = a.zp; p.rsq = a.rsq;
+    p.rtot = sqrt(p.rsq);
+    p.ir = a.ir; p.it = a.it; p.ip = a.ip;
+    L.idust2 = a.sp;
+    rng_seek(C.rng, a.n);
+    p.i_orig = 0;
+    p.iflag = 0;
+    p.exitflag = 0;
+    return 2;
+  }
   if (!outside) {
     // initpspot.f90:27-125: a point on the star; a hot-spot photon redraws
     // until the point lies in a spot, and with fspot < 1 a stellar photon
@@ -1208,11 +1326,11 @@ __device__ __forceinline__ void emit(Ctx<T>& C, Life<T>& L) {
   // propagate_5_fuv.f90:20-40: the first flight
   cell_cache_reset(L);
   L.xpold = p.xp; L.ypold = p.yp; L.zpold = p.zp;
-  L.iscat = 0; L.iint = 0; L.nint = 0;
   p.iflag = 0; p.exitflag = 0;
   T tau = RT<T>::tau_of(uniform<T>(C.rng));
   L.first = true;
   flight_begin(L, tau);
+  return 1;
 }
 
 // end of a packet: SED for escaped ones, outcome counters, trace row
@@ -1395,9 +1513,10 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
         } else {
           rng_init(C.rng, rp.seed, rp.first_packet + k);
           L.pid = k;
-          emit<T>(C, L);
           budget--;
-          st = L_WALK;
+          const int e = emit<T>(C, L);
+          if (e == 0) end_packet<T>(C, L, -1);
+          else st = (e == 1) ? L_WALK : L_READY;
         }
       }
     }

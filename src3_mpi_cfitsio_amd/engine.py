@@ -107,7 +107,18 @@ class Engine:
         self.set_tdust(m.tdust_initial())
         self.set_isrf(m.isrf_frac, m.isrf)
         self.set_spots(m.spots)
+        self.set_accretion(m.accretion)
         self._check(L.mcrt_zero_tallies(self.h))
+
+    def set_accretion(self, accretion=None):
+        """Disk-accretion photons (dict as model.with_accretion makes it:
+        accfrac, ldint, hdisk); None turns them off."""
+        if not accretion or not accretion["accfrac"]:
+            self._check(self.lib.mcrt_set_accretion(self.h, None, None, 0.0))
+            return
+        ld, hd = _f64(accretion["ldint"]), _f64(accretion["hdisk"])
+        d = C.c_double
+        self._check(self.lib.mcrt_set_accretion(self.h, ptr(ld, d), ptr(hd, d), float(accretion["accfrac"])))
 
     def set_spots(self, spots=None):
         """Stellar hot spots (dict as model.with_spots makes it); None turns them off."""

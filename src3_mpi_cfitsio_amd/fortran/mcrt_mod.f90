@@ -83,6 +83,13 @@ module mcrt_mod
       integer(c_int32_t), value :: nspot
       real(c_double), value :: fspot, spotlat, spotlon, tshock, accfrac2
     end function
+    ! Ldint and hdisk of grid_mod (nrg-1 each), accfrac of tts_mod
+    integer(c_int) function mcrt_set_accretion(ctx, ldint, hdisk, accfrac) bind(C, name='mcrt_set_accretion')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: ldint(*), hdisk(*)
+      real(c_double), value :: accfrac
+    end function
     integer(c_int) function mcrt_set_source(ctx, rstar, tstar, rmin) bind(C, name='mcrt_set_source')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -28,7 +28,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Dict, List, Optional
+from typing import Any, Dict, List, Optional
 
 import numpy as np
 
@@ -114,6 +114,9 @@ class Model:
     # stellar hot spots (initpspot.f90, spotset.f90): nspot, fspot,
     # spotlat/spotlon (deg), tshock (K/11605), accfrac2 (hot-spot photons)
     spots: Optional[Dict[str, float]] = None
+    # disk-accretion photons (initpacc_4.f90): accfrac (the fraction of the
+    # packets), ldint and hdisk (grid_mod's Ldint and hdisk, nrg - 1 each)
+    accretion: Optional[Dict[str, Any]] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def ncell(self) -> int:
@@ -492,6 +495,45 @@ def with_spots(m: Model, nspot: int = 1, fspot: float = 0.01, spotlat: float = 6
     return dataclasses.replace(m, spots=dict(nspot=int(nspot), fspot=float(fspot), spotlat=float(spotlat),
                                              spotlon=float(spotlon), tshock=float(tshock_k) / TK,
                                              accfrac2=float(accfrac2)))
+
+
+def accretion_tables(m: Model, rin_au: float = 0.1, rout_au: float = 1000.0, h0_au: float = 0.03,
+                     flare: float = 1.25) -> Dict[str, np.ndarray]:
+    """Ldint and hdisk for the synthetic flared disk of build(): per radial
+    cell j (mid radius r_j = ravearr(j), gridset_5_model13new.f90:525) the
+    scale height h0 r^flare (AU, in R*) and the luminosity of a steady
+    accretion disk, dL_j ~ r_j^-2 (1 - sqrt(R*/r_j)) dr_j over the cells
+    with rin <= r_j <= rout.  initpacc_4.f90 places a photon in the cell j
+    with Ldint(j) <= xi < Ldint(j+1), so Ldint(j) here is the sum over the
+    cells before j (normalized to 1 in the last cell, whose own luminosity
+    is left out): each cell emits its own share.  diskprofile_model11.f90
+    sums through cell j instead, which moves every share one cell inward,
+    the innermost into the dust-free cell inside the disk, where no photon
+    can be placed.  The tables stand in for diskprofile's alpha-disk
+    vertical structure (Bell-Lin opacities), which is part of the model
+    set-up, not of the packet path: any Ldint (0 first, 1 last,
+    non-decreasing) and hdisk >= 0 may be given instead."""
+    rarr = np.asarray(m.rarr, dtype=np.float64)
+    autors = AU / (m.rstar * RSUN)
+    rave = 0.5 * (rarr[:-1] + rarr[1:])
+    r_au = rave / autors
+    hdisk = h0_au * r_au ** flare * autors
+    dr = rarr[1:] - rarr[:-1]
+    inside = (r_au >= rin_au) & (r_au <= rout_au)
+    dl = np.where(inside, rave ** -2 * (1.0 - np.sqrt(1.0 / rave)) * dr, 0.0)
+    before = np.concatenate(([0.0], np.cumsum(dl[:-1])))
+    if not before[-1] > 0.0:
+        raise ValueError("accretion_tables: no radial cell of the grid but the last lies in the disk")
+    ldint = before / before[-1]
+    return dict(ldint=ldint, hdisk=hdisk)
+
+
+def with_accretion(m: Model, accfrac: float, tables: Optional[Dict[str, np.ndarray]] = None, **kw) -> Model:
+    """The model with accfrac of every run's packets emitted by disk
+    accretion, from tables (ldint, hdisk) or accretion_tables(m, **kw)."""
+    t = tables if tables is not None else accretion_tables(m, **kw)
+    return dataclasses.replace(m, accretion=dict(accfrac=float(accfrac), ldint=np.asarray(t["ldint"], np.float64),
+                                                 hdisk=np.asarray(t["hdisk"], np.float64)))
 
 
 def with_isrf(m: Model, isrf_frac: float, nisrf: int = 200) -> Model:
