@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--accfrac", type=float, default=0.0,
+                    help="fraction of each step's packets emitted by disk accretion (c1/c0 workloads)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes on the GPU")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "setemiss"],
                     help="c1: the headline single-model metric; c2: the 512-model batch; c3: the image gradient; "
@@ -313,6 +315,8 @@ def main():
 
     t0 = time.perf_counter()
     m = M.config(args.config)
+    if args.accfrac > 0.0:
+        m = M.with_accretion(m, args.accfrac)
     log(f"model {args.config}: {m.nrg-1}x{m.ntg-1}x{m.npg-1} cells built in {time.perf_counter()-t0:.1f}s")
     mass = m.rho * m.vcell
     eng = Engine(m)
@@ -422,7 +426,7 @@ def main():
                                f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
                                f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
                    "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; FP64 geometry and dtauabs)",
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
                      "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
