@@ -1250,6 +1250,7 @@ __device__ __forceinline__ int emit(Ctx<T>& C, Life<T>& L) {
     p.ir = a.ir; p.it = a.it; p.ip = a.ip;
     L.idust2 = a.sp;
     rng_seek(C.rng, a.n);
+    p.nub = (T)m.nudust[0];   // a table frequency until the re-emission: unpark sets p.op from it
     p.i_orig = 0;
     p.iflag = 0;
     p.exitflag = 0;
