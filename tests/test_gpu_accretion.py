@@ -46,6 +46,28 @@ def test_accretion_packets_match_oracle(accfrac, spots, isrf, oracle_lib):
         assert abs(a.sum() - b.sum()) <= 0.02 * max(abs(b.sum()), 1e-30), name
 
 
+def test_parked_accretion_photons_match_oracle(oracle_lib):
+    """One event per lane and round: a placed accretion photon waits for its
+    absorption across a round boundary (parked with i_orig = 0)."""
+    from src3_mpi_cfitsio_amd.engine import Engine
+    m = M.with_accretion(M.small("accpark", nr=16, nt=10, nphi=2), 0.5)
+    n = 1000
+    print(f"parked accretion photons n={n}, 256 lanes, 1 event per round", flush=True)
+    with Engine(m) as e:
+        e.set_launch(lanes=256, queue_per_lane=8, events_per_round=1)
+        e.zero_tallies()
+        e.enable_trace(n)
+        e.run_packets(n, seed=44)
+        gt = e.get_trace(n)
+        tg = e.tallies()
+    o = oracle_lib.run_packets(m, n, seed=44, trace=True)
+    same = np.all(rel_err(o["trace"], gt) <= 1e-9, axis=1)
+    print(f"agreement {same.mean():.4f}", flush=True)
+    assert same.mean() >= 0.99
+    assert tg["counters"]["errors"] == 0
+    assert abs(tg["nabs"].sum() - o["nabs"].sum()) <= 0.02 * o["nabs"].sum()
+
+
 def test_mixed_precision_accretion_statistics(oracle_lib):
     from src3_mpi_cfitsio_amd.engine import Engine, MCRT_MIXED
     m = M.small("accmix", nr=20, nt=12, nphi=4)
