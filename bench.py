@@ -9,15 +9,21 @@ A step is one Lucy iteration: zero the tallies, run the step's packets
 256x256 images in 3 bands and SEDs, FP64 absorbed-energy tally), then the
 FP64 temperature inversion that the next step re-emits with.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--packets N] [--budget S]
 
 The default step is the workload's full Lucy iteration (10^8 packets on the
 200x180x64 model). At that size the bulk of the packets sets the step time;
 the serial chains of the few longest packets (up to ~10^6 interactions, a
 few seconds) overlap with it except at the very end.  The FP64 comparison
-runs the last timed step again in FP64, from the same starting dust
+runs the first timed step again in FP64, from the same starting dust
 temperature on the same RNG counters, timed the same way; the SED and
 temperature errors compare the two steps' results.
+
+The run keeps to a wall-clock budget (--budget, default 480 s): W (at most
+4) short warm-up Lucy iterations heat the dust, the first timed step, its
+FP64 twin and the CPU leg always run, and further timed steps (up to K)
+follow while the next one still fits.  The JSON line gives the steps and
+warm-ups actually run next to the requested ones.
 
 python bench.py --copy-probe BYTES only copies BYTES of device memory with
 torch (one copy kernel): the known-size kernel that scales rocprofv3's
@@ -97,16 +103,20 @@ def t_error(ta, tb, mass):
 
 
 def cpu_baseline(m, tdust, seconds, threads):
+    """Packets/s of the FP64 C oracle on `threads` host threads: runs of
+    growing size until one lasts at least 60% of `seconds` (the work per
+    packet is heavy-tailed, so a short probe underestimates it)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    probe = 2000
-    t0 = time.perf_counter()
-    oracle_py.run_packets(m, probe, seed=777, first_packet=10 ** 12, tdust=tdust, nthreads=threads)
-    dt = time.perf_counter() - t0
-    n = int(max(probe, min(5_000_000, probe * seconds / max(dt, 1e-3))))
-    t0 = time.perf_counter()
-    oracle_py.run_packets(m, n, seed=778, first_packet=2 * 10 ** 12, tdust=tdust, nthreads=threads)
-    dt = time.perf_counter() - t0
+    n, first = 2000, 10 ** 12
+    for _ in range(6):
+        t0 = time.perf_counter()
+        oracle_py.run_packets(m, n, seed=777, first_packet=first, tdust=tdust, nthreads=threads)
+        dt = time.perf_counter() - t0
+        first += n
+        if dt >= 0.6 * seconds or n >= 20_000_000:
+            break
+        n = int(min(20_000_000, max(2 * n, n * 1.2 * seconds / max(dt, 1e-3))))
     return n / dt, n, dt
 
 
@@ -270,14 +280,190 @@ def run_setemiss_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_leg(m, tdust, seconds, threads_list, config):
+    """The FP64 C oracle on the host cores at each thread count of
+    threads_list (one packet range per thread, as the reference splits its
+    packets over MPI ranks), about `seconds` of work each."""
+    runs = []
+    for th in threads_list:
+        rate, n, dt = cpu_baseline(m, tdust, seconds, th)
+        runs.append(dict(threads=th, value=rate, packets=n, seconds=dt))
+        log(f"cpu baseline: {rate:.4g} packets/s on {th} threads ({n} packets in {dt:.1f}s)")
+    best = max(runs, key=lambda r: r["value"])
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"value": best["value"], "unit": "packets/s", "cores": best["threads"], "kind": "port",
+            "sample": f"FP64 C oracle on the same {config} model and dust temperature, one packet range per "
+                      f"host thread, about {seconds:.0f}s of work per thread count; the reference itself cannot "
+                      f"be built here (its grid module includes mpif.h and the image has no MPI)",
+            "runs": runs, "host_cpus": os.cpu_count(), "cpus_available": avail}
+
+
+def run_single_workload(args, T0):
+    """The headline workload: Lucy iterations of the 200x180x64 model at its
+    full 10^8 packets per step, inside a wall-clock budget.
+
+    Order: temperature warm-up (short Lucy iterations), the first timed
+    step, the same step again in FP64 on the same counters and starting
+    temperature, the CPU leg, then further timed steps while the budget
+    allows one more (each estimated at 1.15x the slowest step so far).
+    --steps / --warmup are upper limits; the JSON line reports the steps
+    actually timed and warmed up."""
+    from src3_mpi_cfitsio_amd import model as M
+    from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine
+
+    def elapsed():
+        return time.perf_counter() - T0
+
+    t0 = time.perf_counter()
+    m = M.config(args.config)
+    if args.accfrac > 0.0:
+        m = M.with_accretion(m, args.accfrac)
+    log(f"model {args.config}: {m.nrg-1}x{m.ntg-1}x{m.npg-1} cells built in {time.perf_counter()-t0:.1f}s")
+    mass = m.rho * m.vcell
+    eng = Engine(m)
+
+    # temperature warm-up: short Lucy iterations so the timed steps transport
+    # packets through a heated disk (re-emission at the dust temperature)
+    nw = 2_000_000
+    nwarm = max(1, min(4, args.warmup)) if args.warmup > 0 else 1
+    for k in range(nwarm):
+        eng.zero_tallies()
+        eng.run_packets(nw, seed=1000 + k, precision=MCRT_MIXED, first_packet=0)
+        eng.synchronize()
+        tw = eng.last_run_ms() / 1e3
+        tcur = eng.dusttemp(nw, update_tdust=True)
+        log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s), t={elapsed():.0f}s")
+    N = args.packets or int(m.npackets)
+    log(f"packets per step: {N}, budget {args.budget:.0f}s")
+
+    seed = 20250101
+    step_ms, kern_ms, counters, stats = [], [], [], []
+    first = 0
+
+    def step(t_in):
+        eng.synchronize()
+        ts = time.perf_counter()
+        eng.zero_tallies()
+        eng.run_packets(N, seed=seed, precision=MCRT_MIXED, first_packet=first)
+        tal_s = eng.tallies()                      # post-reduction tallies to the host
+        t_out = eng.dusttemp(N, update_tdust=True)
+        eng.synchronize()
+        dt = (time.perf_counter() - ts) * 1e3
+        km = eng.last_run_ms()
+        step_ms.append(dt)
+        kern_ms.append(km)
+        counters.append(tal_s["counters"])
+        stats.append(eng.run_stats())
+        log(f"timed step {len(step_ms)}: {dt:.0f} ms, kernel {km:.0f} ms, {N/(dt/1e3):.4g} packets/s, "
+            f"segments/packet {tal_s['counters']['segments']/N:.0f}, peel segments/packet "
+            f"{tal_s['counters']['peel_segments']/N:.0f}, errors {tal_s['counters']['errors']:.0f}, t={elapsed():.0f}s")
+        return tal_s, t_out
+
+    # the first timed step, then its FP64 twin on the same counters
+    t_start, first_step = tcur, first
+    tal, tcur = step(t_start)
+    first += N
+    extra = {}
+    if not args.no_fp64:
+        t_mixed = eng.dusttemp(N, mode=1, update_tdust=False)     # per-cell T of the mixed step's tallies
+        log("fp64 comparison: the first timed step again in FP64 (the engine logs each round to stderr)")
+        eng.set_tdust(t_start)
+        verbose_before = os.environ.get("MCRT_VERBOSE")
+        os.environ["MCRT_VERBOSE"] = "1"             # a progress line per round of this long run
+        eng.synchronize()
+        ts = time.perf_counter()
+        eng.zero_tallies()
+        eng.run_packets(N, seed=seed, precision=MCRT_FP64, first_packet=first_step)
+        tal64 = eng.tallies()
+        eng.dusttemp(N, update_tdust=False)
+        eng.synchronize()
+        dt64 = time.perf_counter() - ts
+        if verbose_before is None:
+            del os.environ["MCRT_VERBOSE"]
+        else:
+            os.environ["MCRT_VERBOSE"] = verbose_before
+        t64 = eng.dusttemp(N, mode=1, update_tdust=False)
+        log(f"fp64 comparison run: {N} packets in {dt64:.2f}s ({N/dt64:.4g} packets/s); "
+            f"mixed step {step_ms[0]/1e3:.2f}s, t={elapsed():.0f}s")
+        sed_e = sed_error(tal, tal64)
+        t_mean, t_p99 = t_error(t_mixed, t64, mass)
+        extra = dict(fp64_value=N / dt64, mixed_over_fp64=dt64 / (step_ms[0] / 1e3),
+                     sed_err_vs_fp64=sed_e, t_err_vs_fp64_mass_weighted=t_mean, t_err_vs_fp64_p99=t_p99,
+                     fp64_comparison_packets=N)
+        del tal64
+        eng.set_tdust(tcur)                          # continue the mixed Lucy sequence
+
+    cpu = None
+    if not args.no_cpu:
+        cap = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1))
+        cpu = cpu_leg(m, t_start, args.cpu_seconds, sorted({cap, 31}), args.config)
+
+    # further timed steps while one more fits in the budget
+    while len(step_ms) < max(1, args.steps):
+        est = 1.15 * max(step_ms) / 1e3
+        if elapsed() + est > args.budget:
+            log(f"budget: {elapsed():.0f}s used, next step ~{est:.0f}s would pass {args.budget:.0f}s; "
+                f"{len(step_ms)} of {args.steps} requested steps timed")
+            break
+        tal, tcur = step(tcur)
+        first += N
+    eng.close()
+
+    ms = float(np.mean(step_ms))
+    value = N / (ms / 1e3)
+    kmean = float(np.mean(kern_ms))
+    c = {k: float(np.mean([cc[k] for cc in counters])) for k in counters[0]}
+    st = {k: float(np.mean([ss[k] for ss in stats])) for k in stats[0]}
+    b_tr, b_pe = kernel_bytes(c, st, st["record_bytes"], st["lane_bytes"])
+    # the dominant kernel's roofline: algorithmic bytes over its summed
+    # HIP-event time (every launch of the step bracketed on the run stream)
+    if st["peel_ms"] >= st["transport_ms"]:
+        kname, kms, kB, nl = "k_peel<float>", st["peel_ms"], b_pe, st["peel_launches"]
+    else:
+        kname, kms, kB, nl = "k_transport<float>", st["transport_ms"], b_tr, st["rounds"]
+    achieved = kB / (kms / 1e3) / 1e9
+    log(f"kernels per step: transport {st['transport_ms']:.1f} ms ({b_tr/1e9:.2f} GB algorithmic), "
+        f"peel {st['peel_ms']:.1f} ms ({b_pe/1e9:.2f} GB), rounds {st['rounds']:.0f}")
+    out = {
+        "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": len(step_ms),
+        "warmup": nwarm, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{args.config}: 3-D polarized clumpy disk+Ulrich envelope, "
+                               f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
+                               f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
+                   "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; "
+                                                       "FP64 geometry and dtauabs)",
+                   "steps_requested": args.steps, "warmup_requested": args.warmup,
+                   "warmup_kind": f"{nwarm} Lucy iterations of {nw} packets (heats the dust; loads the kernels)",
+                   "budget_s": args.budget, "wall_s": round(elapsed(), 1),
+                   "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {})},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
+                     "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
+                     "kernel_ms_per_launch": kms / max(nl, 1.0),
+                     "algorithmic_bytes_per_launch": kB / max(nl, 1.0),
+                     "transport_ms_per_step": st["transport_ms"], "peel_ms_per_step": st["peel_ms"],
+                     "run_ms_per_step": kmean},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
 def main():
+    T0 = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--packets", type=int, default=0, help="packets per step (default: auto)")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--packets", type=int, default=0, help="packets per step (default: the workload's own)")
+    ap.add_argument("--budget", type=float, default=480.0,
+                    help="wall-clock seconds the c1 run may take; further timed steps stop short of it")
     ap.add_argument("--config", default="c1")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--accfrac", type=float, default=0.0,
@@ -309,135 +495,7 @@ def main():
         return run_adjoint_workload(args)
     if args.workload == "setemiss":
         return run_setemiss_workload(args)
-
-    from src3_mpi_cfitsio_amd import model as M
-    from src3_mpi_cfitsio_amd.engine import MCRT_FP64, MCRT_MIXED, Engine
-
-    t0 = time.perf_counter()
-    m = M.config(args.config)
-    if args.accfrac > 0.0:
-        m = M.with_accretion(m, args.accfrac)
-    log(f"model {args.config}: {m.nrg-1}x{m.ntg-1}x{m.npg-1} cells built in {time.perf_counter()-t0:.1f}s")
-    mass = m.rho * m.vcell
-    eng = Engine(m)
-
-    # temperature warm-up: short Lucy iterations so the timed steps transport
-    # packets through a heated disk (re-emission at the dust temperature)
-    nw = 2_000_000
-    tcur = m.tdust_initial()
-    for k in range(4):
-        eng.zero_tallies()
-        eng.run_packets(nw, seed=1000 + k, precision=MCRT_MIXED, first_packet=0)
-        eng.synchronize()
-        tw = eng.last_run_ms() / 1e3
-        tcur = eng.dusttemp(nw, update_tdust=True)
-        log(f"warm-up Lucy iteration {k}: {nw} packets in {tw:.2f}s ({nw/tw:.3g} packets/s)")
-    # the workload's packets per Lucy iteration
-    N = args.packets or int(m.npackets)
-    log(f"packets per step: {N}")
-
-    seed = 20250101
-    step_ms, kern_ms, counters, stats = [], [], [], []
-    first = 0
-    for s in range(args.warmup + args.steps):
-        t_start, first_step = tcur, first         # the step's starting temperature and counters
-        eng.synchronize()
-        t0 = time.perf_counter()
-        eng.zero_tallies()
-        eng.run_packets(N, seed=seed, precision=MCRT_MIXED, first_packet=first)
-        tal = eng.tallies()                        # post-reduction tallies to the host
-        tcur = eng.dusttemp(N, update_tdust=True)
-        eng.synchronize()
-        dt = (time.perf_counter() - t0) * 1e3
-        km = eng.last_run_ms()
-        first += N
-        if s >= args.warmup:
-            step_ms.append(dt)
-            kern_ms.append(km)
-            counters.append(tal["counters"])
-            stats.append(eng.run_stats())
-        log(f"step {s} ({'warm-up' if s < args.warmup else 'timed'}): {dt:.0f} ms, kernel {km:.0f} ms, "
-            f"{N/(dt/1e3):.4g} packets/s, segments/packet {tal['counters']['segments']/N:.0f}, "
-            f"peel segments/packet {tal['counters']['peel_segments']/N:.0f}, errors {tal['counters']['errors']:.0f}")
-    ms = float(np.mean(step_ms))
-    value = N / (ms / 1e3)
-    kmean = float(np.mean(kern_ms))
-    c = {k: float(np.mean([cc[k] for cc in counters])) for k in counters[0]}
-    st = {k: float(np.mean([ss[k] for ss in stats])) for k in stats[0]}
-    b_tr, b_pe = kernel_bytes(c, st, st["record_bytes"], st["lane_bytes"])
-    # the dominant kernel's roofline: algorithmic bytes over its summed
-    # HIP-event time (every launch of the step bracketed on the run stream)
-    if st["peel_ms"] >= st["transport_ms"]:
-        kname, kms, kB, nl = "k_peel<float>", st["peel_ms"], b_pe, st["peel_launches"]
-    else:
-        kname, kms, kB, nl = "k_transport<float>", st["transport_ms"], b_tr, st["rounds"]
-    achieved = kB / (kms / 1e3) / 1e9
-    log(f"kernels per step: transport {st['transport_ms']:.1f} ms ({b_tr/1e9:.2f} GB algorithmic), "
-        f"peel {st['peel_ms']:.1f} ms ({b_pe/1e9:.2f} GB), rounds {st['rounds']:.0f}")
-
-    # FP64 path and the mixed path's error against it: the last timed step
-    # again in FP64, from the same temperature on the same counters, timed
-    # the same way (zero, run, read back, invert)
-    tdust_state = t_start
-    extra = {}
-    if not args.no_fp64:
-        t_mixed = eng.dusttemp(N, mode=1, update_tdust=False)     # per-cell T of the mixed step's tallies
-        log("fp64 comparison: the timed step again in FP64 (the engine logs each round to stderr)")
-        eng.set_tdust(t_start)
-        verbose_before = os.environ.get("MCRT_VERBOSE")
-        os.environ["MCRT_VERBOSE"] = "1"             # a progress line per round of this long run
-        eng.synchronize()
-        t0 = time.perf_counter()
-        eng.zero_tallies()
-        eng.run_packets(N, seed=seed, precision=MCRT_FP64, first_packet=first_step)
-        tal64 = eng.tallies()
-        eng.dusttemp(N, update_tdust=False)
-        eng.synchronize()
-        dt64 = time.perf_counter() - t0
-        if verbose_before is None:
-            del os.environ["MCRT_VERBOSE"]
-        else:
-            os.environ["MCRT_VERBOSE"] = verbose_before
-        t64 = eng.dusttemp(N, mode=1, update_tdust=False)
-        log(f"fp64 comparison run: {N} packets in {dt64:.2f}s ({N/dt64:.4g} packets/s); "
-            f"mixed step {step_ms[-1]/1e3:.2f}s")
-        sed_e = sed_error(tal, tal64)
-        t_mean, t_p99 = t_error(t_mixed, t64, mass)
-        fp64_rate = N / dt64
-        extra = dict(fp64_value=fp64_rate, mixed_over_fp64=dt64 / (step_ms[-1] / 1e3),
-                     sed_err_vs_fp64=sed_e, t_err_vs_fp64_mass_weighted=t_mean, t_err_vs_fp64_p99=t_p99,
-                     fp64_comparison_packets=N)
-
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        rate, n, dt = cpu_baseline(m, tdust_state, args.cpu_seconds, threads)
-        cpu = {"value": rate, "unit": "packets/s", "cores": threads, "kind": "port",
-               "sample": f"FP64 C oracle, same {args.config} model and dust temperature, {n} packets "
-                         f"in {dt:.1f}s on {threads} host threads (one packet range per thread)"}
-        log(f"cpu baseline: {rate:.4g} packets/s on {threads} threads")
-    eng.close()
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "packets/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"{args.config}: 3-D polarized clumpy disk+Ulrich envelope, "
-                               f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
-                               f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
-                   "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; FP64 geometry and dtauabs)",
-                   "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {})},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
-                     "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
-                     "kernel_ms_per_launch": kms / max(nl, 1.0),
-                     "algorithmic_bytes_per_launch": kB / max(nl, 1.0),
-                     "transport_ms_per_step": st["transport_ms"], "peel_ms_per_step": st["peel_ms"],
-                     "run_ms_per_step": kmean},
-        "cpu_baseline": cpu,
-    }
-    out.update(extra)
-    print(json.dumps(out), flush=True)
+    return run_single_workload(args, T0)
 
 
 if __name__ == "__main__":
