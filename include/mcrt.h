@@ -89,6 +89,26 @@ int mcrt_setemiss(mcrt_ctx *ctx, int32_t nopac, int32_t nlam, const int32_t *nla
                   const int32_t *smooth, double *kapint, double *kappap, double *kappar,
                   int64_t *nfail, int64_t *nevals, float *ms);
 
+/* setemiss without allocation (no counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as mcrt_setemiss, on tables already in device memory:
+ * nudust, kappad, adust, Tint, nuint and the outputs kapint, kappap, kappar
+ * are device pointers (layouts as in mcrt_setemiss); nlambda, smooth (may be
+ * NULL), nfail and nevals (may be NULL) are host memory.  Calls no hipMalloc
+ * or hipFree: the interval integrals go through a fixed 16 MiB device
+ * scratch array per GPU, species after species in chunks that fit.  Limits:
+ * nopac <= 1024, nT <= 4096, nnu * nT <= 1048576 (2 nnu nT doubles of one
+ * species); larger sizes are refused with MCRT_E_ARG.  Every call on a GPU
+ * shares that GPU's scratch, so calls that overlap on one GPU (two host
+ * threads, or two contexts on different streams) race; calls in sequence do
+ * not.  Synchronizes the context stream. */
+int mcrt_setemiss_noalloc(mcrt_ctx *ctx, int32_t nopac, int32_t nlam, const int32_t *nlambda,
+                          const double *nudust, const double *kappad, const double *adust,
+                          int32_t nT, const double *Tint, int32_t nnu, const double *nuint,
+                          const int32_t *smooth, double *kapint, double *kappap, double *kappar,
+                          int64_t *nfail, int64_t *nevals);
+
 /* Cell arrays of nmodels synthetic models in one launch: gridset's cell
  * loop (gridset_5_model13new.f90:1105-1180) with cell volumes, the flared
  * disk of the synthetic workloads normalized to its mass, the analytic

@@ -31,6 +31,8 @@ SIGNATURES = {
     "mcrt_set_dust": (C.c_int, [_vp, C.c_int32, C.c_int32] + [_dp] * 6 + [C.c_int32, C.c_int32] + [_dp] * 5),
     "mcrt_setemiss": (C.c_int, [_vp, C.c_int32, C.c_int32, _ip, _dp, _dp, _dp, C.c_int32, _dp, C.c_int32, _dp, _ip,
                                 _dp, _dp, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), _fp]),
+    "mcrt_setemiss_noalloc": (C.c_int, [_vp, C.c_int32, C.c_int32, _ip, _vp, _vp, _vp, C.c_int32, _vp, C.c_int32,
+                                        _vp, _ip, _vp, _vp, _vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mcrt_build_models": (C.c_int, [_vp, C.c_int32, _ip, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _u8p, _dp,
                                     C.POINTER(C.c_int64), _fp]),
     "mcrt_set_isrf": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, C.c_double]),

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -209,6 +210,44 @@ int zero_tallies(mcrt_ctx* c) {
   return MCRT_OK;
 }
 
+// ---- scratch of mcrt_setemiss_noalloc ------------------------------------
+// A file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends.  On the host its name is only a stand-in, so
+// host code uses the address hipGetSymbolAddress returns on each device
+// (g_scratch[device], set up once per device by mcrt_create).  Layout: two
+// 64-bit counters at 0, the per-species table lengths at kScrNlam and the
+// smoothing flags at kScrSmooth (kScratchSpecies int32 each), then the
+// interval integrals of one chunk of species at kScrSeg.
+constexpr int kScratchSpecies = 1024;
+constexpr size_t kScrNlam = 256, kScrSmooth = kScrNlam + 4u * kScratchSpecies;
+constexpr size_t kScrSeg = 16384;
+constexpr size_t kScratchSegBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScrSeg + kScratchSegBytes];
+
+constexpr int kMaxScratchDevices = 256;
+unsigned char* g_scratch[kMaxScratchDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+// With `device` current: its copy of the array (once per device).
+hipError_t scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return hipSuccess;   // the entry point then refuses
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch[device] = (unsigned char*)ptr;
+  return hipSuccess;
+}
+
+bool device_pointer(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
 bool ready(mcrt_ctx* c) {
   return c->has_grid && c->has_medium && c->has_dust && c->has_source && c->has_observer && c->has_tdust;
 }
@@ -226,6 +265,7 @@ int mcrt_create(int device, mcrt_ctx** out) {
   mcrt_ctx* c = new mcrt_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) { delete c; return MCRT_E_HIP; }
+  if (scratch_init(device) != hipSuccess) { delete c; return MCRT_E_HIP; }
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCRT_E_HIP; }
   c->stream = c->own_stream;
   hipDeviceProp_t prop;
@@ -468,6 +508,61 @@ int mcrt_setemiss(mcrt_ctx* c, int32_t nopac, int32_t nlam, const int32_t* nlamb
   if (nfail) *nfail = (int64_t)counts[0];
   if (nevals) *nevals = (int64_t)counts[1];
   if (ms) *ms = t_ms;
+  return MCRT_OK;
+}
+
+int mcrt_setemiss_noalloc(mcrt_ctx* c, int32_t nopac, int32_t nlam, const int32_t* nlambda, const double* nudust,
+                          const double* kappad, const double* adust, int32_t nT, const double* Tint, int32_t nnu,
+                          const double* nuint, const int32_t* smooth, double* kapint, double* kappap, double* kappar,
+                          int64_t* nfail, int64_t* nevals) {
+  if (!c) return MCRT_E_ARG;
+  const char* me = "mcrt_setemiss_noalloc";
+  if (nopac < 1 || nlam < 2 || nT < 2 || nnu < 2) return fail(c, MCRT_E_ARG, std::string(me) + ": nopac >= 1, nlam, nT, nnu >= 2");
+  if (nopac > kScratchSpecies)
+    return fail(c, MCRT_E_ARG, std::string(me) + ": nopac must be <= 1024 (the scratch holds 1024 species' flags)");
+  if (nT > 4096) return fail(c, MCRT_E_ARG, std::string(me) + ": nT must be <= 4096 (kappap smoothing in LDS)");
+  const size_t per_species = 2u * (size_t)nnu * (size_t)nT * sizeof(double);
+  if (per_species > kScratchSegBytes)
+    return fail(c, MCRT_E_ARG, std::string(me) + ": 2*nnu*nT doubles of one species exceed the 16 MiB scratch "
+                               "(nnu*nT must be <= 1048576)");
+  const void* dev[] = {nudust, kappad, adust, Tint, nuint, kapint, kappap, kappar};
+  for (const void* q : dev)
+    if (!device_pointer(q)) return fail(c, MCRT_E_ARG, std::string(me) + ": tables must be device pointers");
+  HIPCHECK(c, hipSetDevice(c->device));
+  unsigned char* s = (c->device < kMaxScratchDevices) ? g_scratch[c->device] : nullptr;
+  if (!s) return fail(c, MCRT_E_STATE, std::string(me) + ": no scratch on this device");
+  int32_t nl[kScratchSpecies], sm[kScratchSpecies];
+  for (int id = 0; id < nopac; ++id) {
+    nl[id] = nlambda ? nlambda[id] : nlam;
+    sm[id] = (smooth && smooth[id]) ? 1 : 0;
+    if (nl[id] < 2 || nl[id] > nlam) return fail(c, MCRT_E_ARG, std::string(me) + ": nlambda must be 2..nlam");
+    if (sm[id] && nT < 21) return fail(c, MCRT_E_ARG, std::string(me) + ": smoothed species need nT >= 21");
+  }
+  double th[4096];
+  HIPCHECK(c, hipMemcpyAsync(th, Tint, (size_t)nT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nT; ++i)
+    if (!(th[i] > 0.0)) return fail(c, MCRT_E_ARG, std::string(me) + ": Tint must be > 0");
+  int* d_nl = (int*)(s + kScrNlam);
+  int* d_sm = (int*)(s + kScrSmooth);
+  unsigned long long* cnt = (unsigned long long*)s;
+  double* seg = (double*)(s + kScrSeg);
+  HIPCHECK(c, hipMemcpyAsync(d_nl, nl, (size_t)nopac * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, hipMemcpyAsync(d_sm, sm, (size_t)nopac * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  HIPCHECK(c, launch_fill_u64(cnt, 2, 0ull, c->stream));
+  const int chunk = (int)std::min<size_t>((size_t)nopac, kScratchSegBytes / per_species);
+  for (int id0 = 0; id0 < nopac; id0 += chunk) {
+    const int n = std::min(chunk, nopac - id0);
+    const size_t t0 = (size_t)id0 * nlam, k0 = (size_t)id0 * nnu * nT, r0 = (size_t)id0 * nT;
+    EmissArgs a{n, nlam, nT, nnu, d_nl + id0, nudust + t0, kappad + t0, adust + t0, Tint, nuint, d_sm + id0, seg,
+                kapint + k0, kappap + r0, kappar + r0, cnt, cnt + 1};
+    HIPCHECK(c, launch_setemiss(a, c->stream));
+  }
+  unsigned long long counts[2] = {0, 0};
+  HIPCHECK(c, hipMemcpyAsync(counts, cnt, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (nfail) *nfail = (int64_t)counts[0];
+  if (nevals) *nevals = (int64_t)counts[1];
   return MCRT_OK;
 }
 
