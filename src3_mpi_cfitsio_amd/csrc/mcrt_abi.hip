@@ -885,6 +885,11 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.acc_start = (const AccStart*)p;
   }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
+  uint32_t peel_regroup = 16;
+  if (const char* pr = std::getenv("MCRT_PEEL_REGROUP")) {
+    const int v = std::atoi(pr);
+    if (v >= 1 && v <= 64) peel_regroup = (uint32_t)v;
+  }
   double st[MCRT_NSTATS] = {0};
   const char* venv = std::getenv("MCRT_VERBOSE");
   const bool verbose = venv && venv[0] && venv[0] != '0';
@@ -915,6 +920,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     rp.round = (uint32_t)(round < 0xffffffffll ? round : 0xffffffffll);
     rp.ctl = ctl;
     rp.regroup = (uint32_t)(c->regroup > 0 ? c->regroup : 32);
+    rp.peel_regroup = peel_regroup;
     rp.lds = lds;
     // q_count and live restart every round (a fill kernel, not a memset)
     HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));

@@ -69,6 +69,7 @@ struct RunParams {
   uint32_t events;         // interaction events per lane in this round
   uint32_t round;          // 0 = first round of the run (no saved lanes)
   uint32_t regroup;        // waiting lanes of a wavefront that trigger an interaction / emission / tally pass
+  uint32_t peel_regroup;   // idle lanes of a peel wavefront that trigger taking new items (with their first crossing)
   RunCtl* ctl;
   LdsPlan lds;
 };
@@ -689,8 +690,12 @@ __device__ __forceinline__ int peel_ray_first(Ctx<T>& C, PeelWalk<T>& w, PeelRay
 // One crossing of the ray walk: 0 goes on, 1 left the grid toward the
 // observer (w.p holds the exit point), 2 ended (tau > taumax, the star, or
 // the segment guard).  Whatever axis a lane crosses, it then solves one new
-// wall: one square root and two divisions, computed by every lane alike (a
-// branch per axis would make a wavefront run all three solvers).
+// wall: one square root and two divisions.  The step is written without
+// branches on the axis: every lane loads the two table entries of its wall,
+// forms the coefficients of all three wall kinds and keeps its own by
+// selection, so a wavefront runs one stream of instructions with every
+// walking lane active instead of one masked stream per axis.  The arithmetic
+// of each lane is that of the wall it crosses.
 template <class T>
 __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<T>& R) {
   const DeviceModel& m = C.m;
@@ -712,82 +717,83 @@ __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<
   }
   if (w.tsum > (T)m.taumax) return 2;
   R.t = tn;
-  // the crossing, and the coefficients of the one wall to solve next
+  const bool ax0 = axis == 0, ax1 = axis == 1, ax2 = axis == 2, ax3 = axis == 3, pol = ax1 || ax2;
   const G sc = -R.b;
-  G sqarg = 0.0, n1 = 1.0, d1 = 1.0, n2 = 1.0, d2 = 1.0, A = 0.0, B = 0.0, Cq = 0.0, R2o = 0.0;
-  int kind = 3;          // 0 shell, 1 cone, 2 plane, 3 nothing to solve
-  bool inward = false;
-  if (axis == 0) {
-    p.ir += R.dr;
-    if (p.ir == m.nrg || p.ir == 0) {
-      p.xp = R.ox + tn * w.ux;
-      p.yp = R.oy + tn * w.uy;
-      p.zp = R.oz + tn * w.uz;
-      return p.ir == 0 ? 2 : 1;
-    }
-    const G R2lo = m.r2arr[p.ir - 1];
-    R2o = m.r2arr[p.ir];
-    inward = tn < sc && R2lo > R.h2;
-    sqarg = (inward ? R2lo : R2o) - R.h2;
-    kind = 0;
-  } else if (axis == 3) {
-    p.ip += R.dp;
-    if (p.ip == 0) p.ip = m.npg - 1;
-    if (p.ip == m.npg) p.ip = 1;
-    const int j = (R.dp < 0) ? p.ip - 1 : p.ip;
-    const G a = m.aarr[j], bb = m.barr[j];
-    n1 = -(a * R.ox + bb * R.oy);
-    d1 = a * w.ux + bb * w.uy;
-    kind = (d1 != 0.0) ? 2 : 3;
-  } else {
-    int j;
-    if (axis == 1) { p.it -= 1; R.thi = R.tlo2; R.thi2 = kRayNone; j = p.it - 1; }
-    else { p.it += 1; R.tlo = R.thi2; R.tlo2 = kRayNone; j = p.it; }
-    const G t2 = m.tan2arr[j];
-    if (t2 < 0.0) {                      // equator plane
-      n1 = -R.oz;
-      d1 = w.uz;
-      kind = (d1 != 0.0) ? 2 : 3;
-    } else if (t2 > 0.0) {
-      A = w.ux * w.ux + w.uy * w.uy - t2 * w.uz * w.uz;
-      B = R.ox * w.ux + R.oy * w.uy - t2 * R.oz * w.uz;
-      Cq = R.ox * R.ox + R.oy * R.oy - t2 * R.oz * R.oz;
-      if (fabs(A) > 1.0e-10) { sqarg = B * B - A * Cq; kind = (sqarg > 0.0) ? 1 : 3; }
-      else if (fabs(B) > 0.5e-10) { n1 = -Cq; d1 = 2.0 * B; kind = 2; }
-    }
+  // the crossing
+  const int ir = p.ir + (ax0 ? R.dr : 0);
+  if (ax0 && (ir == m.nrg || ir == 0)) {
+    p.ir = ir;
+    p.xp = R.ox + tn * w.ux;
+    p.yp = R.oy + tn * w.uy;
+    p.zp = R.oz + tn * w.uz;
+    return ir == 0 ? 2 : 1;
   }
-  // the shared solve
+  p.ir = ir;
+  int ip = p.ip + (ax3 ? R.dp : 0);
+  ip = (ip == 0) ? m.npg - 1 : ((ip == m.npg) ? 1 : ip);
+  p.ip = ip;
+  const int it = p.it + (ax1 ? -1 : (ax2 ? 1 : 0));
+  p.it = it;
+  // the wall to solve next and its two table entries: the shells r2arr(ir-1),
+  // r2arr(ir); the phi plane (aarr, barr) ahead; the polar wall tan2arr
+  const int jp = (R.dp < 0) ? ip - 1 : ip;
+  const int jt = ax1 ? it - 1 : it;
+  const double* e0 = ax0 ? m.r2arr + (ir - 1) : (ax3 ? m.aarr + jp : m.tan2arr + jt);
+  const double* e1 = ax0 ? m.r2arr + ir : m.barr + jp;
+  const G v0 = *e0, v1 = *e1;
+  // shell
+  const bool inward = ax0 && tn < sc && v0 > R.h2;
+  // cone (tan2 > 0) or the equator plane (tan2 < 0); tan2 = 0 is a pole
+  const G t2 = v0;
+  const bool cone = pol && t2 > 0.0, eqp = pol && t2 < 0.0;
+  const G A = w.ux * w.ux + w.uy * w.uy - t2 * w.uz * w.uz;
+  const G B = R.ox * w.ux + R.oy * w.uy - t2 * R.oz * w.uz;
+  const G Cq = R.ox * R.ox + R.oy * R.oy - t2 * R.oz * R.oz;
+  const bool quad = cone && fabs(A) > 1.0e-10;
+  const bool lin = cone && !quad && fabs(B) > 0.5e-10;
+  // phi plane A x + B y = 0
+  const G pn = -(v0 * R.ox + v1 * R.oy), pd = v0 * w.ux + v1 * w.uy;
+  const G sqarg = ax0 ? (inward ? v0 : v1) - R.h2 : (quad ? B * B - A * Cq : 0.0);
+  // the shared solve: 1 two quadratic roots, 2 one root n1/d1, 3 none
+  G n1 = ax3 ? pn : (eqp ? -R.oz : (lin ? -Cq : 1.0));
+  G d1 = ax3 ? pd : (eqp ? w.uz : (lin ? 2.0 * B : 1.0));
+  int kind = ((ax3 || eqp) && d1 != 0.0) || lin ? 2 : 3;
+  if (quad && sqarg > 0.0) kind = 1;
   const G sq = sqrt(sqarg > 0.0 ? sqarg : 0.0);
+  const G q = -(B + copysign(sq, B));
+  G n2 = 1.0, d2 = 1.0;
   if (kind == 1) {
-    const G q = -(B + copysign(sq, B));
     n1 = q; d1 = A;
     n2 = Cq; d2 = (q != 0.0) ? q : A;
   }
   const G x1 = n1 / d1, x2 = n2 / d2;
-  if (axis == 0) {
-    G root = inward ? sc - sq : sc + sq;
-    int dr = inward ? -1 : 1;
-    if (inward && !(root > tn)) {        // rounding at a tangent: leave through the outer wall
-      const G d = R2o - R.h2;
-      root = sc + sqrt(d > 0.0 ? d : 0.0);
-      dr = 1;
-    }
-    R.tr = root;
-    R.dr = dr;
-  } else if (axis == 3) {
-    R.tp = (kind == 2 && x1 > tn) ? x1 : kRayNone;
-  } else {
-    G nx = kRayNone, nx2 = kRayNone;
-    if (kind == 1) {
-      const G r1 = fmin(x1, x2), r2 = fmax(x1, x2);
-      if (r1 > tn) { nx = r1; nx2 = r2; }
-      else if (r2 > tn) nx = r2;
-    } else if (kind == 2) {
-      if (x1 > tn) nx = x1;
-    }
-    if (axis == 1) { R.tlo = nx; R.tlo2 = nx2; }
-    else { R.thi = nx; R.thi2 = nx2; }
+  // radial exit of the new cell
+  G root = inward ? sc - sq : sc + sq;
+  int dr = inward ? -1 : 1;
+  if (inward && !(root > tn)) {        // rounding at a tangent: leave through the outer wall
+    const G d = v1 - R.h2;
+    root = sc + sqrt(d > 0.0 ? d : 0.0);
+    dr = 1;
   }
+  // next crossings of the polar wall just solved
+  const G r1 = fmin(x1, x2), r2 = fmax(x1, x2);
+  G nx = kRayNone, nx2 = kRayNone;
+  if (kind == 1) {
+    nx = (r1 > tn) ? r1 : ((r2 > tn) ? r2 : kRayNone);
+    nx2 = (r1 > tn) ? r2 : kRayNone;
+  } else if (kind == 2) {
+    nx = (x1 > tn) ? x1 : kRayNone;
+  }
+  const G otlo2 = R.tlo2, othi2 = R.thi2;
+  R.tr = ax0 ? root : R.tr;
+  R.dr = ax0 ? dr : R.dr;
+  R.tp = ax3 ? ((kind == 2 && x1 > tn) ? x1 : kRayNone) : R.tp;
+  // crossing the lower wall (ax1) keeps its second root as the new upper
+  // wall's next crossing; crossing the upper wall (ax2) likewise downward
+  R.tlo = ax1 ? nx : (ax2 ? othi2 : R.tlo);
+  R.tlo2 = ax1 ? nx2 : (ax2 ? kRayNone : R.tlo2);
+  R.thi = ax2 ? nx : (ax1 ? otlo2 : R.thi);
+  R.thi2 = ax2 ? nx2 : (ax1 ? kRayNone : R.thi2);
   return 0;
 }
 
@@ -1564,9 +1570,14 @@ __device__ __forceinline__ bool peel_weight(const DeviceModel& m, const PeelRec<
 
 // Peel kernel: each lane walks its share of the round's (record, direction)
 // items, item = lane, lane + stride, ...  Every step advances each walking
-// lane by one face crossing; the tally of walks that reached the observer
-// (exponential weight, findangle3, stokespeel, image and SED binning) runs
-// once rp.regroup lanes of the wavefront are waiting for it, or nobody walks.
+// lane by one face crossing.  Two pieces of code run only for gathered
+// lanes, so that they run with most of the wavefront active instead of
+// stalling the walkers on nearly every step: taking the next item with the
+// walk's first crossing (the reference solvers from the record position,
+// several times the cost of a ray step), once rp.peel_regroup lanes are idle;
+// and the tally of walks that reached the observer (exponential weight,
+// findangle3, stokespeel, image and SED binning), once rp.regroup lanes wait
+// for it.  Either also runs when no lane walks.
 template <class T>
 __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, RunParams rp) {
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
@@ -1590,29 +1601,9 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
   PeelRay<T> ray;
   ray.started = false;
   for (;;) {
-    while (st == P_IDLE && item < total) {
-      const uint64_t ri = item / npeel;
-      const int k = (int)(item - ri * npeel);
-      const PeelRec<T>& r = recs[ri];
-      if (r.flags != kSlotHole) {
-        T s0;
-        if (peel_weight<T>(m, r, k, s0)) {
-          Pkt<T> q;
-          rec_to_pkt<T>(r, q);
-          if (peel_begin<T>(m, q, k, w)) {
-            // testgrid re-placements only: one stream per (run, round, walk)
-            rng_init(C.rng, rkey, ((uint64_t)rp.round << 40) + item);
-            ray.started = false;
-            st = P_WALK;
-            break;
-          }
-        }
-      }
-      item += stride;
-    }
     if (st == P_WALK) {
       int r;
-      if constexpr (kRayPeel<T>) r = ray.started ? peel_ray_step<T>(C, w, ray) : peel_ray_first<T>(C, w, ray, C.rng);
+      if constexpr (kRayPeel<T>) r = peel_ray_step<T>(C, w, ray);
       else r = peel_step<T>(C, w, C.rng);
       if (r == 1) st = P_DONE;
       else if (r == 2) { st = P_IDLE; item += stride; }
@@ -1633,6 +1624,33 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
         peel_tally<T>(C, q, s0, k, r.g, r.pl, w.p.xp, w.p.yp, w.p.zp, w.tsum);
         st = P_IDLE;
         item += stride;
+      }
+    }
+    if (more && (walking == 0ull || (unsigned)__popcll(more) >= rp.peel_regroup)) {
+      bool found = false;
+      while (st == P_IDLE && !found && item < total) {
+        const uint64_t ri = item / npeel;
+        const int k = (int)(item - ri * npeel);
+        const PeelRec<T>& r = recs[ri];
+        if (r.flags != kSlotHole) {
+          T s0;
+          if (peel_weight<T>(m, r, k, s0)) {
+            Pkt<T> q;
+            rec_to_pkt<T>(r, q);
+            found = peel_begin<T>(m, q, k, w);
+          }
+        }
+        if (!found) item += stride;
+      }
+      if (found) {
+        // testgrid re-placements only: one stream per (run, round, walk)
+        rng_init(C.rng, rkey, ((uint64_t)rp.round << 40) + item);
+        int r;
+        if constexpr (kRayPeel<T>) r = peel_ray_first<T>(C, w, ray, C.rng);
+        else r = peel_step<T>(C, w, C.rng);
+        if (r == 0) st = P_WALK;
+        else if (r == 1) st = P_DONE;
+        else item += stride;
       }
     }
   }
