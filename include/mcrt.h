@@ -302,6 +302,19 @@ int mcrt_batch_autotune(mcrt_ctx *ctx, int64_t packets_per_model, uint64_t seed,
 int mcrt_dusttemp(mcrt_ctx *ctx, double nphot, int32_t mode, int32_t update_tdust,
                   double *tdave);
 
+/* Convergence test of the Lucy iteration (tfinal_5_test1.f90:334-368):
+ * statistics of the last mcrt_dusttemp call.  Over the upper hemisphere at
+ * ip = 1, every cell whose old temperature (the dust temperature the run
+ * re-emitted with) and new one both exceed 0.1 K and differ gives
+ * x = (T_new - T_old) / T_new; sigma = sqrt(<x^2> - <|x|>^2) over those
+ * ncells (NaN when there are none, as 0/0 in the reference).  sigma_prev is
+ * the sigma of the last mcrt_dusttemp call before it that updated the dust
+ * temperature (the previous Lucy iteration; NaN before the second one), and
+ * converge = 1 when |sigma_prev - sigma| / sigma < 0.25, the reference's
+ * converge = 'yes'.  Any pointer may be NULL. */
+int mcrt_dusttemp_sigma(mcrt_ctx *ctx, double *sigma, double *sigma_prev, int64_t *ncells,
+                        int32_t *converge);
+
 /* Deterministic entries for parity tests (no counterpart in the reference;
  * they expose tauint's face loop and the stokes/stokespeel Mueller update).
  * trace_rays: n rays from pos (3n) along u (3n) in cells ijk (3n) until

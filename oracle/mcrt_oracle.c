@@ -649,6 +649,29 @@ void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double 
                     tdave[CELL(m, ir, it, ipp)] = tdave[CELL(m, ir, it, 1)];
 }
 
+/* tfinal_5_test1.f90:334-368: the convergence statistic.  Over the upper
+ * hemisphere at ip = 1, cells whose old and new temperatures both exceed
+ * the 0.1 K floor and differ give x = (tnew - told)/tnew; out[0] = sum |x|,
+ * out[1] = sum x^2, out[2] = count; returns sig = sqrt(<x^2> - <|x|>^2). */
+double orc_lucy_sigma(const orc_model *m, const double *told, const double *tnew, double out[3])
+{
+    const double floor_t = 0.1 / 11605.0;
+    double a = 0.0, b = 0.0, n = 0.0;
+    for (int it = 1; it <= (m->ntg - 1) / 2; ++it)
+        for (int ir = 1; ir <= m->nrg - 1; ++ir) {
+            size_t c = CELL(m, ir, it, 1);
+            double to = told[c], tn = tnew[c];
+            if (tn != to && tn > floor_t && to > floor_t) {
+                double x = (tn - to) / tn;
+                a = fabs(x) + a;
+                b = x * x + b;
+                n = n + 1.0;
+            }
+        }
+    out[0] = a; out[1] = b; out[2] = n;
+    return sqrt(b / n - (a / n) * (a / n));
+}
+
 /* ======================================================================
  * Packet state and the cell walk
  * ====================================================================== */

@@ -135,6 +135,8 @@ double orc_bbfreq_lecuyer(double teff);   /* bbfreq with the L'Ecuyer stream (ra
 double orc_dusttemp(const orc_model *m, double na, double nphot, int32_t iop, double mcell);
 /* tfinal with maxii=1 and no velocities: Lucy update, midplane mirror, phi replication */
 void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double *tdave);
+/* tfinal_5_test1.f90:334-368 convergence statistic (sums in out[3]) */
+double orc_lucy_sigma(const orc_model *m, const double *told, const double *tnew, double out[3]);
 
 /* Deterministic fixed-direction walk (tauint without interaction tallies):
  * walks from (x,y,z) in cell ijk along u until tau_stop is reached, the packet

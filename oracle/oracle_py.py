@@ -308,6 +308,18 @@ def tfinal(m, dtauabs, nphot):
     return out
 
 
+def lucy_sigma(m, told, tnew):
+    """tfinal's convergence statistic (orc_lucy_sigma): (sigma, ncells)."""
+    L = oracle()
+    om = OracleModel(m)
+    a = np.ascontiguousarray(told, dtype=np.float64)
+    b = np.ascontiguousarray(tnew, dtype=np.float64)
+    out = np.zeros(3)
+    L.orc_lucy_sigma.restype = C.c_double
+    sig = L.orc_lucy_sigma(C.byref(om.s), _p(a, C.c_double), _p(b, C.c_double), _p(out, C.c_double))
+    return float(sig), int(out[2])
+
+
 def dusttemp(m, na, nphot, iop, mcell):
     om = OracleModel(m)
     return oracle().orc_dusttemp(C.byref(om.s), float(na), float(nphot), int(iop), float(mcell))

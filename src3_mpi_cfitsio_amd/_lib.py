@@ -49,6 +49,7 @@ SIGNATURES = {
     "mcrt_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "mcrt_get_tallies": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _fp, _fp, _fp, _dp]),
     "mcrt_dusttemp": (C.c_int, [_vp, C.c_double, C.c_int32, C.c_int32, _dp]),
+    "mcrt_dusttemp_sigma": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_int64), _ip]),
     "mcrt_trace_rays": (C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp, _ip, _u8p, _dp, C.c_int32,
                                   _ip, _ip, _dp, _ip, _dp, _ip, _ip]),
     "mcrt_mueller": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp]),

@@ -121,6 +121,11 @@ hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nph
   return hipGetLastError();
 }
 
+hipError_t launch_lucy_sigma(const DeviceModel& m, const double* told, const double* tnew, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_lucy_sigma, dim3(1), dim3(256), 0, s, m, told, tnew, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* npk, int mode, double* tdave, size_t max_cells,
                                hipStream_t s) {
   const unsigned nm = (unsigned)L.bd.nmodels;

@@ -42,6 +42,11 @@ struct mcrt_ctx {
   const double *acc_ldint = nullptr, *acc_hdisk = nullptr;
   size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
   int64_t trace_n = 0;
+  // Lucy convergence (mcrt_dusttemp_sigma): sums of the last mcrt_dusttemp,
+  // sigma of the last call that updated tdust
+  double lucy_sum[3] = {0.0, 0.0, 0.0};
+  double lucy_sigma = NAN, lucy_sigma_iter = NAN, lucy_sigma_prev = NAN;
+  bool lucy_valid = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool ev_valid = false;
   int cu_count = 256;
@@ -1500,6 +1505,14 @@ int mcrt_dusttemp(mcrt_ctx* c, double nphot, int32_t mode, int32_t update_tdust,
   if (rc) return rc;
   double* d_tdave = (double*)p;
   HIPCHECK(c, launch_tfinal(c->dm, c->dt.dtauabs, nphot, mode, d_tdave, c->stream));
+  // tfinal's convergence statistic against the temperatures this run
+  // re-emitted with (the previous iteration's tdave)
+  if (c->dm.tdust) {
+    void* q;
+    if ((rc = dev_alloc(c, "lucy.sums", 3 * 8, &q))) return rc;
+    HIPCHECK(c, launch_lucy_sigma(c->dm, c->dm.tdust, d_tdave, (double*)q, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(c->lucy_sum, q, 3 * 8, hipMemcpyDeviceToHost, c->stream));
+  }
   if (update_tdust) {
     void* q;
     if ((rc = dev_alloc(c, "tdust", c->ncell * 8, &q))) return rc;
@@ -1509,6 +1522,26 @@ int mcrt_dusttemp(mcrt_ctx* c, double nphot, int32_t mode, int32_t update_tdust,
   }
   if (tdave) HIPCHECK(c, hipMemcpyAsync(tdave, d_tdave, c->ncell * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  // sig = sqrt(<x^2> - <|x|>^2) (tfinal_5_test1.f90:337-345); no counted
+  // cell gives 0/0 as in the reference
+  const double n = c->lucy_sum[2];
+  const double m1 = c->lucy_sum[0] / n, m2 = c->lucy_sum[1] / n;
+  c->lucy_sigma = std::sqrt(m2 - m1 * m1);
+  c->lucy_sigma_prev = c->lucy_sigma_iter;
+  if (update_tdust) c->lucy_sigma_iter = c->lucy_sigma;
+  c->lucy_valid = true;
+  return MCRT_OK;
+}
+
+int mcrt_dusttemp_sigma(mcrt_ctx* c, double* sigma, double* sigma_prev, int64_t* ncells, int32_t* converge) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->lucy_valid) return fail(c, MCRT_E_STATE, "mcrt_dusttemp_sigma: no mcrt_dusttemp call yet");
+  if (sigma) *sigma = c->lucy_sigma;
+  if (sigma_prev) *sigma_prev = c->lucy_sigma_prev;
+  if (ncells) *ncells = (int64_t)c->lucy_sum[2];
+  // tfinal_5_test1.f90:354: converged when sigma changed by less than a
+  // quarter of itself since the previous Lucy iteration (NaN compares false)
+  if (converge) *converge = (std::fabs(c->lucy_sigma_prev - c->lucy_sigma) / c->lucy_sigma < 0.25) ? 1 : 0;
   return MCRT_OK;
 }
 

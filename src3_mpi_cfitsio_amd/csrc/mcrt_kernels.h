@@ -197,6 +197,42 @@ __global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtau
   tcell_at(m, dtauabs, nphot, tdave, c);
 }
 
+// Convergence statistic of tfinal (tfinal_5_test1.f90:334-368): over the
+// upper hemisphere at ip = 1, the cells whose old and new temperatures both
+// exceed the 0.1 K floor and differ contribute x = (tnew - told) / tnew;
+// out[0] = sum |x|, out[1] = sum x^2, out[2] = their count.  One block:
+// each thread sums a fixed strided subset in index order, then a fixed tree
+// in LDS, so the sums are the same bits on every run.
+__global__ void __launch_bounds__(256) k_lucy_sigma(DeviceModel m, const double* told, const double* tnew, double* out) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const double floor_t = 0.1 / 11605.0;
+  const int nr = m.nrg - 1, nh = (m.ntg - 1) / 2;
+  const int n = nr * nh;
+  double a = 0.0, b = 0.0, k = 0.0;
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    const int ir = idx % nr + 1, it = idx / nr + 1;
+    const size_t c = cell_index(m, ir, it, 1);
+    const double to = told[c], tn = tnew[c];
+    if (tn != to && tn > floor_t && to > floor_t) {
+      const double x = (tn - to) / tn;
+      a = fabs(x) + a;
+      b = x * x + b;
+      k = k + 1.0;
+    }
+  }
+  s0[threadIdx.x] = a; s1[threadIdx.x] = b; s2[threadIdx.x] = k;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+      s2[threadIdx.x] += s2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[0] = s0[0]; out[1] = s1[0]; out[2] = s2[0]; }
+}
+
 // Lucy temperatures of every model of a batch: blockIdx.y = model, with the
 // per-model packet count as nphot.  mode 0: tfinal's upper hemisphere and
 // mirror; mode 1: every cell on its own.

@@ -227,16 +227,47 @@ class Engine:
                                            ptr(t, C.c_double)))
         return t
 
+    def dusttemp_sigma(self) -> dict:
+        """tfinal's convergence statistic of the last dusttemp call
+        (mcrt_dusttemp_sigma): sigma, sigma_prev, ncells, converge."""
+        sig, prev = C.c_double(), C.c_double()
+        n, conv = C.c_int64(), C.c_int32()
+        self._check(self.lib.mcrt_dusttemp_sigma(self.h, C.byref(sig), C.byref(prev), C.byref(n), C.byref(conv)))
+        return dict(sigma=sig.value, sigma_prev=prev.value, ncells=int(n.value), converge=bool(conv.value))
+
     def disk(self, npackets: int, seed: int, precision: int = MCRT_FP64,
-             tmode: int = MCRT_T_REFERENCE) -> Dict[str, np.ndarray]:
-        """One Lucy iteration: zero tallies, run every packet, read the
-        reduced tallies, and invert the absorbed energy to a new dust
-        temperature that the next iteration re-emits with."""
+             tmode: int = MCRT_T_REFERENCE, first_packet: int = 0) -> Dict[str, np.ndarray]:
+        """One Lucy iteration (disk, newdisktherm_5_fuv.f90:1): zero tallies,
+        run every packet, read the reduced tallies, and invert the absorbed
+        energy to a new dust temperature that the next iteration re-emits
+        with; tal["converge"] is tfinal's verdict ("yes"/"no") and
+        tal["sigma"] its statistic."""
         self.zero_tallies()
-        self.run_packets(npackets, seed, precision)
+        self.run_packets(npackets, seed, precision, first_packet=first_packet)
         tal = self.tallies()
         tal["tdave"] = self.dusttemp(npackets, tmode, True)
+        st = self.dusttemp_sigma()
+        tal["sigma"], tal["converge"] = st["sigma"], "yes" if st["converge"] else "no"
         return tal
+
+    def lucy(self, npackets: int, seed: int, precision: int = MCRT_FP64, n_iter_min: int = 1,
+             n_iter_max: int = 10, tmode: int = MCRT_T_REFERENCE):
+        """The Lucy driver of newtts.f90:215-256: iterations of disk until
+        tfinal says converge = 'yes', kept going below n_iter_min and
+        stopped at n_iter_max.  Iteration `it` (1-based) runs packets
+        it*npackets .. of the run seeded by `seed`.  Returns the list of
+        per-iteration (sigma, converge) and the last iteration's tallies."""
+        hist, it, converge, tal = [], 0, "no", None
+        while converge == "no":
+            it += 1
+            tal = self.disk(npackets, seed, precision, tmode, first_packet=it * npackets)
+            converge = tal["converge"]
+            hist.append((tal["sigma"], converge))
+            if it < n_iter_min:
+                converge = "no"
+            if it >= n_iter_max:
+                converge = "yes"
+        return hist, tal
 
     # -- deterministic entries ----------------------------------------------
     def trace_rays(self, pos, u, ijk, on_wall, tau_stop, max_steps=4096, precision=MCRT_FP64):

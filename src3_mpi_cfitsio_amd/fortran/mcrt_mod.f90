@@ -133,6 +133,15 @@ module mcrt_mod
       integer(c_int32_t), value :: mode, update_tdust
       type(c_ptr), value :: tdave
     end function
+    ! tfinal's convergence test of the last mcrt_dusttemp (tfinal_5_test1.f90:334-368)
+    integer(c_int) function mcrt_dusttemp_sigma(ctx, sigma, sigma_prev, ncells, converge) &
+         bind(C, name='mcrt_dusttemp_sigma')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: sigma, sigma_prev
+      integer(c_int64_t), intent(out) :: ncells
+      integer(c_int32_t), intent(out) :: converge
+    end function
     ! loss sum(W * image_I) and d loss / d rho per cell (path replay)
     integer(c_int) function mcrt_adjoint_image(ctx, first_packet, npackets, seed, precision, w, loss, grad) &
          bind(C, name='mcrt_adjoint_image')

@@ -21,6 +21,7 @@ program shim_driver
   real(4), allocatable :: sed(:,:,:,:,:,:), psed(:,:,:,:,:), image(:,:,:,:,:)
   real(8) :: counters(16)
   integer(c_int32_t), allocatable :: bnw(:)
+  character(len=3) :: converge
 
   call get_command_argument(1, inpath)
   call get_command_argument(2, outpath)
@@ -61,7 +62,8 @@ program shim_driver
   allocate(dtauabs(nrg, ntg, npg), tdave(nrg, ntg, npg), nabs(nrg, ntg, npg))
   allocate(sed(nfreq, nmu, nph, nap, 8, 10), psed(nfreq, npeel, nap, 8, 9), image(npeel, nx, nx, nband, 8))
   dtauabs = 0.d0; tdave = 0.d0; nabs = 0
-  call disk_gpu(ctx, np, seed, iter, precision, MCRT_T_REFERENCE, nrg, ntg, npg, dtauabs, nabs, tdave, &
+  converge = 'no'
+  call disk_gpu(converge, ctx, np, seed, iter, precision, MCRT_T_REFERENCE, nrg, ntg, npg, dtauabs, nabs, tdave, &
        nfreq, nmu, nph, nap, npeel, nx, nband, sed, psed, image, counters)
   if (mcrt_destroy(ctx) /= 0) stop 3
 

@@ -342,3 +342,33 @@ def test_diskflux_error_columns_are_sample_statistics():
     assert abs(f["si"][idx] / (tal["si"][idx] * m.nmu * m.nph * lsol / npk) - 1.0) < 1e-6
     idx = (0, 0, 1, 3)
     assert abs(f["ti"][idx] / (tal["ti"][idx] * lsol / npk) - 1.0) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# tfinal's convergence statistic
+# ---------------------------------------------------------------------------
+def test_lucy_sigma_is_population_std_of_abs_change(oracle_lib):
+    """tfinal_5_test1.f90:334-368 takes sqrt(<x^2> - <|x|>^2) of
+    x = (tnew - told)/tnew over upper-hemisphere ip = 1 cells where both
+    temperatures exceed 0.1 K and differ: numpy's population standard
+    deviation of |x| over the same cells."""
+    m = M.small("sig", nr=14, nt=10, nphi=3)
+    rng = np.random.default_rng(81)
+    floor_t = 0.1 / 11605.0
+    told = rng.uniform(5, 900, m.cell_shape) / 11605.0
+    tnew = told * rng.uniform(0.7, 1.4, m.cell_shape)
+    told[0, :, :3] = floor_t                                    # floored before
+    tnew[0, 1, 3:6] = floor_t                                   # floored after
+    tnew[0, 2, 6:9] = told[0, 2, 6:9]                           # unchanged
+    sig, n = oracle_lib.lucy_sigma(m, told, tnew)
+    nh = (m.ntg - 1) // 2
+    to, tn = told[0, :nh, :], tnew[0, :nh, :]
+    keep = (tn != to) & (tn > floor_t) & (to > floor_t)
+    x = (tn[keep] - to[keep]) / tn[keep]
+    assert n == keep.sum() and 0 < n < keep.size
+    assert abs(sig - np.std(np.abs(x))) < 1e-13
+    # lower hemisphere and other azimuths do not enter
+    told2 = told.copy()
+    told2[1:] *= 2.0
+    told2[0, nh:] *= 3.0
+    assert oracle_lib.lucy_sigma(m, told2, tnew) == (sig, n)
