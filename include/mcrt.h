@@ -64,7 +64,11 @@ int mcrt_set_medium(mcrt_ctx *ctx, const double *rho, const uint8_t *species,
 /* Dust tables (dust_mod_5.f90:7-21, filled by dustinit/setemiss,
  * dust_5.f90:345-699): per opacity index, nudust descending in eV with
  * kappa/kappav, albedo, g, pl; the Planck tables lTint,Tint (nT), nuint (nnu),
- * kapint[(iop*nnu + j)*nT + i] normalized CDFs and kappap[iop*nT + i]. */
+ * kapint[(iop*nnu + j)*nT + i] normalized CDFs and kappap[iop*nT + i].
+ * Limit: nopac <= 4 opacity tables per model (a packet carries the
+ * opacities of every table at its frequency, opacset's per-packet kapd);
+ * the reference allows up to 187 (dust_mod_5.f90), one per species and
+ * grain-size bin of findopac.  More are refused with MCRT_E_ARG. */
 int mcrt_set_dust(mcrt_ctx *ctx, int32_t nopac, int32_t nlam,
                   const double *nudust, const double *kappad, const double *adust,
                   const double *gdust, const double *pldust, const double *kappav,
@@ -235,7 +239,9 @@ int mcrt_get_tallies(mcrt_ctx *ctx, double *dtauabs, int32_t *nabs, int32_t *dbi
  * for its loss, a replay pass that scatters the derivative of every flight's
  * sampling density and of every peel-off's exp(-tau)); no path is stored.
  * The gradient is taken at the current dust temperature.  The run's own
- * tallies are left untouched. */
+ * tallies are left untouched.  Only stellar photons are replayed: with
+ * accretion, hot-spot or external photons set on the context the call
+ * returns MCRT_E_STATE. */
 int mcrt_adjoint_image(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
                        uint64_t seed, int32_t precision, const float *W,
                        double *loss, double *grad);

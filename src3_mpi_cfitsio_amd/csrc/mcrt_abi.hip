@@ -986,6 +986,12 @@ int mcrt_adjoint_image(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint
   if (!W || !loss) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: null W or loss");
   if (c->dm.ipeel != 1 || c->dm.npeel <= 0 || c->dm.nband <= 0)
     return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: the observer has no peeled images");
+  // the replay runs stellar photons only: a context whose mcrt_run_packets
+  // would emit accretion, hot-spot or external photons describes another
+  // forward model than the one the gradient would be of
+  if (c->accfrac > 0.0 || c->spot.ispot || (c->nisrf > 0 && c->isrf_frac > 0.0))
+    return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: accretion, hot-spot and external photons are not part of the "
+                                 "replayed model; turn them off (mcrt_set_accretion, mcrt_set_spots, mcrt_set_isrf)");
   if (lds_always_doubles(c->dm) > kLdsAlways) return fail(c, MCRT_E_ARG, "mcrt_adjoint_image: tables exceed the LDS budget");
   HIPCHECK(c, hipSetDevice(c->device));
   const bool fp64 = (precision == MCRT_FP64);

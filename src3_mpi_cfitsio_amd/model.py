@@ -536,6 +536,39 @@ def with_accretion(m: Model, accfrac: float, tables: Optional[Dict[str, np.ndarr
                                                  hdisk=np.asarray(t["hdisk"], np.float64)))
 
 
+def with_dusts(m: Model, dusts, r_split_au: float = 10.0) -> Model:
+    """The model with several dust tables, as findopac assigns opacity
+    tables to the species of densarr (grid_mod_5.f90, findopacid_5.f90):
+    dusts is a list of 2 to 4 synthetic_dust tables of one length.  With two
+    tables the disk (species 1) takes the first and the envelope (species 3)
+    the second.  With three or four, cells beyond r_split_au move to
+    species 2 (disk) and 4 (envelope), which take tables 2 and 3 (the
+    fourth shared with the envelope when there are three).  Each table gets
+    its own emission tables (kapint, kappap) on the model's nT x nnu grid."""
+    nd = len(dusts)
+    if not 2 <= nd <= 4:
+        raise ValueError("with_dusts: 2 to 4 dust tables")
+    nlam = len(dusts[0]["nudust"])
+    if any(len(d["nudust"]) != nlam for d in dusts):
+        raise ValueError("with_dusts: every table needs the same number of wavelengths")
+    ems = [emission_tables(d, nT=m.nT, nnu=m.nnu) for d in dusts]
+    species = m.species.copy()
+    so = np.zeros(11, dtype=np.int32)
+    so[1], so[3] = 0, 1
+    if nd >= 3:
+        autors = AU / (m.rstar * RSUN)
+        rc = (0.5 * (m.rarr[1:] + m.rarr[:-1]) / autors)[None, None, :]
+        far = np.broadcast_to(rc > r_split_au, species.shape)
+        species = np.where(far & (species == 1), 2, np.where(far & (species == 3), 4, species)).astype(np.uint8)
+        so[2], so[4] = 2, (3 if nd == 4 else 1)
+    st = lambda k: np.stack([np.asarray(d[k], np.float64) for d in dusts])   # noqa: E731
+    return dataclasses.replace(
+        m, species=species, species_opac=so, nopac=nd, nlam=nlam, nudust=st("nudust"), kappad=st("kappad"),
+        adust=st("adust"), gdust=st("gdust"), pldust=st("pldust"),
+        kappav=np.array([d["kappav"] for d in dusts], np.float64),
+        kapint=np.stack([e["kapint"] for e in ems]), kappap=np.stack([e["kappap"] for e in ems]))
+
+
 def with_isrf(m: Model, isrf_frac: float, nisrf: int = 200) -> Model:
     """The model with external illumination carrying isrf_frac of the packets."""
     return dataclasses.replace(m, isrf_frac=float(isrf_frac), isrf=isrf_tables(nisrf))

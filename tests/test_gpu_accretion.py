@@ -104,3 +104,36 @@ def test_accretion_rejects_bad_tables():
         with pytest.raises(MCRTError, match="accfrac"):
             e.set_accretion(dict(accfrac=1.0, ldint=t["ldint"], hdisk=t["hdisk"]))
         e.set_accretion(None)
+
+
+def test_accretion_at_warm_dust_matches_oracle(oracle_lib):
+    """Accretion photons are re-emitted at the dust temperature of the cell
+    they start in: with a warm field (150-900 K over the disk, falling with
+    radius) re-emission samples interior rows of every CDF, not the f = 0
+    row of the 0.1 K start."""
+    from src3_mpi_cfitsio_amd.engine import Engine
+    m = M.with_accretion(M.small("accw", nr=20, nt=12, nphi=4, clump_sigma=0.5), 0.5)
+    rc = 0.5 * (m.rarr[1:] + m.rarr[:-1])
+    warm = (900.0 * (rc / rc[0]) ** -0.4)[None, None, :] * np.ones(m.cell_shape)
+    tdust = np.where(m.species > 0, np.maximum(warm, 150.0), 0.1) / 11605.0
+    n = 3000
+    print(f"accretion parity at warm dust, n={n}", flush=True)
+    with Engine(m) as e:
+        e.set_tdust(tdust)
+        e.zero_tallies()
+        e.enable_trace(n)
+        e.run_packets(n, seed=43)
+        gt = e.get_trace(n)
+        tg = e.tallies()
+    o = oracle_lib.run_packets(m, n, seed=43, tdust=tdust, trace=True)
+    same = np.all(rel_err(o["trace"], gt) <= 1e-9, axis=1)
+    acc_to = int(0.5 * n) - 1
+    print(f"agreement acc {same[:acc_to].mean():.4f}, rest {same[acc_to:].mean():.4f}", flush=True)
+    assert same[:acc_to].mean() >= 0.99 and same[acc_to:].mean() >= 0.99
+    # re-emitted frequencies of the accretion photons follow the warm rows:
+    # far above the 0.1 K row's frequencies
+    assert np.median(gt[:acc_to, 0]) > 50.0 * 0.1 / 11605.0
+    assert tg["counters"]["errors"] == 0
+    for name in ("nabs", "si", "image_b_i"):
+        a, b = tg[name].astype(np.float64), o[name].astype(np.float64)
+        assert abs(a.sum() - b.sum()) <= 0.02 * max(abs(b.sum()), 1e-30), name

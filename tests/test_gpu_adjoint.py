@@ -143,3 +143,22 @@ def test_adjoint_through_torch_autograd(eng_adj):
     # the two adjoint runs differ only in the order of their FP64 atomics
     assert abs(float(loss.detach()) - l_ref) <= 1e-9 * abs(l_ref)
     np.testing.assert_allclose(rho.grad.numpy(), 2.0 * g_ref, rtol=0, atol=1e-9 * np.abs(g_ref).max())
+
+
+@pytest.mark.parametrize("extra", ["accretion", "spots", "isrf"])
+def test_adjoint_refuses_sources_it_does_not_replay(extra):
+    """The replay runs stellar photons only: a context that would also emit
+    accretion, hot-spot or external photons is refused, not silently
+    differentiated as another model."""
+    from src3_mpi_cfitsio_amd.engine import Engine, MCRTError
+    m = M.small("adjx", nr=16, nt=10, nphi=1)
+    m = {"accretion": lambda x: M.with_accretion(x, 0.2), "spots": lambda x: M.with_spots(x),
+         "isrf": lambda x: M.with_isrf(x, 0.1)}[extra](m)
+    W = np.ones((m.nband, m.nx, m.nx, m.npeel), np.float32)
+    with Engine(m) as e:
+        with pytest.raises(MCRTError, match="not part of the replayed model"):
+            e.adjoint_image(W, 100, seed=1)
+        {"accretion": lambda: e.set_accretion(None), "spots": lambda: e.set_spots(None),
+         "isrf": lambda: e.set_isrf(0.0)}[extra]()
+        loss, _ = e.adjoint_image(W, 100, seed=1, grad=False)
+        assert loss > 0.0
