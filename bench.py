@@ -321,15 +321,26 @@ def run_single_workload(args, T0):
     m = M.config(args.config)
     if args.accfrac > 0.0:
         m = M.with_accretion(m, args.accfrac)
+    if args.diffusion:
+        m = M.with_diffusion(m)
     log(f"model {args.config}: {m.nrg-1}x{m.ntg-1}x{m.npg-1} cells built in {time.perf_counter()-t0:.1f}s")
     mass = m.rho * m.vcell
     eng = Engine(m)
+    lucy_iter = [0]
+
+    def setdiffus():
+        # gridset calls setdiffus at the start of every Lucy iteration
+        if args.diffusion:
+            lucy_iter[0] += 1
+            nd = eng.setdiffus(lucy_iter[0])
+            log(f"diffusion region: {nd} cells (Lucy iteration {lucy_iter[0]})")
 
     # temperature warm-up: short Lucy iterations so the timed steps transport
     # packets through a heated disk (re-emission at the dust temperature)
     nw = 2_000_000
     nwarm = max(1, min(4, args.warmup)) if args.warmup > 0 else 1
     for k in range(nwarm):
+        setdiffus()
         eng.zero_tallies()
         eng.run_packets(nw, seed=1000 + k, precision=MCRT_MIXED, first_packet=0)
         eng.synchronize()
@@ -344,6 +355,7 @@ def run_single_workload(args, T0):
     first = 0
 
     def step(t_in):
+        setdiffus()
         eng.synchronize()
         ts = time.perf_counter()
         eng.zero_tallies()
@@ -367,7 +379,7 @@ def run_single_workload(args, T0):
     tal, tcur = step(t_start)
     first += N
     extra = {}
-    if not args.no_fp64:
+    if not args.no_fp64 and not args.diffusion:
         t_mixed = eng.dusttemp(N, mode=1, update_tdust=False)     # per-cell T of the mixed step's tallies
         log("fp64 comparison: the first timed step again in FP64 (the engine logs each round to stderr)")
         eng.set_tdust(t_start)
@@ -439,7 +451,8 @@ def run_single_workload(args, T0):
                    "steps_requested": args.steps, "warmup_requested": args.warmup,
                    "warmup_kind": f"{nwarm} Lucy iterations of {nw} packets (heats the dust; loads the kernels)",
                    "budget_s": args.budget, "wall_s": round(elapsed(), 1),
-                   "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {})},
+                   "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {}),
+                   **({"diffusion": True} if args.diffusion else {})},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
                      "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
@@ -468,6 +481,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--accfrac", type=float, default=0.0,
                     help="fraction of each step's packets emitted by disk accretion (c1/c0 workloads)")
+    ap.add_argument("--diffusion", action="store_true",
+                    help="c1/c0 workloads with the diffusion approximation of thick regions (DIFFUSION = YES); "
+                         "no FP64 twin (the diffusion region is set per Lucy iteration)")
     ap.add_argument("--copy-probe", type=int, default=0, help="only copy this many bytes on the GPU")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "setemiss"],
                     help="c1: the headline single-model metric; c2: the 512-model batch; c3: the image gradient; "

@@ -174,6 +174,36 @@ int mcrt_set_spots(mcrt_ctx *ctx, int32_t nspot, double fspot, double spotlat, d
  * accretion photons. */
 int mcrt_set_accretion(mcrt_ctx *ctx, const double *ldint, const double *hdisk, double accfrac);
 
+/* The diffusion approximation of very optically thick regions (DIFFUSION =
+ * YES in the reference's parameter file; setdiffus_5.f90,
+ * emit_common_2.f90:88-378, tfinal_5_test1.f90:232-233).  on = 0 turns it
+ * off.  kappar[iop*nT + i]: the Rosseland means setemiss leaves (kappar of
+ * mcrt_setemiss); hdisk[ir-1]: the disk scale height (R*) of radial cell
+ * ir; rmaxd: the disk's outer radius (R*); nr0: the first radial cell
+ * that may diffuse (1 when the dust destruction radius is the inner wall,
+ * else 3).  Set grid, medium, dust and source first.  Theta diffusion
+ * only (the reference's setdiffus sets no other direction). */
+int mcrt_set_diffusion(mcrt_ctx *ctx, int32_t on, const double *kappar, const double *hdisk,
+                       double rmaxd, int32_t nr0);
+/* setdiffus for Lucy iteration iter (called by gridset at the start of each
+ * disk call): the region of every radial column where the Rosseland
+ * optical depth toward the pole reaches 1 within a twentieth of the disk
+ * scale height, from the temperatures of the last mcrt_dusttemp (below
+ * iter 2 floored at 50 K), mirrored about the midplane and the same in
+ * every phi wedge; the dust temperature of those cells restarts at 0.1 K
+ * (gridset_5_model13new.f90:1857-1868).  Then mcrt_run_packets moves every
+ * photon absorbed there to the region's surface and re-emits it at the
+ * surface temperature, which grows with each such photon (T^4 += 4 pi T*^4
+ * / (ns dA), ns the run's stellar photons; the GPU orders the photons of a
+ * surface cell by arrival), and mcrt_dusttemp keeps those temperatures for
+ * the region.  *ncells (may be NULL): diffusion cells. */
+int mcrt_setdiffus(mcrt_ctx *ctx, int32_t iter, int64_t *ncells);
+/* The diffusion region and its photons: itmin[ir-1] (nrg-1) = the first
+ * diffusion cell of column ir from the pole (0: none; cells itmin ..
+ * ntg-itmin diffuse); count[cell] = photons re-emitted from each surface
+ * cell since the tallies were zeroed.  Either pointer may be NULL. */
+int mcrt_get_diffusion(mcrt_ctx *ctx, int32_t *itmin, uint32_t *count);
+
 /* Star (tts_mod_5.f90: rstar [R_sun], tstar [K/11605]; rmin = rarr(1)). */
 int mcrt_set_source(mcrt_ctx *ctx, double rstar, double tstar, double rmin);
 

@@ -618,7 +618,14 @@ double orc_dusttemp(const orc_model *m, double na, double nphot, int32_t iop, do
  * and gradient terms of :61-205 do not change T): Lucy temperature over the
  * upper hemisphere at ip=1 averaged with its mirror cell, mirrored to the
  * lower hemisphere and replicated in phi. */
-void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double *tdave)
+static int orc_diffusing(const orc_model *m, int ir, int it)
+{
+    if (!m->diff_itmin) return 0;
+    int a = m->diff_itmin[ir - 1];
+    return a > 0 && it >= a && it <= m->ntg - a;
+}
+
+void orc_tfinal2(const orc_model *m, const double *dtauabs, double nphot, const double *tdust, double *tdave)
 {
     const double Tfloor = 0.1 / 11605.0;
     int nrg = m->nrg, ntg = m->ntg, npg = m->npg, ip = 1;
@@ -627,7 +634,10 @@ void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double 
             size_t c = CELL(m, ir, it, ip), cm = CELL(m, ir, ntg - it, ip);
             double t = Tfloor;
             int sp = m->species[c];
-            if (sp > 0 && m->rho[c] > 0.0) {
+            if (sp > 0 && m->rho[c] > 0.0 && tdust && orc_diffusing(m, ir, it)) {
+                t = (tdust[c] + tdust[cm]) / 2.0;           /* tfinal_5_test1.f90:232-233 */
+                if (t * 11605.0 >= TMAXK) t = TMAXK / 11605.0;
+            } else if (sp > 0 && m->rho[c] > 0.0) {
                 int iop = m->species_opac[sp];
                 double rho2 = (m->species[cm] == sp) ? m->rho[cm] : 0.0;
                 double mcell1 = pow(m->rho[c] * m->vcell[c], 0.25);
@@ -647,6 +657,133 @@ void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double 
             for (int it = 1; it <= ntg - 1; ++it)
                 for (int ir = 1; ir <= nrg - 1; ++ir)
                     tdave[CELL(m, ir, it, ipp)] = tdave[CELL(m, ir, it, 1)];
+}
+
+void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double *tdave)
+{
+    orc_tfinal2(m, dtauabs, nphot, NULL, tdave);
+}
+
+/* chiR, dust_5.f90:112-141 (the index clamped into the table instead of the stop) */
+static double orc_chiR(const orc_model *m, const double *kappar, double T, int iop)
+{
+    int nT = m->nT;
+    double lT = log(T);
+    int k = (int)((double)(nT - 1) * (lT - m->lTint[0]) / (m->lTint[nT - 1] - m->lTint[0])) + 1;
+    if (k < 1) k = 1;
+    if (k > nT - 1) k = nT - 1;
+    double f = (lT - m->lTint[k - 1]) / (m->lTint[k] - m->lTint[k - 1]);
+    return (1.0 - f) * kappar[(size_t)iop * nT + k - 1] + f * kappar[(size_t)iop * nT + k];
+}
+
+/* setdiffus_5.f90:22-103 as written: for every column ir from nr0 while
+ * rarr(ir) < rmaxd and every it from (ntg-1)/2 down to 2, the Rosseland
+ * optical depth toward the pole and the mean free path at depth 1; cells
+ * it..(ntg-1)/2 diffuse (diffdir -2) when the depth reaches 1 within
+ * hdisk/20; mirrored (+2) and copied to every phi. */
+void orc_setdiffus(const orc_model *m, const double *kappar, const double *hdisk, const double *tdave,
+                   double rmaxd, int32_t nr0, int32_t iter, uint8_t *diffus, int8_t *diffdir, int32_t *itmin)
+{
+    const int nrg = m->nrg, ntg = m->ntg, npg = m->npg;
+    const size_t nc = (size_t)(nrg - 1) * (ntg - 1) * (npg - 1);
+    memset(diffus, 0, nc);
+    memset(diffdir, 0, nc);
+    const double kapd_cm = m->rstar * RSOL;           /* kapd = kappav * rmincgs (gridset:79) */
+    int ir = nr0;
+    double rad = m->rarr[ir - 1];
+    while (rad < rmaxd && ir <= nrg - 1) {
+        double zmr = hdisk[ir - 1];
+        for (int it = (ntg - 1) / 2; it >= 2; --it) {
+            double tau = 0.0, tau1 = 0.0;
+            int itt = it;
+            while (tau < 1.0 && itt > 1) {
+                double dt = m->thetarr[itt] - m->thetarr[itt - 1];
+                tau1 = tau;
+                size_t c = CELL(m, ir, itt, 1);
+                double tt = tdave[c];
+                if (iter < 2) tt = fmax(tt, 50.0 / 11605.0);
+                int sp = m->species[c];
+                if (sp > 0 && m->rho[c] > 0.0) {
+                    int iop = m->species_opac[sp];
+                    tau = tau + m->kappav[iop] * kapd_cm * orc_chiR(m, kappar, tt, iop) * m->rho[c] * dt * rad;
+                }
+                itt = itt - 1;
+            }
+            double lmfp1;
+            if (tau > 1.0) lmfp1 = rad * (m->thetarr[itt + 1] - m->thetarr[itt]) / (tau - tau1) * (1.0 - tau1);
+            else lmfp1 = rad * (m->thetarr[itt + 1] - m->thetarr[itt]);
+            double lmfp = rad * (m->thetarr[it] - m->thetarr[itt + 1]) + lmfp1;
+            if (tau >= 1.0 && lmfp < zmr / 20.0)
+                for (int it1 = it; it1 <= (ntg - 1) / 2; ++it1) {
+                    diffus[CELL(m, ir, it1, 1)] = 1;
+                    diffdir[CELL(m, ir, it1, 1)] = -2;
+                }
+        }
+        ir = ir + 1;
+        rad = m->rarr[ir - 1];
+    }
+    for (int r = 1; r <= nrg - 1; ++r)
+        for (int it = (ntg + 1) / 2; it <= ntg - 1; ++it) {
+            diffus[CELL(m, r, it, 1)] = diffus[CELL(m, r, ntg - it, 1)];
+            diffdir[CELL(m, r, it, 1)] = (int8_t)-diffdir[CELL(m, r, ntg - it, 1)];
+        }
+    if (npg > 2)
+        for (int ip = 2; ip <= npg - 1; ++ip)
+            for (int it = 1; it <= ntg - 1; ++it)
+                for (int r = 1; r <= nrg - 1; ++r) {
+                    diffus[CELL(m, r, it, ip)] = diffus[CELL(m, r, it, 1)];
+                    diffdir[CELL(m, r, it, ip)] = diffdir[CELL(m, r, it, 1)];
+                }
+    for (int r = 1; r <= nrg - 1; ++r) {
+        itmin[r - 1] = 0;
+        for (int it = 1; it <= (ntg - 1) / 2; ++it)
+            if (diffus[CELL(m, r, it, 1)]) { itmin[r - 1] = it; break; }
+    }
+}
+
+/* emit_common_2.f90:256-341, the state a column's last diffusion photon
+ * leaves: the surface T^4 = T0^4 + N delta/dA, the interior integrated down
+ * to the midplane with the accretion photons below (nabs). */
+void orc_diff_columns(const orc_model *m, const double *kappar, const uint32_t *count, const int32_t *nabs,
+                      double delta, double *tdust)
+{
+    const int nrg = m->nrg, ntg = m->ntg, npg = m->npg;
+    const double Tcap = TMAXK / 11605.0, T0 = 0.1 / 11605.0;
+    for (int ip = 1; ip <= npg - 1; ++ip)
+        for (int ir = 1; ir <= nrg - 1; ++ir)
+            for (int h = 0; h < 2; ++h) {
+                int a = m->diff_itmin[ir - 1];
+                if (a <= 0) continue;
+                int upper = (h == 0);
+                int its = upper ? a : ntg - a, mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2, step = upper ? 1 : -1;
+                int w = upper ? its : its + 1;
+                double sth = sin(m->thetarr[w - 1]);
+                double dA = PI * sth * (m->rarr[ir] * m->rarr[ir] - m->rarr[ir - 1] * m->rarr[ir - 1]);
+                size_t cs = CELL(m, ir, its, ip);
+                double T4 = T0 * T0 * T0 * T0 + (double)count[cs] * delta / dA;
+                double Ts = pow(T4, 0.25);
+                if (Ts > Tcap) Ts = Tcap;
+                if (!(m->species[cs] > 0 && m->rho[cs] > 0.0)) Ts = T0;
+                tdust[cs] = Ts;
+                double nacc = 0.0;
+                for (int i = its; upper ? i <= mid : i >= mid; i += step) nacc += (double)nabs[CELL(m, ir, i, ip)];
+                double T4d = T4;
+                double rc = 0.5 * (m->rarr[ir - 1] + m->rarr[ir]);
+                for (int i = its + step; upper ? i <= mid : i >= mid; i += step) {
+                    int ib = i - step;
+                    size_t cb = CELL(m, ir, ib, ip);
+                    int sp = m->species[cb];
+                    if (sp > 0 && m->rho[cb] > 0.0) {
+                        int iop = m->species_opac[sp];
+                        double dth = upper ? m->thetarr[i - 1] - m->thetarr[i - 2] : m->thetarr[i + 1] - m->thetarr[i];
+                        T4d = T4d + 0.75 * delta / dA * nacc * m->kappav[iop] * RSOL * m->rstar * m->rho[cb] *
+                                        orc_chiR(m, kappar, tdust[cb], iop) * rc * dth;
+                        double Ti = pow(T4d, 0.25);
+                        tdust[CELL(m, ir, i, ip)] = Ti > Tcap ? Tcap : Ti;
+                        nacc = nacc - (double)nabs[cb];
+                    }
+                }
+            }
 }
 
 /* tfinal_5_test1.f90:334-368: the convergence statistic.  Over the upper
@@ -1067,6 +1204,67 @@ static void reemit(orc_run *R, orc_pkt *p, double tave, int iop)
     p->uy = p->sint * p->sinp; p->ux = p->sint * p->cosp; p->uz = p->cost;
 }
 
+/* emit_common_2.f90:226-378 (theta diffusion): a photon absorbed in a
+ * diffusion cell moves at fixed radius and azimuth to just outside the
+ * region's surface wall, leaves with a cosine law about the outward normal
+ * (isotrp.f90) at the surface temperature, which grows by 4 pi T*^4/(ns dA)
+ * in T^4 with every photon of that surface cell in the order they arrive
+ * (one counter for all threads), plus 0.75 mu T4eff of the accretion
+ * photons at the midplane.  The Stokes vector is left as it was. */
+static void diffuse(orc_run *R, orc_pkt *p)
+{
+    const orc_model *m = R->m;
+    const int ntg = m->ntg;
+    const int a = m->diff_itmin[p->ir - 1];
+    const int upper = p->it <= (ntg - 1) / 2;
+    const int its = upper ? a : ntg - a;
+    const int w = upper ? its : its + 1;
+    const double cw = m->costarr[w - 1], sw = sin(m->thetarr[w - 1]);
+    double cosb = p->zp / p->rtot;
+    double sinb = sqrt(1.0 - cosb * cosb);
+    double cosbnew = 1.00001 * cw;
+    double sinbnew = sqrt(1.0 - cosbnew * cosbnew);
+    p->xp = p->xp * sinbnew / sinb;
+    p->yp = p->yp * sinbnew / sinb;
+    p->zp = p->rtot * cosbnew;
+    double sux, suy, suz;
+    if (cw > 0.0) { sux = -cw * p->xp / sw / p->rtot; suy = -cw * p->yp / sw / p->rtot; suz = sw; }
+    else { sux = cw * p->xp / sw / p->rtot; suy = cw * p->yp / sw / p->rtot; suz = -sw; }
+    double x = 0.0, y = 0.0, z = 0.0, r2 = 2.0, mur = 0.0;
+    while (r2 * r2 > fabs(mur)) {
+        x = ran(&R->rng); x = x + x - 1.0;
+        y = ran(&R->rng); y = y + y - 1.0;
+        z = ran(&R->rng); z = z + z - 1.0;
+        mur = x * sux + y * suy + z * suz;
+        r2 = x * x + y * y + z * z;
+    }
+    double r = sqrt(r2);
+    if (mur < 0.0) r = -r;
+    double ux = x / r, uy = y / r, uz = z / r;
+    p->cost = uz;
+    p->sint = sqrt(1.0 - uz * uz);
+    p->cosp = ux / p->sint;
+    p->sinp = uy / p->sint;
+    p->phi = atan2(p->sinp, p->cosp);
+    double dA = PI * sw * (m->rarr[p->ir] * m->rarr[p->ir] - m->rarr[p->ir - 1] * m->rarr[p->ir - 1]);
+    size_t cs = CELL(m, p->ir, its, p->ip);
+    uint32_t k;
+    #pragma omp atomic capture
+    k = m->diff_count[cs]++;
+    const double Tcap = TMAXK / 11605.0, T0 = 0.1 / 11605.0;
+    double Ts = pow(T0 * T0 * T0 * T0 + (double)(k + 1u) * m->diff_delta / dA, 0.25);
+    if (Ts > Tcap) Ts = Tcap;
+    int mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2;
+    double nacc = (double)R->t->nabs[CELL(m, p->ir, mid, p->ip)];
+    double T4eff = m->diff_delta * nacc / dA;
+    double mu = ux * sux + uy * suy + uz * suz;
+    double Te = pow(Ts * Ts * Ts * Ts + 0.75 * mu * T4eff, 0.25);
+    if (Te > Tcap) Te = Tcap;
+    p->nub = bbfreq(&R->rng, Te);
+    p->it = upper ? its - 1 : its + 1;
+    p->ux = p->sint * p->cosp; p->uy = p->sint * p->sinp; p->uz = p->cost;
+}
+
 /* propagate_5_fuv.f90:1-326.  Returns exit code: 1 escaped, 2 killed, -1 error. */
 static int propagate(orc_run *R, orc_pkt *p, int *n_inter)
 {
@@ -1092,9 +1290,13 @@ static int propagate(orc_run *R, orc_pkt *p, int *n_inter)
             } else {
                 t->dbig[c] += 1;
                 p->i_orig = i_orig_of(idust2);
-                t->nabs[c] += 1;
-                /* emit_common_final.f90:73-81 -> reemit.f90:17-34 */
-                reemit(R, p, R->tdust[c], iop);
+                /* propagate_5_fuv.f90:139, emit_common_2.f90:29-100 */
+                if (orc_diffusing(m, p->ir, p->it)) diffuse(R, p);
+                else {
+                    t->nabs[c] += 1;
+                    /* emit_common_final.f90:73-81 -> reemit.f90:17-34 */
+                    reemit(R, p, R->tdust[c], iop);
+                }
                 p->iflag = 0;
                 opacset(m, p->nub, p->op);
                 p->wave = 1.2398 / p->nub;
@@ -1287,7 +1489,8 @@ static int emit_acc(orc_run *R, orc_pkt *p, int *idust2)
     t->dbig[c] += 1;
     t->nabs[c] += 1;
     const int iop = m->species_opac[*idust2];
-    reemit(R, p, R->tdust[c], iop);
+    if (orc_diffusing(m, p->ir, p->it)) diffuse(R, p);
+    else reemit(R, p, R->tdust[c], iop);
     p->on_wall = 0;
     p->iflag = 0;
     p->i_orig = 2;

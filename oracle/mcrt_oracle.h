@@ -81,6 +81,14 @@ typedef struct {
      * cells (grid_mod) */
     int64_t acc_to;
     const double *acc_ldint, *acc_hdisk;
+    /* diffusion approximation (setdiffus_5.f90, emit_common_2.f90:88-378):
+     * diff_itmin[ir-1] = first diffusion cell of column ir from the pole (0
+     * none; cells itmin..ntg-itmin diffuse), NULL: off; diff_delta = 4 pi
+     * Tstar^4 / ns; diff_count: photons re-emitted per surface cell (shared
+     * by all threads, written) */
+    const int32_t *diff_itmin;
+    double diff_delta;
+    uint32_t *diff_count;
 } orc_model;
 
 /* Tallies: every pointer is caller-owned, zeroed by the caller, accumulated here.
@@ -135,6 +143,20 @@ double orc_bbfreq_lecuyer(double teff);   /* bbfreq with the L'Ecuyer stream (ra
 double orc_dusttemp(const orc_model *m, double na, double nphot, int32_t iop, double mcell);
 /* tfinal with maxii=1 and no velocities: Lucy update, midplane mirror, phi replication */
 void orc_tfinal(const orc_model *m, const double *dtauabs, double nphot, double *tdave);
+/* setdiffus_5.f90:36-103: the diffusion flags of every cell (diffus,
+ * cells, 1 = diffusion) and diffdir (-2 upper half, +2 lower half), and
+ * itmin per radial column; tdave: the last Lucy temperatures; kappar
+ * [iop*nT + i] the Rosseland means; hdisk (nrg-1) R*; rmaxd R*. */
+void orc_setdiffus(const orc_model *m, const double *kappar, const double *hdisk, const double *tdave,
+                   double rmaxd, int32_t nr0, int32_t iter, uint8_t *diffus, int8_t *diffdir, int32_t *itmin);
+/* emit_common_2.f90:256-341 after a run: the temperatures of every diffusion
+ * column from its surface cell's photon count and the accretion photons
+ * (nabs) below it; writes tdust in the diffusion cells. */
+void orc_diff_columns(const orc_model *m, const double *kappar, const uint32_t *count, const int32_t *nabs,
+                      double delta, double *tdust);
+/* tfinal with the dust temperatures of the run (a diffusion cell keeps its
+ * own, tfinal_5_test1.f90:232-233; tdust may be NULL without diffusion) */
+void orc_tfinal2(const orc_model *m, const double *dtauabs, double nphot, const double *tdust, double *tdave);
 /* tfinal_5_test1.f90:334-368 convergence statistic (sums in out[3]) */
 double orc_lucy_sigma(const orc_model *m, const double *told, const double *tnew, double out[3]);
 

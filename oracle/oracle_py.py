@@ -46,6 +46,7 @@ class OrcModel(C.Structure):
         ("sspot1", C.c_double), ("scspot2", C.c_double), ("ssspot2", C.c_double), ("cspot2", C.c_double),
         ("sspot2", C.c_double), ("hot_to", C.c_int64),
         ("acc_to", C.c_int64), ("acc_ldint", C.POINTER(C.c_double)), ("acc_hdisk", C.POINTER(C.c_double)),
+        ("diff_itmin", _ip), ("diff_delta", C.c_double), ("diff_count", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -262,10 +263,20 @@ class OracleModel:
             self.s.hot_to = 0
 
 
-def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_mode=0, trace=False):
-    """Oracle packet loop; returns tallies named like Engine.tallies()."""
+def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_mode=0, trace=False, diffusion=None):
+    """Oracle packet loop; returns tallies named like Engine.tallies().
+    diffusion: dict(itmin, count) from setdiffus (count, a uint32 array of
+    cell_shape, is updated in place) to run with the diffusion approximation;
+    its surface increment is 4 pi Tstar^4 / ns of this run."""
     L = oracle()
     om = OracleModel(m)
+    if diffusion is not None:
+        om.arrs["diff_itmin"] = np.ascontiguousarray(diffusion["itmin"], dtype=np.int32)
+        om.s.diff_itmin = _p(om.arrs["diff_itmin"], C.c_int32)
+        cnt = diffusion["count"]
+        assert cnt.dtype == np.uint32 and cnt.flags["C_CONTIGUOUS"]
+        om.s.diff_count = cnt.ctypes.data_as(C.POINTER(C.c_uint32))
+        om.s.diff_delta = diffusion_delta(m, npackets)
     cs = m.cell_shape
     out = dict(dtauabs=np.zeros(cs), nabs=np.zeros(cs, np.int32), dbig=np.zeros(cs, np.int32),
                killed=np.zeros(cs, np.int32))
@@ -297,6 +308,64 @@ def run_packets(m, npackets, seed, first_packet=0, tdust=None, nthreads=1, rng_m
     if trace:
         out["trace"] = tr
     return out
+
+
+def diffusion_delta(m, npackets):
+    """4 pi Tstar^4 / ns, ns = the run's stellar photons (np - npacc - npout)."""
+    nout = 0
+    if getattr(m, "isrf", None) is not None and m.isrf_frac > 0.0:
+        nout = int(m.isrf_frac * float(npackets))
+        if m.isrf_frac * float(npackets) - nout > 0.5:
+            nout += 1
+    acc = getattr(m, "accretion", None)
+    nacc = int(acc["accfrac"] * float(npackets)) if acc and acc["accfrac"] > 0.0 else 0
+    return 4.0 * np.pi * m.tstar ** 4 / float(npackets - nacc - nout)
+
+
+def setdiffus(m, tdave, iteration):
+    """setdiffus_5.f90 (orc_setdiffus) with the model's diffusion set-up:
+    returns diffus (bool, cell_shape), diffdir (int8) and itmin (nrg - 1)."""
+    L = oracle()
+    om = OracleModel(m)
+    d = m.diffusion
+    kr, hd = np.ascontiguousarray(d["kappar"], np.float64), np.ascontiguousarray(d["hdisk"], np.float64)
+    td = np.ascontiguousarray(tdave, np.float64)
+    dif = np.zeros(m.cell_shape, np.uint8)
+    ddir = np.zeros(m.cell_shape, np.int8)
+    itmin = np.zeros(m.nrg - 1, np.int32)
+    L.orc_setdiffus(C.byref(om.s), _p(kr, C.c_double), _p(hd, C.c_double), _p(td, C.c_double), C.c_double(d["rmaxd"]),
+                    C.c_int32(int(d["nr0"])), C.c_int32(int(iteration)), _p(dif, C.c_uint8),
+                    ddir.ctypes.data_as(C.POINTER(C.c_int8)),
+                    _p(itmin, C.c_int32))
+    return dif.astype(bool), ddir, itmin
+
+
+def diff_columns(m, itmin, count, nabs, npackets, tdust):
+    """orc_diff_columns: tdust with the diffusion columns' temperatures after a run."""
+    L = oracle()
+    om = OracleModel(m)
+    om.arrs["diff_itmin"] = np.ascontiguousarray(itmin, dtype=np.int32)
+    om.s.diff_itmin = _p(om.arrs["diff_itmin"], C.c_int32)
+    kr = np.ascontiguousarray(m.diffusion["kappar"], np.float64)
+    cnt = np.ascontiguousarray(count, np.uint32)
+    na = np.ascontiguousarray(nabs, np.int32)
+    out = np.array(tdust, dtype=np.float64, order="C", copy=True)
+    L.orc_diff_columns(C.byref(om.s), _p(kr, C.c_double), cnt.ctypes.data_as(C.POINTER(C.c_uint32)),
+                       _p(na, C.c_int32), C.c_double(diffusion_delta(m, npackets)), _p(out, C.c_double))
+    return out
+
+
+def tfinal2(m, dtauabs, nphot, tdust, itmin):
+    """tfinal with the diffusion region itmin keeping its run temperatures."""
+    L = oracle()
+    om = OracleModel(m)
+    om.arrs["diff_itmin"] = np.ascontiguousarray(itmin, dtype=np.int32)
+    om.s.diff_itmin = _p(om.arrs["diff_itmin"], C.c_int32)
+    d = np.ascontiguousarray(dtauabs, dtype=np.float64)
+    td = np.ascontiguousarray(tdust, dtype=np.float64)
+    t = np.zeros(m.cell_shape)
+    L.orc_tfinal2(C.byref(om.s), _p(d, C.c_double), C.c_double(nphot), _p(td, C.c_double), _p(t, C.c_double))
+    return t
 
 
 def tfinal(m, dtauabs, nphot):

@@ -38,6 +38,9 @@ SIGNATURES = {
     "mcrt_set_isrf": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, C.c_double]),
     "mcrt_set_spots": (C.c_int, [_vp, C.c_int32] + [C.c_double] * 5),
     "mcrt_set_accretion": (C.c_int, [_vp, _dp, _dp, C.c_double]),
+    "mcrt_set_diffusion": (C.c_int, [_vp, C.c_int32, _dp, _dp, C.c_double, C.c_int32]),
+    "mcrt_setdiffus": (C.c_int, [_vp, C.c_int32, C.POINTER(C.c_int64)]),
+    "mcrt_get_diffusion": (C.c_int, [_vp, _ip, C.POINTER(C.c_uint32)]),
     "mcrt_set_source": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double]),
     "mcrt_set_observer": (C.c_int, [_vp, C.c_int32, C.c_int32, _dp, _dp, C.c_int32, C.c_double,
                                     C.c_int32, C.c_int32, _ip, _dp, _dp,

@@ -126,6 +126,33 @@ hipError_t launch_lucy_sigma(const DeviceModel& m, const double* told, const dou
   return hipGetLastError();
 }
 
+hipError_t launch_setdiffus(const DeviceModel& m, const DiffArgs& a, hipStream_t s) {
+  const int nr = m.nrg - 1, nh = (m.ntg - 1) / 2;
+  hipError_t e = launch_fill_i32(a.itmin, (size_t)nr, 0x7fffffff, s);
+  if (e != hipSuccess) return e;
+  if (nh >= 2) {
+    const int n = nr * (nh - 1);
+    hipLaunchKernelGGL(k_setdiffus, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // columns where no cell qualifies: 0
+  hipLaunchKernelGGL(k_itmin_none, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, a.itmin, nr);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_reset(const DeviceModel& m, double* tdust, hipStream_t s) {
+  const size_t n = (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1);
+  hipLaunchKernelGGL(k_diff_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, tdust);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_columns(const DeviceModel& m, const DiffArgs& a, const unsigned* count, const int* nabs,
+                               double delta, double* tdust, hipStream_t s) {
+  const int n = (m.nrg - 1) * (m.npg - 1) * 2;
+  hipLaunchKernelGGL(k_diff_columns, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, a, count, nabs, delta, tdust);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_tfinal(const BatchLaunch& L, const unsigned long long* npk, int mode, double* tdave, size_t max_cells,
                                hipStream_t s) {
   const unsigned nm = (unsigned)L.bd.nmodels;

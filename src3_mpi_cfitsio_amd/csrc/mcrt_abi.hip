@@ -42,6 +42,10 @@ struct mcrt_ctx {
   const double *acc_ldint = nullptr, *acc_hdisk = nullptr;
   size_t ncell = 0, nsed = 0, npsed = 0, nimage = 0;
   int64_t trace_n = 0;
+  // diffusion approximation (mcrt_set_diffusion, mcrt_setdiffus)
+  bool diff_on = false, diff_set = false;
+  int diff_nr0 = 3;
+  double diff_rmaxd = 0.0, diff_delta_run = 0.0;
   // Lucy convergence (mcrt_dusttemp_sigma): sums of the last mcrt_dusttemp,
   // sigma of the last call that updated tdust
   double lucy_sum[3] = {0.0, 0.0, 0.0};
@@ -198,6 +202,8 @@ int ensure_tallies(mcrt_ctx* c) {
   c->dt.image = (float*)p;
   if ((rc = dev_alloc(c, "t.counters", C_NCOUNTERS * 8, &p))) return rc;
   c->dt.counters = (unsigned long long*)p;
+  if ((rc = dev_alloc(c, "t.diffcount", c->ncell * 4, &p))) return rc;
+  c->dt.diff_count = (unsigned*)p;
   return MCRT_OK;
 }
 
@@ -212,6 +218,7 @@ int zero_tallies(mcrt_ctx* c) {
   HIPCHECK(c, launch_fill_f32(c->dt.psed, c->npsed, 0.f, c->stream));
   HIPCHECK(c, launch_fill_f32(c->dt.image, c->nimage, 0.f, c->stream));
   HIPCHECK(c, launch_fill_u64(c->dt.counters, C_NCOUNTERS, 0ull, c->stream));
+  HIPCHECK(c, launch_fill_i32((int*)c->dt.diff_count, c->ncell, 0, c->stream));
   return MCRT_OK;
 }
 
@@ -834,6 +841,20 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.isrf_logfreq = c->isrf_logfreq;
     dm.isrf_from = (unsigned long long)(npackets - npout);
   }
+  if (c->diff_on) {
+    if (!c->diff_set) return fail(c, MCRT_E_STATE, "mcrt_run_packets: diffusion is on: call mcrt_setdiffus first");
+    // ns = np - npacc - npout, the stellar photons (newdisktherm_5_fuv.f90:158-190),
+    // whose count scales the diffusion surface temperatures
+    int64_t nout = 0;
+    if (c->nisrf > 0 && c->isrf_frac > 0.0) {
+      nout = (int64_t)(c->isrf_frac * (double)npackets);
+      if (c->isrf_frac * (double)npackets - (double)nout > 0.5) nout += 1;
+    }
+    const int64_t ns = npackets - (int64_t)(c->accfrac * (double)npackets) - nout;
+    if (ns < 1) return fail(c, MCRT_E_ARG, "mcrt_run_packets: no stellar photons to scale the diffusion surfaces");
+    dm.diff_delta = 4.0 * kPi * std::pow(c->dm.tstar, 4.0) / (double)ns;
+    c->diff_delta_run = dm.diff_delta;
+  }
   if (c->spot.ispot) {
     const DeviceModel& sp = c->spot;
     dm.ispot = 1; dm.nspot = sp.nspot; dm.fspot = sp.fspot; dm.csprad = sp.csprad; dm.tshock = sp.tshock;
@@ -989,6 +1010,9 @@ int mcrt_adjoint_image(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint
   // the replay runs stellar photons only: a context whose mcrt_run_packets
   // would emit accretion, hot-spot or external photons describes another
   // forward model than the one the gradient would be of
+  if (c->diff_on)
+    return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: the diffusion approximation's surface temperatures depend on the "
+                                 "order photons arrive in, which a replay cannot reproduce; turn it off (mcrt_set_diffusion)");
   if (c->accfrac > 0.0 || c->spot.ispot || (c->nisrf > 0 && c->isrf_frac > 0.0))
     return fail(c, MCRT_E_STATE, "mcrt_adjoint_image: accretion, hot-spot and external photons are not part of the "
                                  "replayed model; turn them off (mcrt_set_accretion, mcrt_set_spots, mcrt_set_isrf)");
@@ -1075,6 +1099,7 @@ BatchLaunch batch_launch(mcrt_ctx* c, int block, bool combine) {
     L.m.phi_tol2 = e * e;
   }
   L.m.species_opac = c->b_species_opac;
+  L.m.diff_itmin = nullptr;            // the diffusion approximation is a single-model option
   L.t = c->bt;
   L.bd = c->bd;
   L.run = c->b_run;
@@ -1510,13 +1535,23 @@ int mcrt_dusttemp(mcrt_ctx* c, double nphot, int32_t mode, int32_t update_tdust,
   int rc = dev_alloc(c, "tdave", c->ncell * 8, &p);
   if (rc) return rc;
   double* d_tdave = (double*)p;
+  const double* told = c->dm.tdust;
+  if (c->diff_on && c->diff_set && c->dm.tdust) {
+    // the temperatures the run's diffusion photons left in the diffusion
+    // columns (they stand for the reference's in-run updates of tdust)
+    DiffArgs a{(const double*)c->buf["diff.kappar"], (const double*)c->buf["diff.hdisk"], nullptr, c->diff_rmaxd,
+               c->diff_nr0, 0, (int32_t*)c->buf["diff.itmin"]};
+    HIPCHECK(c, launch_diff_columns(c->dm, a, c->dt.diff_count, c->dt.nabs, c->diff_delta_run, (double*)c->buf["tdust"],
+                                    c->stream));
+    told = (const double*)c->buf["diff.told"];
+  }
   HIPCHECK(c, launch_tfinal(c->dm, c->dt.dtauabs, nphot, mode, d_tdave, c->stream));
   // tfinal's convergence statistic against the temperatures this run
   // re-emitted with (the previous iteration's tdave)
-  if (c->dm.tdust) {
+  if (told) {
     void* q;
     if ((rc = dev_alloc(c, "lucy.sums", 3 * 8, &q))) return rc;
-    HIPCHECK(c, launch_lucy_sigma(c->dm, c->dm.tdust, d_tdave, (double*)q, c->stream));
+    HIPCHECK(c, launch_lucy_sigma(c->dm, told, d_tdave, (double*)q, c->stream));
     HIPCHECK(c, hipMemcpyAsync(c->lucy_sum, q, 3 * 8, hipMemcpyDeviceToHost, c->stream));
   }
   if (update_tdust) {
@@ -1536,6 +1571,78 @@ int mcrt_dusttemp(mcrt_ctx* c, double nphot, int32_t mode, int32_t update_tdust,
   c->lucy_sigma_prev = c->lucy_sigma_iter;
   if (update_tdust) c->lucy_sigma_iter = c->lucy_sigma;
   c->lucy_valid = true;
+  return MCRT_OK;
+}
+
+int mcrt_set_diffusion(mcrt_ctx* c, int32_t on, const double* kappar, const double* hdisk, double rmaxd, int32_t nr0) {
+  if (!c) return MCRT_E_ARG;
+  if (!on) {
+    c->diff_on = c->diff_set = false;
+    c->dm.diff_itmin = nullptr;
+    return MCRT_OK;
+  }
+  if (!(c->has_grid && c->has_dust && c->has_medium && c->has_source))
+    return fail(c, MCRT_E_STATE, "mcrt_set_diffusion: set the grid, medium, dust and source first");
+  if (!kappar || !hdisk) return fail(c, MCRT_E_ARG, "mcrt_set_diffusion: null array");
+  if (nr0 < 1 || !(rmaxd > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_diffusion: nr0 >= 1 and rmaxd > 0");
+  if ((c->dm.ntg - 1) % 2 != 0) return fail(c, MCRT_E_STATE, "mcrt_set_diffusion: theta diffusion needs an even number of theta cells");
+  const size_t nk = (size_t)c->dm.nopac * c->dm.nT, nr = (size_t)c->dm.nrg - 1;
+  for (size_t i = 0; i < nk; ++i)
+    if (!(kappar[i] > 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_diffusion: kappar must be > 0");
+  for (size_t i = 0; i < nr; ++i)
+    if (!(hdisk[i] >= 0.0)) return fail(c, MCRT_E_ARG, "mcrt_set_diffusion: hdisk must be >= 0");
+  HIPCHECK(c, hipSetDevice(c->device));
+  const double* d;
+  int rc = upload(c, "diff.kappar", kappar, nk, &d);
+  if (!rc) rc = upload(c, "diff.hdisk", hdisk, nr, &d);
+  void* p;
+  if (!rc) rc = dev_alloc(c, "diff.itmin", nr * 4, &p);
+  if (!rc) rc = dev_alloc(c, "diff.told", c->ncell * 8, &p);
+  if (rc) return rc;
+  HIPCHECK(c, launch_fill_i32((int*)c->buf["diff.itmin"], nr, 0, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  c->diff_on = true;
+  c->diff_set = false;
+  c->diff_nr0 = nr0;
+  c->diff_rmaxd = rmaxd;
+  c->dm.diff_itmin = nullptr;
+  return MCRT_OK;
+}
+
+int mcrt_setdiffus(mcrt_ctx* c, int32_t iter, int64_t* ncells) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->diff_on) return fail(c, MCRT_E_STATE, "mcrt_setdiffus: diffusion is off (mcrt_set_diffusion)");
+  if (!c->has_tdust || !c->dm.tdust) return fail(c, MCRT_E_STATE, "mcrt_setdiffus: set the dust temperatures first");
+  HIPCHECK(c, hipSetDevice(c->device));
+  // the last Lucy update's temperatures, else the ones the run would re-emit with
+  auto tv = c->buf.find("tdave");
+  const double* tdave = (tv != c->buf.end() && c->lucy_valid) ? (const double*)tv->second : c->dm.tdust;
+  DiffArgs a{(const double*)c->buf["diff.kappar"], (const double*)c->buf["diff.hdisk"], tdave, c->diff_rmaxd, c->diff_nr0,
+             (int)iter, (int32_t*)c->buf["diff.itmin"]};
+  HIPCHECK(c, launch_setdiffus(c->dm, a, c->stream));
+  c->dm.diff_itmin = a.itmin;
+  // the temperatures before the reset: tfinal's convergence test compares with them
+  HIPCHECK(c, hipMemcpyAsync(c->buf["diff.told"], c->dm.tdust, c->ncell * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHECK(c, launch_diff_reset(c->dm, (double*)c->buf["tdust"], c->stream));
+  std::vector<int32_t> itmin((size_t)c->dm.nrg - 1);
+  HIPCHECK(c, hipMemcpyAsync(itmin.data(), a.itmin, itmin.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  int64_t n = 0;
+  for (int32_t v : itmin)
+    if (v > 0) n += (int64_t)(c->dm.ntg - 2 * v + 1) * (c->dm.npg - 1);
+  if (ncells) *ncells = n;
+  c->diff_set = true;
+  return MCRT_OK;
+}
+
+int mcrt_get_diffusion(mcrt_ctx* c, int32_t* itmin, uint32_t* count) {
+  if (!c) return MCRT_E_ARG;
+  if (!c->diff_on || !c->diff_set) return fail(c, MCRT_E_STATE, "mcrt_get_diffusion: no diffusion region set");
+  HIPCHECK(c, hipSetDevice(c->device));
+  if (itmin) HIPCHECK(c, hipMemcpyAsync(itmin, c->buf["diff.itmin"], ((size_t)c->dm.nrg - 1) * 4, hipMemcpyDeviceToHost, c->stream));
+  if (count && c->dt.diff_count)
+    HIPCHECK(c, hipMemcpyAsync(count, c->dt.diff_count, c->ncell * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
   return MCRT_OK;
 }
 

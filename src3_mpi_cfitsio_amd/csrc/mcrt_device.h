@@ -92,6 +92,14 @@ struct DeviceModel {
   unsigned long long acc_to;
   const double *acc_ldint, *acc_hdisk;
   const struct AccStart* acc_start;   // [acc_to], written by k_acc_place
+  // diffusion approximation (setdiffus_5.f90, emit_common_2.f90:88-378):
+  // diff_itmin[ir-1] is the first diffusion cell of radial column ir counted
+  // from the pole (0: none); cells diff_itmin..ntg-diff_itmin of the column
+  // diffuse, the upper half toward the pole above, the lower half toward the
+  // pole below.  Null: diffusion off.  diff_delta = 4 pi Tstar^4 / ns, the
+  // surface temperature increment per photon times the surface area.
+  const int32_t* diff_itmin;
+  double diff_delta;
 };
 
 // where k_acc_place put accretion photon k: position, cell, species and the
@@ -110,10 +118,18 @@ struct DeviceTallies {
   float *sed, *psed, *image;
   unsigned long long* counters;  // C_NCOUNTERS
   double* trace;                 // optional: 8 doubles per packet
+  unsigned* diff_count;          // photons re-emitted from each diffusion surface cell (diffusion on)
 };
 
 __device__ __forceinline__ size_t cell_index(const DeviceModel& m, int ir, int it, int ip) {
   return (size_t)(ir - 1) + (size_t)(m.nrg - 1) * ((size_t)(it - 1) + (size_t)(m.ntg - 1) * (size_t)(ip - 1));
+}
+
+// cell (ir, it) lies in the diffusion region of its column
+__device__ __forceinline__ bool diffusing(const DeviceModel& m, int ir, int it) {
+  if (!m.diff_itmin) return false;
+  const int a = m.diff_itmin[ir - 1];
+  return a > 0 && it >= a && it <= m.ntg - a;
 }
 
 // ---------------------------------------------------------------------------

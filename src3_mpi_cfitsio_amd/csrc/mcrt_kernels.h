@@ -93,6 +93,17 @@ __global__ void __launch_bounds__(256) k_mueller(int n, int kind, const double* 
   for (int i = 0; i < 4; ++i) s[4 * r + i] = (double)st[i];
 }
 
+// Arguments of the diffusion kernels.
+struct DiffArgs {
+  const double* kappar;      // [nopac * nT] Rosseland means (setemiss)
+  const double* hdisk;       // [nrg - 1] disk scale height (R*) per radial cell
+  const double* tdave;       // temperatures of the last Lucy update (K/11605)
+  double rmaxd;              // disk outer radius (R*)
+  int nr0;                   // first radial cell that may diffuse
+  int iter;                  // Lucy iteration (below 2: temperatures floored at 50 K)
+  int32_t* itmin;            // [nrg - 1] out: first diffusion cell from the pole, 0 none
+};
+
 #ifdef MCRT_FP64_ONLY_KERNELS
 // ---------------------------------------------------------------------------
 // Lucy temperature (FP64): dusttemp_5.f90:8-152 with zero velocities,
@@ -143,7 +154,12 @@ __device__ __forceinline__ void tfinal_half_at(const DeviceModel& m, const doubl
   size_t c = cell_index(m, ir, it, 1), cm = cell_index(m, ir, m.ntg - it, 1);
   double t = 0.1 / 11605.0;
   int sp = m.species[c];
-  if (sp > 0 && m.rho64[c] > 0.0) {
+  if (sp > 0 && m.rho64[c] > 0.0 && diffusing(m, ir, it)) {
+    // a diffusion cell keeps the temperatures the run's diffusion photons
+    // left (tfinal_5_test1.f90:232-233)
+    t = (m.tdust[c] + m.tdust[cm]) / 2.0;
+    if (t * 11605.0 >= kTmaxK) t = kTmaxK / 11605.0;
+  } else if (sp > 0 && m.rho64[c] > 0.0) {
     int iop = m.species_opac[sp];
     double rho2 = (m.species[cm] == sp) ? m.rho64[cm] : 0.0;
     double mcell1 = pow(m.rho64[c] * m.vcell[c], 0.25);
@@ -171,7 +187,10 @@ __device__ __forceinline__ void tfinal_fill_at(const DeviceModel& m, double* tda
 __device__ __forceinline__ void tcell_at(const DeviceModel& m, const double* dtauabs, double nphot, double* tdave, size_t c) {
   double t = 0.1 / 11605.0;
   int sp = m.species[c];
-  if (sp > 0 && m.rho64[c] > 0.0) {
+  const int nr = m.nrg - 1, nt = m.ntg - 1;
+  if (sp > 0 && m.rho64[c] > 0.0 && diffusing(m, (int)(c % nr) + 1, (int)((c / nr) % nt) + 1)) {
+    t = m.tdust[c];
+  } else if (sp > 0 && m.rho64[c] > 0.0) {
     int iop = m.species_opac[sp];
     t = dusttemp_dev(m, dtauabs[c], nphot, iop, pow(m.rho64[c] * m.vcell[c], 0.25));
     if (t * 11605.0 >= kTmaxK) t = kTmaxK / 11605.0;
@@ -195,6 +214,117 @@ __global__ void __launch_bounds__(256) k_tcell(DeviceModel m, const double* dtau
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (size_t)(m.nrg - 1) * (m.ntg - 1) * (m.npg - 1)) return;
   tcell_at(m, dtauabs, nphot, tdave, c);
+}
+
+// chiR (dust_5.f90:112-141): the Rosseland mean of table iop at T,
+// linear in log T between the nT table temperatures (clamped to the table)
+__device__ __forceinline__ double chiR_dev(const DeviceModel& m, const double* kappar, double T, int iop) {
+  const int nT = m.nT;
+  const double lT = log(T);
+  int k = (int)((double)(nT - 1) * (lT - m.lTint[0]) / (m.lTint[nT - 1] - m.lTint[0])) + 1;
+  if (k < 1) k = 1;
+  if (k > nT - 1) k = nT - 1;
+  double f = (lT - m.lTint[k - 1]) / (m.lTint[k] - m.lTint[k - 1]);
+  const double* kr = kappar + (size_t)iop * nT;
+  return (1.0 - f) * kr[k - 1] + f * kr[k];
+}
+
+
+// setdiffus_5.f90:36-103 for cell it of column ir (one thread each, it in
+// 2..(ntg-1)/2): the Rosseland optical depth from the cell toward the pole
+// and the mean free path at depth 1; the column diffuses from it to the
+// midplane when the depth reaches 1 within less than a twentieth of the
+// disk scale height.  The column's diffusion region starts at the smallest
+// such it (atomicMin: the same result in any order).
+__global__ void __launch_bounds__(256) k_setdiffus(DeviceModel m, DiffArgs a) {
+  const int nr = m.nrg - 1, nh = (m.ntg - 1) / 2;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * (nh - 1)) return;
+  const int ir = idx % nr + 1, it = idx / nr + 2;
+  const double rad = m.rarr[ir - 1];
+  if (ir < a.nr0 || !(rad < a.rmaxd)) return;
+  const double kapd_cm = m.rstar * kRsol;           // kapd = kappav * rmincgs (gridset_5_model13new.f90:79)
+  double tau = 0.0, tau1 = 0.0;
+  int itt = it;
+  while (tau < 1.0 && itt > 1) {
+    const double dt = m.thetarr[itt] - m.thetarr[itt - 1];
+    tau1 = tau;
+    const size_t c = cell_index(m, ir, itt, 1);
+    double tt = a.tdave[c];
+    if (a.iter < 2) tt = fmax(tt, 50.0 / 11605.0);
+    const int sp = m.species[c];
+    if (sp > 0 && m.rho64[c] > 0.0) {
+      const int iop = m.species_opac[sp];
+      tau = tau + m.kappav[iop] * kapd_cm * chiR_dev(m, a.kappar, tt, iop) * m.rho64[c] * dt * rad;
+    }
+    itt = itt - 1;
+  }
+  double lmfp1;
+  if (tau > 1.0) lmfp1 = rad * (m.thetarr[itt + 1] - m.thetarr[itt]) / (tau - tau1) * (1.0 - tau1);
+  else lmfp1 = rad * (m.thetarr[itt + 1] - m.thetarr[itt]);
+  const double lmfp = rad * (m.thetarr[it] - m.thetarr[itt + 1]) + lmfp1;
+  if (tau >= 1.0 && lmfp < a.hdisk[ir - 1] / 20.0) atomicMin(&a.itmin[ir - 1], it);
+}
+
+__global__ void __launch_bounds__(256) k_itmin_none(int32_t* itmin, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && itmin[i] == 0x7fffffff) itmin[i] = 0;
+}
+
+// gridset_5_model13new.f90:1857-1868: a diffusion cell starts the run at the
+// 0.1 K floor; its temperature is then built by the run's diffusion photons.
+__global__ void __launch_bounds__(256) k_diff_reset(DeviceModel m, double* tdust) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nr = m.nrg - 1, nt = m.ntg - 1;
+  if (c >= (size_t)nr * nt * (m.npg - 1)) return;
+  if (diffusing(m, (int)(c % nr) + 1, (int)((c / nr) % nt) + 1)) tdust[c] = 0.1 / 11605.0;
+}
+
+// The temperatures the reference's diffusion photons leave in a column
+// (emit_common_2.f90:256-341) after the run, one thread per (ir, ip,
+// hemisphere): the surface cell's T^4 = T0^4 + N Delta / dA for its N
+// photons, then the interior integrated toward the midplane with the
+// accretion photons absorbed below (nabs; none without accretion, when the
+// whole region takes the surface temperature).
+__global__ void __launch_bounds__(256) k_diff_columns(DeviceModel m, const DiffArgs a, const unsigned* count,
+                                                      const int* nabs, double delta, double* tdust) {
+  const int nr = m.nrg - 1, np = m.npg - 1;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nr * np * 2) return;
+  const int ir = idx % nr + 1, ip = (idx / nr) % np + 1;
+  const bool upper = idx / (nr * np) == 0;
+  const int am = m.diff_itmin[ir - 1];
+  if (am <= 0) return;
+  const int ntg = m.ntg, mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2;
+  const int its = upper ? am : ntg - am, step = upper ? 1 : -1;
+  const int w = upper ? its : its + 1;
+  const double sth = sin(m.thetarr[w - 1]);
+  const double dA = kPi * sth * (m.r2arr[ir] - m.r2arr[ir - 1]);
+  const double Tcap = kTmaxK / 11605.0, T0 = 0.1 / 11605.0;
+  const size_t cs = cell_index(m, ir, its, ip);
+  const double T4 = pow4d(T0) + (double)count[cs] * delta / dA;
+  double Ts = pow(T4, 0.25);
+  if (Ts > Tcap) Ts = Tcap;
+  if (!(m.species[cs] > 0 && m.rho64[cs] > 0.0)) Ts = T0;
+  tdust[cs] = Ts;
+  double nacc = 0.0;
+  for (int i = its; upper ? i <= mid : i >= mid; i += step) nacc += (double)nabs[cell_index(m, ir, i, ip)];
+  double T4d = T4;
+  const double rc = 0.5 * (m.rarr[ir - 1] + m.rarr[ir]);
+  for (int i = its + step; upper ? i <= mid : i >= mid; i += step) {
+    const int ib = i - step;                                   // the cell above toward the surface
+    const size_t cb = cell_index(m, ir, ib, ip);
+    const int sp = m.species[cb];
+    if (sp > 0 && m.rho64[cb] > 0.0) {
+      const int iop = m.species_opac[sp];
+      const double dth = upper ? m.thetarr[i - 1] - m.thetarr[i - 2] : m.thetarr[i + 1] - m.thetarr[i];
+      T4d = T4d + 0.75 * delta / dA * nacc * m.kappav[iop] * kRsol * m.rstar * m.rho64[cb] *
+                      chiR_dev(m, a.kappar, tdust[cb], iop) * rc * dth;
+      double Ti = pow(T4d, 0.25);
+      tdust[cell_index(m, ir, i, ip)] = Ti > Tcap ? Tcap : Ti;
+      nacc = nacc - (double)nabs[cb];
+    }
+  }
 }
 
 // Convergence statistic of tfinal (tfinal_5_test1.f90:334-368): over the

@@ -57,6 +57,11 @@ hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s);
 // model cell arrays: k_build_cells, k_build_mass, k_build_finish (FP64, contraction off)
 hipError_t launch_build(const BuildArgs& a, hipStream_t s);
 hipError_t launch_lucy_sigma(const DeviceModel& m, const double* told, const double* tnew, double* out, hipStream_t s);
+struct DiffArgs;
+hipError_t launch_setdiffus(const DeviceModel& m, const DiffArgs& a, hipStream_t s);
+hipError_t launch_diff_reset(const DeviceModel& m, double* tdust, hipStream_t s);
+hipError_t launch_diff_columns(const DeviceModel& m, const DiffArgs& a, const unsigned* count, const int* nabs,
+                               double delta, double* tdust, hipStream_t s);
 hipError_t launch_tfinal(const DeviceModel& m, const double* dtauabs, double nphot, int mode, double* tdave, hipStream_t s);
 hipError_t launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 hipError_t launch_fill_f64(double* p, size_t n, double v, hipStream_t s);

@@ -1035,6 +1035,75 @@ __device__ __forceinline__ void reemit(Ctx<T>& C, Pkt<T>& p, T tave, int iop) {
   set_dir<T>(p);
 }
 
+// A photon absorbed in a diffusion cell (emit_common_2.f90:226-378, theta
+// diffusion).  It is moved at fixed radius and azimuth to just outside the
+// wall between the diffusion region and the free medium above (upper
+// half) or below it (lower half), emitted into the outer hemisphere with
+// a cosine law (isotrp.f90), and its frequency drawn from a blackbody at
+// the surface temperature.  That temperature grows with every photon the
+// surface cell re-emits: the reference adds 4 pi T*^4 / (ns dA) to T^4
+// per photon in the order the photons arrive; here the arrival order on
+// the GPU is an atomic counter per surface cell, the k-th photon seeing
+// T0^4 + (k+1) Delta / dA.  The accretion photons absorbed at the midplane
+// add 0.75 mu T4eff (grey atmosphere).  The Stokes vector is left as the
+// absorbed photon had it, as the reference leaves it.  The packet ends in
+// the free cell next to the surface.
+template <class T>
+__device__ __forceinline__ void diffuse(Ctx<T>& C, Pkt<T>& p) {
+  const DeviceModel& m = C.m;
+  const int ntg = m.ntg;
+  const int am = m.diff_itmin[p.ir - 1];
+  const bool upper = p.it <= (ntg - 1) / 2;
+  const int its = upper ? am : ntg - am;            // the surface diffusion cell
+  const int w = upper ? its : its + 1;               // its wall toward the free medium (1-based)
+  const double cw = m.costarr[w - 1], sw = sin(m.thetarr[w - 1]);
+  // move (emit_common_2.f90:230-237)
+  const double cosb = p.zp / p.rtot;
+  const double sinb = sqrt(1.0 - cosb * cosb);
+  const double cosbnew = 1.00001 * cw;
+  const double sinbnew = sqrt(1.0 - cosbnew * cosbnew);
+  p.xp = p.xp * sinbnew / sinb;
+  p.yp = p.yp * sinbnew / sinb;
+  p.zp = p.rtot * cosbnew;
+  // outward normal of the surface (:240-249) and a cosine-law direction about it (isotrp.f90)
+  double sux, suy, suz;
+  if (cw > 0.0) { sux = -cw * p.xp / sw / p.rtot; suy = -cw * p.yp / sw / p.rtot; suz = sw; }
+  else { sux = cw * p.xp / sw / p.rtot; suy = cw * p.yp / sw / p.rtot; suz = -sw; }
+  double x = 0.0, y = 0.0, z = 0.0, r2 = 2.0, mur = 0.0;
+  while (r2 * r2 > fabs(mur)) {
+    x = (double)uniform<T>(C.rng); x = x + x - 1.0;
+    y = (double)uniform<T>(C.rng); y = y + y - 1.0;
+    z = (double)uniform<T>(C.rng); z = z + z - 1.0;
+    mur = x * sux + y * suy + z * suz;
+    r2 = x * x + y * y + z * z;
+  }
+  double r = sqrt(r2);
+  if (mur < 0.0) r = -r;
+  const double ux = x / r, uy = y / r, uz = z / r;
+  p.cost = (T)uz;
+  const double sint = sqrt(1.0 - uz * uz);
+  p.sint = (T)sint;
+  p.cosp = (T)(ux / sint);
+  p.sinp = (T)(uy / sint);
+  p.phi = (T)atan2(uy / sint, ux / sint);
+  // surface temperature after this photon (:258-270) and the emission temperature (:343-361)
+  const double dA = kPi * sw * (m.r2arr[p.ir] - m.r2arr[p.ir - 1]);
+  const size_t cs = cell_index(m, p.ir, its, p.ip);
+  const unsigned k = atomicAdd(&C.t.diff_count[cs], 1u);
+  const double Tcap = kTmaxK / 11605.0, T0 = 0.1 / 11605.0;
+  double Ts = pow(T0 * T0 * T0 * T0 + (double)(k + 1u) * m.diff_delta / dA, 0.25);
+  if (Ts > Tcap) Ts = Tcap;
+  const int mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2;
+  const double nacc = (double)*(volatile const int*)&C.t.nabs[cell_index(m, p.ir, mid, p.ip)];
+  const double T4eff = m.diff_delta * nacc / dA;
+  const double mu = ux * sux + uy * suy + uz * suz;
+  double Te = pow(Ts * Ts * Ts * Ts + 0.75 * mu * T4eff, 0.25);
+  if (Te > Tcap) Te = Tcap;
+  p.nub = bbfreq<T>(C.rng, (T)Te, C.inczeta);
+  p.it = upper ? its - 1 : its + 1;
+  set_dir<T>(p);
+}
+
 template <class T, bool kAdj = false>
 __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
@@ -1054,8 +1123,13 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   } else {
     atomicAdd(&t.dbig[c], 1);
     p.i_orig = start ? 2 : i_orig_of(L.idust2);
-    atomicAdd(&t.nabs[c], 1);
-    reemit<T>(C, p, cell_tdust<T>(m, L, c), iop);
+    // propagate_5_fuv.f90:139: a diffusion cell counts only the accretion
+    // photons placed in it (initpacc_4.f90); its absorptions re-emit from
+    // the diffusion surface (emit_common_2.f90:88)
+    const bool diff = diffusing(m, p.ir, p.it);
+    if (start || !diff) atomicAdd(&t.nabs[c], 1);
+    if (diff) diffuse<T>(C, p);
+    else reemit<T>(C, p, cell_tdust<T>(m, L, c), iop);
     p.iflag = 0;
     opacset<T>(m, p.nub, p.op);
     p.wave = (T)1.2398 / p.nub;

@@ -108,7 +108,38 @@ class Engine:
         self.set_isrf(m.isrf_frac, m.isrf)
         self.set_spots(m.spots)
         self.set_accretion(m.accretion)
+        self.set_diffusion(m.diffusion)
         self._check(L.mcrt_zero_tallies(self.h))
+        self._iter = 0
+
+    def set_diffusion(self, diffusion=None):
+        """The diffusion approximation (dict as model.with_diffusion makes it:
+        kappar, hdisk, rmaxd, nr0); None turns it off."""
+        if not diffusion:
+            self._check(self.lib.mcrt_set_diffusion(self.h, 0, None, None, 0.0, 0))
+            self._diffusion = False
+            return
+        kr, hd = _f64(diffusion["kappar"]), _f64(diffusion["hdisk"])
+        d = C.c_double
+        self._check(self.lib.mcrt_set_diffusion(self.h, 1, ptr(kr, d), ptr(hd, d), float(diffusion["rmaxd"]),
+                                                int(diffusion["nr0"])))
+        self._diffusion = True
+
+    def setdiffus(self, iteration: int) -> int:
+        """setdiffus for Lucy iteration `iteration` (mcrt_setdiffus):
+        flags the diffusion region from the last Lucy temperatures and
+        restarts its dust at 0.1 K; returns the number of diffusion cells."""
+        n = C.c_int64()
+        self._check(self.lib.mcrt_setdiffus(self.h, int(iteration), C.byref(n)))
+        return int(n.value)
+
+    def diffusion_state(self):
+        """(itmin per radial cell, photons re-emitted per surface cell in cell_shape)."""
+        m = self.model
+        itmin = np.zeros(m.nrg - 1, np.int32)
+        cnt = np.zeros(m.cell_shape, np.uint32)
+        self._check(self.lib.mcrt_get_diffusion(self.h, ptr(itmin, C.c_int32), ptr(cnt, C.c_uint32)))
+        return itmin, cnt
 
     def set_accretion(self, accretion=None):
         """Disk-accretion photons (dict as model.with_accretion makes it:
@@ -236,12 +267,18 @@ class Engine:
         return dict(sigma=sig.value, sigma_prev=prev.value, ncells=int(n.value), converge=bool(conv.value))
 
     def disk(self, npackets: int, seed: int, precision: int = MCRT_FP64,
-             tmode: int = MCRT_T_REFERENCE, first_packet: int = 0) -> Dict[str, np.ndarray]:
-        """One Lucy iteration (disk, newdisktherm_5_fuv.f90:1): zero tallies,
-        run every packet, read the reduced tallies, and invert the absorbed
-        energy to a new dust temperature that the next iteration re-emits
-        with; tal["converge"] is tfinal's verdict ("yes"/"no") and
-        tal["sigma"] its statistic."""
+             tmode: int = MCRT_T_REFERENCE, first_packet: int = 0, iteration: Optional[int] = None
+             ) -> Dict[str, np.ndarray]:
+        """One Lucy iteration (disk, newdisktherm_5_fuv.f90:1): with the
+        diffusion approximation on, setdiffus for this iteration (numbered
+        from 1 by default, as gridset calls it); zero tallies, run every
+        packet, read the reduced tallies, and invert the absorbed energy to
+        a new dust temperature that the next iteration re-emits with;
+        tal["converge"] is tfinal's verdict ("yes"/"no") and tal["sigma"]
+        its statistic."""
+        self._iter = self._iter + 1 if iteration is None else int(iteration)
+        if self._diffusion:
+            self.setdiffus(self._iter)
         self.zero_tallies()
         self.run_packets(npackets, seed, precision, first_packet=first_packet)
         tal = self.tallies()
@@ -260,7 +297,7 @@ class Engine:
         hist, it, converge, tal = [], 0, "no", None
         while converge == "no":
             it += 1
-            tal = self.disk(npackets, seed, precision, tmode, first_packet=it * npackets)
+            tal = self.disk(npackets, seed, precision, tmode, first_packet=it * npackets, iteration=it)
             converge = tal["converge"]
             hist.append((tal["sigma"], converge))
             if it < n_iter_min:

@@ -117,6 +117,11 @@ class Model:
     # disk-accretion photons (initpacc_4.f90): accfrac (the fraction of the
     # packets), ldint and hdisk (grid_mod's Ldint and hdisk, nrg - 1 each)
     accretion: Optional[Dict[str, Any]] = dataclasses.field(default=None, repr=False, compare=False)
+    # Rosseland means of each opacity table (nopac, nT), as setemiss leaves kappar
+    kappar: Optional[np.ndarray] = dataclasses.field(default=None, repr=False, compare=False)
+    # the diffusion approximation (setdiffus_5.f90): kappar, hdisk (nrg - 1,
+    # R*), rmaxd (R*), nr0
+    diffusion: Optional[Dict[str, Any]] = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def ncell(self) -> int:
@@ -368,6 +373,14 @@ def planck_bx(x):
         return np.where(x == 0.0, 0.0, 0.1539897 * x * x * x / expm1_as(x))
 
 
+def planck_dbx(x):
+    """dBnu of mathsubs.f90:29-47: (15/pi^4) x^4 e^x / (4 (e^x - 1)^2), the
+    temperature derivative of planck_bx normalized to unit integral."""
+    x = np.asarray(x, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(x == 0.0, 0.0, -0.1539897 * x ** 4 / (4.0 * expm1_as(x) * expm1_as(-x)))
+
+
 def _opacset_vec(nu, nudust, tab):
     """dust_5.f90:21-55 linear interpolation in nu, clamped at the table ends."""
     return np.interp(nu, nudust[::-1], tab[::-1])
@@ -397,8 +410,15 @@ def emission_tables(dust, nT=500, nnu=513, Tmin=0.1, Tmax=200000.0, nodes=8):
     kappap = cum[:, -1].copy()
     kapint = cum / kappap[:, None]
     kapint[:, -1] = 1.0
+    # kappar: 1 / sum over the intervals of (dB/dT)/kappa_ext (chidB,
+    # dust_5.f90:814-826), the Rosseland mean of kappa/kappav
+    kext = lambda nu: _opacset_vec(nu, dust["nudust"], dust["kappad"])    # noqa: E731
+    e0, e1 = kext(x0[None, :] * Tint[:, None]), kext(x1[None, :] * Tint[:, None])
+    ef = (1.0 - fg)[None, None, :] * e0[:, :, None] + fg[None, None, :] * e1[:, :, None]
+    dB = planck_dbx(xs)
+    kappar = 1.0 / np.sum(dx[None, :] * np.sum(dB[None, :, :] / ef * wg[None, None, :], axis=2), axis=1)
     return dict(nT=nT, nnu=nnu, lTint=lTint, Tint=Tint, nuint=nuint,
-                kapint=np.ascontiguousarray(kapint.T), kappap=kappap)
+                kapint=np.ascontiguousarray(kapint.T), kappap=kappap, kappar=kappar)
 
 
 # ---------------------------------------------------------------------------
@@ -566,7 +586,24 @@ def with_dusts(m: Model, dusts, r_split_au: float = 10.0) -> Model:
         m, species=species, species_opac=so, nopac=nd, nlam=nlam, nudust=st("nudust"), kappad=st("kappad"),
         adust=st("adust"), gdust=st("gdust"), pldust=st("pldust"),
         kappav=np.array([d["kappav"] for d in dusts], np.float64),
-        kapint=np.stack([e["kapint"] for e in ems]), kappap=np.stack([e["kappap"] for e in ems]))
+        kapint=np.stack([e["kapint"] for e in ems]), kappap=np.stack([e["kappap"] for e in ems]),
+        kappar=np.stack([e["kappar"] for e in ems]))
+
+
+def with_diffusion(m: Model, rin_au: float = 0.1, rout_au: float = 1000.0, h0_au: float = 0.03,
+                   flare: float = 1.25) -> Model:
+    """The model with the diffusion approximation on (DIFFUSION = YES,
+    setdiffus_5.f90): its Rosseland tables, the disk scale height of each
+    radial cell (the flared disk of build(): h0 r^flare, at the cell's mid
+    radius), the disk's outer radius and the first radial cell that may
+    diffuse (1 when the disk reaches the inner wall, else 3)."""
+    if m.kappar is None:
+        raise ValueError("with_diffusion: the model has no Rosseland tables (kappar)")
+    autors = AU / (m.rstar * RSUN)
+    hdisk = accretion_tables(m, rin_au=rin_au, rout_au=rout_au, h0_au=h0_au, flare=flare)["hdisk"]
+    nr0 = 1 if rin_au * autors <= float(m.rarr[0]) * (1.0 + 1e-12) else 3
+    return dataclasses.replace(m, diffusion=dict(kappar=np.ascontiguousarray(m.kappar, np.float64), hdisk=hdisk,
+                                                 rmaxd=float(min(rout_au * autors, m.rarr[-1])), nr0=nr0))
 
 
 def with_isrf(m: Model, isrf_frac: float, nisrf: int = 200) -> Model:
@@ -643,6 +680,7 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
         pldust=dust["pldust"][None, :].copy(), kappav=np.array([dust["kappav"]]),
         nT=em["nT"], nnu=em["nnu"], lTint=em["lTint"], Tint=em["Tint"], nuint=em["nuint"],
         kapint=em["kapint"][None, :, :].copy(), kappap=em["kappap"][None, :].copy(),
+        kappar=em["kappar"][None, :].copy(),
         rstar=rstar, tstar=tstar_k / TK, rmin=1.0,
         ipeel=1 if npeel > 0 else 0, npeel=npeel, thete=thete,
         phie=np.full(npeel, math.radians(phie_deg)), nx=nx, rmaxi=1.1 * rmax,
