@@ -445,7 +445,8 @@ def run_single_workload(args, T0):
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
         "config": {"workload": f"{args.config}: 3-D polarized clumpy disk+Ulrich envelope, "
                                f"{m.nrg-1}x{m.ntg-1}x{m.npg-1} cells, 10 peel-off inclinations, "
-                               f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, one Lucy iteration per step",
+                               f"{m.nx}x{m.nx}x4-Stokes images in {m.nband} bands, dustmat scattering matrix "
+                               f"g={m.gdust[0, 0]:g} pl={m.pldust[0, 0]:g}, one Lucy iteration per step",
                    "packets_per_step": N, "precision": "mixed (FP32 opacities, optical depths, sampling, Stokes; "
                                                        "FP64 geometry and dtauabs)",
                    "steps_requested": args.steps, "warmup_requested": args.warmup,

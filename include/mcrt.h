@@ -248,6 +248,12 @@ int mcrt_run_packets(mcrt_ctx *ctx, int64_t first_packet, int64_t npackets,
  * values. */
 int mcrt_set_launch(mcrt_ctx *ctx, int64_t lanes, int32_t queue_per_lane,
                     int32_t events_per_round, int32_t peel_blocks_per_cu, int32_t regroup);
+/* Gather thresholds of the peel kernel (no counterpart in the reference):
+ * idle lanes of a wavefront that trigger taking new (record, direction)
+ * items with their first crossing, and lanes whose walks reached the
+ * observer that trigger a tally pass.  0 keeps the defaults (4 and 8).
+ * Results do not depend on these values. */
+int mcrt_set_peel_gather(mcrt_ctx *ctx, int32_t fetch_lanes, int32_t tally_lanes);
 
 /* Post-ALLREDUCE tallies (newdisktherm_5_fuv.f90:524-602): any pointer may be
  * NULL.  Layouts: dtauabs/nabs/dbig/killed per cell; sed

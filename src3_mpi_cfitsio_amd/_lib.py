@@ -50,6 +50,7 @@ SIGNATURES = {
     "mcrt_zero_tallies": (C.c_int, [_vp]),
     "mcrt_run_packets": (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_uint64, C.c_int32]),
     "mcrt_set_launch": (C.c_int, [_vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
+    "mcrt_set_peel_gather": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "mcrt_get_tallies": (C.c_int, [_vp, _dp, _ip, _ip, _ip, _fp, _fp, _fp, _dp]),
     "mcrt_dusttemp": (C.c_int, [_vp, C.c_double, C.c_int32, C.c_int32, _dp]),
     "mcrt_dusttemp_sigma": (C.c_int, [_vp, _dp, _dp, C.POINTER(C.c_int64), _ip]),

@@ -60,6 +60,7 @@ struct mcrt_ctx {
   int events_per_round = 0;
   int peel_blocks_per_cu = 0;
   int regroup = 0;
+  int peel_fetch = 0, peel_tally = 0;   // peel-kernel gather thresholds (0: defaults)
   RunCtl* host_ctl = nullptr;             // pinned read-back of the run control block
   hipEvent_t evr[3] = {nullptr, nullptr, nullptr};   // per-round kernel bracketing
   double stats[MCRT_NSTATS] = {0};        // mcrt_run_stats of the last run
@@ -446,6 +447,29 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
   rc |= upload(c, "kapint", kapint, (size_t)nopac * nT * nnu, &m.kapint);
   rc |= upload(c, "kappap", kappap, (size_t)nopac * nT, &m.kappap);
   if (rc) return rc;
+  // guide table of the re-emission CDFs (lucydustfreq_xi): for every row,
+  // the number of entries <= k/K, k = 0..K
+  m.kguide = nullptr;
+  m.kguide_k = 0;
+  if (nnu <= 65535) {
+    const int K = kGuideBuckets;
+    std::vector<uint16_t> g((size_t)nopac * nT * (K + 1));
+    for (int id = 0; id < nopac; ++id)
+      for (int i = 0; i < nT; ++i) {
+        const double* row = kapint + (size_t)id * nT * nnu + i;     // entry j at row[j * nT]
+        uint16_t* out = g.data() + ((size_t)id * nT + i) * (K + 1);
+        int n = 0;
+        for (int k = 0; k <= K; ++k) {
+          const double edge = (double)k / K;
+          while (n < nnu && row[(size_t)n * nT] <= edge) ++n;
+          out[k] = (uint16_t)n;
+        }
+      }
+    rc = upload(c, "kguide", g.data(), g.size(), &m.kguide);
+    if (rc) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));   // g goes out of scope
+    m.kguide_k = K;
+  }
   for (int i = 0; i < kMaxOpac; ++i) m.kappav[i] = (i < nopac) ? kappav[i] : 1.0;
   // opacset's index guess: only for a strictly decreasing, log-uniform nudust
   for (int id = 0; id < kMaxOpac; ++id) {
@@ -911,10 +935,21 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.acc_start = (const AccStart*)p;
   }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
-  uint32_t peel_regroup = 16;
+  // Gather thresholds of the peel kernel.  Lanes whose walk reached the
+  // observer wait for a tally pass and idle lanes for a fetch pass; while
+  // they wait they do not walk, so high thresholds leave most of a
+  // wavefront idle during the walk steps (that take ~84% of the kernel's
+  // cycles).  4 idle / 8 done lanes measured fastest on the 200x180x64
+  // clumpy workload (fetch 16 / tally 32 walked with 27 of 64 lanes).
+  uint32_t peel_regroup = c->peel_fetch > 0 ? (uint32_t)c->peel_fetch : 4u;
+  uint32_t peel_tally = c->peel_tally > 0 ? (uint32_t)c->peel_tally : 8u;
   if (const char* pr = std::getenv("MCRT_PEEL_REGROUP")) {
     const int v = std::atoi(pr);
     if (v >= 1 && v <= 64) peel_regroup = (uint32_t)v;
+  }
+  if (const char* pr = std::getenv("MCRT_PEEL_TALLY")) {
+    const int v = std::atoi(pr);
+    if (v >= 1 && v <= 64) peel_tally = (uint32_t)v;
   }
   double st[MCRT_NSTATS] = {0};
   const char* venv = std::getenv("MCRT_VERBOSE");
@@ -947,6 +982,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     rp.ctl = ctl;
     rp.regroup = (uint32_t)(c->regroup > 0 ? c->regroup : 32);
     rp.peel_regroup = peel_regroup;
+    rp.peel_tally = peel_tally;
     rp.lds = lds;
     // q_count and live restart every round (a fill kernel, not a memset)
     HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
@@ -995,6 +1031,15 @@ int mcrt_set_launch(mcrt_ctx* c, int64_t lanes, int32_t queue_per_lane, int32_t 
   c->queue_per_lane = queue_per_lane;
   c->events_per_round = events_per_round;
   c->peel_blocks_per_cu = peel_blocks_per_cu;
+  return MCRT_OK;
+}
+
+int mcrt_set_peel_gather(mcrt_ctx* c, int32_t fetch_lanes, int32_t tally_lanes) {
+  if (!c) return MCRT_E_ARG;
+  if (fetch_lanes < 0 || fetch_lanes > 64 || tally_lanes < 0 || tally_lanes > 64)
+    return fail(c, MCRT_E_ARG, "mcrt_set_peel_gather: thresholds are 0..64 lanes of a wavefront");
+  c->peel_fetch = fetch_lanes;
+  c->peel_tally = tally_lanes;
   return MCRT_OK;
 }
 

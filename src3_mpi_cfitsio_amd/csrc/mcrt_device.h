@@ -22,6 +22,7 @@
 namespace mcrt {
 
 constexpr int kMaxOpac = 4;     // opacity tables resident per packet
+constexpr int kGuideBuckets = 128;   // buckets of the re-emission CDF guide table
 constexpr int kNo = 8;          // SED components (tts_mod_5.f90:19)
 constexpr double kPi = 3.1415926535897932384626433;
 constexpr double kR2p = 2.0 * 3.1415926535897932384626433;
@@ -55,6 +56,11 @@ struct DeviceModel {
   double kappav[kMaxOpac];
   int nT, nnu;
   const double *lTint, *Tint, *nuint, *kapint, *kappap;
+  // guide table of the re-emission CDFs: kguide[(iop*nT + i)*(kguide_k+1) + k]
+  // = the number of frequency entries of temperature row i with
+  // kapint <= k/kguide_k (null: none)
+  const uint16_t* kguide;
+  int kguide_k;
   const double* tdust;
   // source
   double rstar, tstar, rmin;
@@ -575,7 +581,53 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
   // i1 = i0 + 1.  The same bracket is found here by an 8-way search: each
   // pass probes up to 7 indices with independent loads, so a 513-entry row
   // takes 3 dependent rounds instead of 9.  P0 and P1 are the same values.
+  //
+  // Before that, a guide table (the index table of Chen & Asau, 1974)
+  // narrows the start.  For xi in [k/K, (k+1)/K), every index whose entries
+  // in both rows j and j+1 are <= k/K has P <= xi, and every index whose
+  // entries in both rows exceed (k+1)/K has P > xi; the guide's counts of
+  // entries <= k/K give those indices, widened by one for rounding.  The
+  // range is checked against P at its two ends (P(lo) <= xi < P(hi)), read
+  // in the same pass as 7 points inside it, and the full row is searched
+  // when the check fails, so the bracket is the same either way.
   (void)i;
+  if (m.kguide) {
+    const int nb = m.kguide_k;
+    int k = (int)(xi * (T)nb);
+    k = k < 0 ? 0 : (k > nb - 1 ? nb - 1 : k);
+    const uint16_t* ga = m.kguide + ((size_t)iop * nT + (size_t)(j - 1)) * (size_t)(nb + 1);
+    const uint16_t* gb = ga + (nb + 1);
+    const int a0 = ga[k], b0 = gb[k], a1 = ga[k + 1], b1 = gb[k + 1];
+    int lo = (a0 < b0 ? a0 : b0) - 1, hi = (a1 > b1 ? a1 : b1) + 2;
+    lo = lo < 1 ? 1 : lo;
+    hi = hi > nnu ? nnu : hi;
+    if (hi - lo >= 1 && (lo > 1 || hi < nnu)) {
+      const int step = (hi - lo + 7) >> 3;
+      T Pq[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        int ix = lo + q * step;
+        ix = ix < hi ? ix : hi;
+        T v = ((T)1 - f) * KAP(j, ix) + f * KAP(j + 1, ix);
+        Pq[q] = (ix > 1 && v > (T)1) ? (T)1 : v;
+      }
+      if (!(Pq[0] > xi) && Pq[8] > xi) {
+        int n0 = lo, n1 = hi;
+        T p0 = Pq[0], p1 = Pq[8];
+#pragma unroll
+        for (int q = 7; q >= 1; --q) {
+          const int ix = lo + q * step;
+          if (ix < hi && Pq[q] > xi) { n1 = ix; p1 = Pq[q]; }
+        }
+#pragma unroll
+        for (int q = 1; q < 8; ++q) {
+          const int ix = lo + q * step;
+          if (ix < hi && !(Pq[q] > xi)) { n0 = ix; p0 = Pq[q]; }
+        }
+        i0 = n0; i1 = n1; P0 = p0; P1 = p1;
+      }
+    }
+  }
   while (i1 - i0 > 1) {
     const int step = (i1 - i0 + 7) >> 3;
     T Pq[7];

@@ -70,6 +70,7 @@ struct RunParams {
   uint32_t round;          // 0 = first round of the run (no saved lanes)
   uint32_t regroup;        // waiting lanes of a wavefront that trigger an interaction / emission / tally pass
   uint32_t peel_regroup;   // idle lanes of a peel wavefront that trigger taking new items (with their first crossing)
+  uint32_t peel_tally;     // lanes of a peel wavefront whose walks reached the observer that trigger a tally pass
   RunCtl* ctl;
   LdsPlan lds;
 };
@@ -1684,9 +1685,9 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
     }
     const unsigned long long walking = __ballot(st == P_WALK);
     const unsigned long long done = __ballot(st == P_DONE);
-    const unsigned long long more = __ballot(st == P_IDLE && item < total);
+    unsigned long long more = __ballot(st == P_IDLE && item < total);
     if ((walking | done | more) == 0ull) break;
-    if (done && (walking == 0ull || (unsigned)__popcll(done) >= rp.regroup)) {
+    if (done && (walking == 0ull || (unsigned)__popcll(done) >= rp.peel_tally)) {
       if (st == P_DONE) {
         const uint64_t ri = item / npeel;
         const int k = (int)(item - ri * npeel);
@@ -1699,6 +1700,8 @@ __global__ void __launch_bounds__(256) k_peel(DeviceModel m0, DeviceTallies t, R
         st = P_IDLE;
         item += stride;
       }
+      // the lanes just tallied take their next items in the same pass
+      more = __ballot(st == P_IDLE && item < total);
     }
     if (more && (walking == 0ull || (unsigned)__popcll(more) >= rp.peel_regroup)) {
       bool found = false;

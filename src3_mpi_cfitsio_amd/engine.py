@@ -213,6 +213,12 @@ class Engine:
         self._check(self.lib.mcrt_set_launch(self.h, int(lanes), int(queue_per_lane),
                                              int(events_per_round), int(peel_blocks_per_cu), int(regroup)))
 
+    def set_peel_gather(self, fetch_lanes=0, tally_lanes=0):
+        """Peel-kernel gather thresholds: idle lanes of a wavefront that
+        trigger taking new items, and lanes waiting for a tally pass.
+        0 = default."""
+        self._check(self.lib.mcrt_set_peel_gather(self.h, int(fetch_lanes), int(tally_lanes)))
+
     def synchronize(self):
         self._check(self.lib.mcrt_synchronize(self.h))
 

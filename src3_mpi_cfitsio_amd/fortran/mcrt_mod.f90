@@ -90,6 +90,26 @@ module mcrt_mod
       real(c_double), intent(in) :: ldint(*), hdisk(*)
       real(c_double), value :: accfrac
     end function
+    ! diffusion approximation of thick regions (setdiffus_5.f90, emit_common_2.f90)
+    integer(c_int) function mcrt_set_diffusion(ctx, on, kappar, hdisk, rmaxd, nr0) &
+         bind(C, name='mcrt_set_diffusion')
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on, nr0
+      real(c_double), intent(in) :: kappar(*), hdisk(*)
+      real(c_double), value :: rmaxd
+    end function
+    integer(c_int) function mcrt_setdiffus(ctx, iter, ncells) bind(C, name='mcrt_setdiffus')
+      import :: c_int, c_ptr, c_int32_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: iter
+      integer(c_int64_t), intent(out) :: ncells
+    end function
+    integer(c_int) function mcrt_get_diffusion(ctx, itmin, count) bind(C, name='mcrt_get_diffusion')
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: itmin(*), count(*)
+    end function
     integer(c_int) function mcrt_set_source(ctx, rstar, tstar, rmin) bind(C, name='mcrt_set_source')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -632,7 +632,7 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
           rstar=2.0, tstar_k=4000.0, mdisk=0.01, mdot=1e-6, rc_au=100.0,
           clump_sigma=0.0, clump_dim=2.6, clump_seed=1001, pl=0.0,
           npeel=10, nx=101, bands=((0.01, 5000.0),), nfreq=100, nmu=10,
-          npackets=1_000_000, niter=5, nlam=100, nT=500, nnu=513,
+          npackets=1_000_000, niter=5, nlam=100, lam_max=1000.0, g=(0.6, 0.0), nT=500, nnu=513,
           thete_deg=None, phie_deg=0.0, flare=1.25, _tables=None, device=False, _cells=None) -> Model:
     """One model; device=True builds its cell arrays (volumes, densities,
     species) on the GPU (engine.build_models) instead of in numpy."""
@@ -657,7 +657,7 @@ def build(name: str, *, nr: int, nt: int, nphi: int, rin_au=0.1, rout_au=1000.0,
                                    flare=flare, mdot_msun_yr=mdot, rc_au=rc_au, clump_sigma=clump_sigma,
                                    clump_dim=clump_dim, clump_seed=clump_seed)
     if _tables is None:          # (dust, emission tables) reused across models of a batch
-        dust = synthetic_dust(nlam=nlam, pl=pl)
+        dust = synthetic_dust(nlam=nlam, lam_max=lam_max, g=g, pl=pl)
         em = emission_tables(dust, nT=nT, nnu=nnu)
     else:
         dust, em = _tables
@@ -694,17 +694,19 @@ CONFIGS: Dict[str, dict] = {
     # 2-D axisymmetric, FP64 plain method (CPU-runnable case)
     "c0": dict(nr=100, nt=90, nphi=1, npeel=10, nx=101, bands=((0.01, 5000.0),),
                npackets=1_000_000, niter=5),
-    # 3-D polarized clumpy; FP32 transport with FP64 face roots and tallies
-    "c1": dict(nr=200, nt=180, nphi=64, clump_sigma=0.5, clump_seed=1001, pl=0.5,
+    # 3-D polarized clumpy; FP32 transport with FP64 face roots and tallies.
+    # Scattering matrix of the dustmat form (dustmat.f90:42-48) with g = 0.5
+    # and peak linear polarization 0.5 at every wavelength
+    "c1": dict(nr=200, nt=180, nphi=64, clump_sigma=0.5, clump_seed=1001, pl=0.5, g=(0.5, 0.5),
                npeel=10, nx=256, bands=((1.1, 1.4), (2.0, 2.4), (3.2, 4.0)),
                npackets=100_000_000, niter=5),
     # adjoint model; the observer sits 0.05 deg off 60 deg, which is a theta
     # wall of this grid (a peel angle on a wall is refused, gridset:695-701)
-    "c3": dict(nr=150, nt=120, nphi=32, clump_sigma=0.3, clump_seed=1003, pl=0.5,
+    "c3": dict(nr=150, nt=120, nphi=32, clump_sigma=0.3, clump_seed=1003, pl=0.5, g=(0.5, 0.5),
                npeel=1, thete_deg=[60.05], nx=128, bands=((2.0, 2.4), (3.2, 4.0)),
                npackets=10_000_000, niter=1),
-    # large optically thick Lucy run
-    "c4": dict(nr=400, nt=360, nphi=128, mdisk=0.1, pl=0.5, npeel=10, nx=512,
+    # large optically thick Lucy run, 250 wavelengths over 0.1-3000 um
+    "c4": dict(nr=400, nt=360, nphi=128, mdisk=0.1, pl=0.5, nlam=250, lam_max=3000.0, npeel=10, nx=512,
                bands=((2.0, 2.4),), npackets=1_000_000_000, niter=10),
 }
 
