@@ -22,11 +22,52 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"),
 ]
 # kernels_fp64 must round exactly like the reference: no FMA contraction.
+# The kernel units also print the compiler's per-kernel resource usage, which
+# the build checks against MIN_OCCUPANCY.
+RESOURCE_REMARKS = "-Rpass-analysis=kernel-resource-usage"
 UNITS = [
-    ("kernels_fp64.hip", ["-ffp-contract=off"]),
-    ("kernels_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
+    ("kernels_fp64.hip", ["-ffp-contract=off", RESOURCE_REMARKS]),
+    ("kernels_mixed.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", RESOURCE_REMARKS]),
     ("mcrt_abi.hip", ["-ffp-contract=off"]),
 ]
+
+# Waves per SIMD the hot kernels must keep (VGPRs <= 512 / waves).  The
+# resident-lane transport kernel sizes its lanes from the occupancy it gets,
+# so a few more registers in a rarely taken branch would silently halve its
+# lanes (235 -> 281 VGPRs once took k_transport<float> from 2 waves to 1 and
+# c1 from 1.5 to 1.3 M packets/s).
+MIN_OCCUPANCY = {
+    "_ZN4mcrt11k_transportIfEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
+    "_ZN4mcrt6k_peelIfEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 3,
+    "_ZN4mcrt6k_peelIdEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
+}
+
+
+def kernel_resources(remarks):
+    """{kernel: {field: value}} from the compiler's kernel-resource-usage
+    remarks ("Function Name: ...", then "VGPRs: ...", "Occupancy [waves/SIMD]:
+    ..." lines)."""
+    out, cur = {}, None
+    for line in remarks.splitlines():
+        if "remark:" not in line or "kernel-resource-usage" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split("[-Rpass-analysis", 1)[0].strip()
+        key, _, val = body.partition(":")
+        key, val = key.strip(), val.strip()
+        if key == "Function Name":
+            cur = out.setdefault(val, {})
+        elif cur is not None:
+            cur[key] = val
+    return out
+
+
+def check_occupancy(res):
+    for name, need in MIN_OCCUPANCY.items():
+        if name in res:
+            got = int(res[name].get("Occupancy [waves/SIMD]", "0"))
+            if got < need:
+                raise RuntimeError(f"{name}: occupancy {got} waves/SIMD with {res[name].get('VGPRs')} VGPRs "
+                                   f"+ {res[name].get('AGPRs')} AGPRs; the engine is sized for {need}")
 
 
 def _run(cmd, cwd=None):
@@ -44,7 +85,13 @@ def build_hip(verbose=False):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
         out = _run(cmd)
-        if verbose and out.strip():
+        res = kernel_resources(out)
+        if res:
+            check_occupancy(res)
+            with open(os.path.join(BUILD, src.replace(".hip", ".resources.txt")), "w") as f:
+                for k, v in sorted(res.items()):
+                    f.write(f"{k} " + " ".join(f"{a}={b}" for a, b in v.items()) + "\n")
+        elif verbose and out.strip():
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"

@@ -883,6 +883,12 @@ __device__ __attribute__((noinline)) double azimuth(double x, double y) {
 __device__ __attribute__((noinline)) double shell_radius(double u, double ra, double rb) {
   return pow(u * (pow(ra, 3.0) - pow(rb, 3.0)) + pow(rb, 3.0), (double)(1.0f / 3.0f));
 }
+// emit_common_2.f90:258-270, :343-361: the fourth roots of the diffusion
+// surface's T^4 (only photons absorbed in a diffusion region reach them)
+__device__ __attribute__((noinline)) double quarter_power(double x) { return pow(x, 0.25); }
+// sin of a wall angle and the azimuth of a direction, for the same photons
+__device__ __attribute__((noinline)) double sin_nl(double x) { return sin(x); }
+__device__ __attribute__((noinline)) double atan2_nl(double y, double x) { return atan2(y, x); }
 struct Point3 { double x, y, z; };
 // testgrid.f90:80-86
 __device__ __attribute__((noinline)) Point3 sphere_point(double rtot, double cosb, double phi) {

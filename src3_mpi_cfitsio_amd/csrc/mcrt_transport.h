@@ -1057,7 +1057,7 @@ __device__ __forceinline__ void diffuse(Ctx<T>& C, Pkt<T>& p) {
   const bool upper = p.it <= (ntg - 1) / 2;
   const int its = upper ? am : ntg - am;            // the surface diffusion cell
   const int w = upper ? its : its + 1;               // its wall toward the free medium (1-based)
-  const double cw = m.costarr[w - 1], sw = sin(m.thetarr[w - 1]);
+  const double cw = m.costarr[w - 1], sw = sin_nl(m.thetarr[w - 1]);
   // move (emit_common_2.f90:230-237)
   const double cosb = p.zp / p.rtot;
   const double sinb = sqrt(1.0 - cosb * cosb);
@@ -1086,19 +1086,19 @@ __device__ __forceinline__ void diffuse(Ctx<T>& C, Pkt<T>& p) {
   p.sint = (T)sint;
   p.cosp = (T)(ux / sint);
   p.sinp = (T)(uy / sint);
-  p.phi = (T)atan2(uy / sint, ux / sint);
+  p.phi = (T)atan2_nl(uy / sint, ux / sint);
   // surface temperature after this photon (:258-270) and the emission temperature (:343-361)
   const double dA = kPi * sw * (m.r2arr[p.ir] - m.r2arr[p.ir - 1]);
   const size_t cs = cell_index(m, p.ir, its, p.ip);
   const unsigned k = atomicAdd(&C.t.diff_count[cs], 1u);
   const double Tcap = kTmaxK / 11605.0, T0 = 0.1 / 11605.0;
-  double Ts = pow(T0 * T0 * T0 * T0 + (double)(k + 1u) * m.diff_delta / dA, 0.25);
+  double Ts = quarter_power(T0 * T0 * T0 * T0 + (double)(k + 1u) * m.diff_delta / dA);
   if (Ts > Tcap) Ts = Tcap;
   const int mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2;
   const double nacc = (double)*(volatile const int*)&C.t.nabs[cell_index(m, p.ir, mid, p.ip)];
   const double T4eff = m.diff_delta * nacc / dA;
   const double mu = ux * sux + uy * suy + uz * suz;
-  double Te = pow(Ts * Ts * Ts * Ts + 0.75 * mu * T4eff, 0.25);
+  double Te = quarter_power(Ts * Ts * Ts * Ts + 0.75 * mu * T4eff);
   if (Te > Tcap) Te = Tcap;
   p.nub = bbfreq<T>(C.rng, (T)Te, C.inczeta);
   p.it = upper ? its - 1 : its + 1;
