@@ -369,9 +369,12 @@ def run_single_workload(args, T0):
         kern_ms.append(km)
         counters.append(tal_s["counters"])
         stats.append(eng.run_stats())
+        sx = stats[-1]
         log(f"timed step {len(step_ms)}: {dt:.0f} ms, kernel {km:.0f} ms, {N/(dt/1e3):.4g} packets/s, "
             f"segments/packet {tal_s['counters']['segments']/N:.0f}, peel segments/packet "
-            f"{tal_s['counters']['peel_segments']/N:.0f}, errors {tal_s['counters']['errors']:.0f}, t={elapsed():.0f}s")
+            f"{tal_s['counters']['peel_segments']/N:.0f}, errors {tal_s['counters']['errors']:.0f} "
+            f"(walk {sx['walk_errors']:.0f}, peel {sx['peel_errors']:.0f}, other {sx['other_errors']:.0f}), "
+            f"t={elapsed():.0f}s")
         return tal_s, t_out
 
     # the first timed step, then its FP64 twin on the same counters

@@ -399,7 +399,11 @@ enum {
   MCRT_STAT_EVENTS0 = 7,       /* events per lane in the first round */
   MCRT_STAT_RECORD_BYTES = 8,  /* bytes of one peel-off record */
   MCRT_STAT_LANE_BYTES = 9,    /* bytes of one parked lane */
-  MCRT_NSTATS = 10
+  /* errors since mcrt_zero_tallies (each also in counters[15]) by kind: */
+  MCRT_STAT_WALK_ERRORS = 10,  /* a transport flight found no wall or passed the segment guard */
+  MCRT_STAT_PEEL_ERRORS = 11,  /* a peel-off walk did */
+  MCRT_STAT_OTHER_ERRORS = 12, /* an emission placed nowhere, a full peel-off queue */
+  MCRT_NSTATS = 13
 };
 int mcrt_run_stats(mcrt_ctx *ctx, double *out);
 

@@ -444,7 +444,26 @@ int mcrt_set_dust(mcrt_ctx* c, int32_t nopac, int32_t nlam, const double* nudust
   rc |= upload(c, "lTint", lTint, nT, &m.lTint);
   rc |= upload(c, "Tint", Tint, nT, &m.Tint);
   rc |= upload(c, "nuint", nuint, nnu, &m.nuint);
-  rc |= upload(c, "kapint", kapint, (size_t)nopac * nT * nnu, &m.kapint);
+  {
+    // the re-emission CDFs frequency-contiguous per temperature row
+    // ([iop][iT][inu]; the reference's kapint(nT, nnu, nsp) has the
+    // temperature fastest), so the entries one search probes share cache
+    // lines; in FP64 and, for the mixed path, rounded to FP32
+    const size_t n = (size_t)nopac * nT * nnu;
+    std::vector<double> kt(n);
+    std::vector<float> kf(n);
+    for (int id = 0; id < nopac; ++id)
+      for (int j = 0; j < nnu; ++j)
+        for (int i = 0; i < nT; ++i) {
+          const double v = kapint[((size_t)id * nnu + j) * nT + i];
+          kt[((size_t)id * nT + i) * nnu + j] = v;
+          kf[((size_t)id * nT + i) * nnu + j] = (float)v;
+        }
+    rc |= upload(c, "kapint", kt.data(), n, &m.kapint);
+    rc |= upload(c, "kapint32", kf.data(), n, &m.kapint32);
+    if (rc) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));   // kt, kf go out of scope
+  }
   rc |= upload(c, "kappap", kappap, (size_t)nopac * nT, &m.kappap);
   if (rc) return rc;
   // guide table of the re-emission CDFs (lucydustfreq_xi): for every row,
@@ -935,6 +954,79 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     dm.acc_start = (const AccStart*)p;
   }
   const int pgrid = c->cu_count * (c->peel_blocks_per_cu > 0 ? c->peel_blocks_per_cu : 8);
+  // The mixed path walks its peel-offs in three kernels (mcrt_peel.h) unless
+  // MCRT_PEEL_SPLIT=0 asks for the one-kernel k_peel.  Ray states and hits
+  // are sized for every (record, direction) item of a full round.
+  bool split = queued && !fp64;
+  if (const char* ps = std::getenv("MCRT_PEEL_SPLIT")) split = split && std::atoi(ps) != 0;
+  PeelRays rays{};
+  PeelHits hits{};
+  unsigned* split_ctr = nullptr;
+  int wgrid = 0;
+  size_t chunk = 0;
+  const int sgrid = pgrid * 4;            // blocks of k_peel_start and k_peel_hits
+  // (an item is named by a 32-bit index within its round)
+  const size_t round_items = (size_t)capacity * (size_t)c->dm.npeel;
+  if (round_items >= (size_t)0xFFFFFFFFu) split = false;
+  if (split) {
+    // a round's items in chunks of at most 64 Mi (about 10 GB of ray states
+    // and hits)
+    size_t max_items = (size_t)64 << 20;
+    if (const char* pc = std::getenv("MCRT_PEEL_CHUNK")) {     // smaller chunks, for tests
+      const long long v = std::atoll(pc);
+      if (v >= 256 && (size_t)v < max_items) max_items = (size_t)v;
+    }
+    chunk = round_items < max_items ? round_items : max_items;
+    int bpc = peel_walk_blocks_per_cu_mixed(plan_lds(c->dm).bytes);
+    if (bpc < 1) bpc = 1;
+    wgrid = c->cu_count * bpc;
+    // every wavefront of k_peel_start and k_peel_walk reserves slots 64 at a
+    // time and may leave up to 63 of them unused
+    const size_t n = chunk + (size_t)64 * 4 * ((size_t)sgrid + (size_t)wgrid);
+    double** d[12] = {&rays.ox, &rays.oy, &rays.oz, &rays.b, &rays.h2, &rays.t, &rays.tr, &rays.tlo, &rays.tlo2,
+                      &rays.thi, &rays.thi2, &rays.tp};
+    static const char* dn[12] = {"ray.ox", "ray.oy", "ray.oz", "ray.b", "ray.h2", "ray.t", "ray.tr", "ray.tlo",
+                                 "ray.tlo2", "ray.thi", "ray.thi2", "ray.tp"};
+    for (int i = 0; i < 12; ++i) {
+      if ((rc = dev_alloc(c, dn[i], n * 8, &p))) return rc;
+      *d[i] = (double*)p;
+    }
+    static const char* kn[kMaxOpac] = {"ray.kapd0", "ray.kapd1", "ray.kapd2", "ray.kapd3"};
+    for (int i = 0; i < kMaxOpac; ++i) {
+      if ((rc = dev_alloc(c, kn[i], n * 4, &p))) return rc;
+      rays.kapd[i] = (float*)p;
+    }
+    if ((rc = dev_alloc(c, "ray.tsum", n * 4, &p))) return rc;
+    rays.tsum = (float*)p;
+    if ((rc = dev_alloc(c, "ray.cell", n * 4, &p))) return rc;
+    rays.cell = (uint32_t*)p;
+    if ((rc = dev_alloc(c, "ray.item", n * 4, &p))) return rc;
+    rays.item = (uint32_t*)p;
+    if ((rc = dev_alloc(c, "ray.steps", n, &p))) return rc;
+    rays.steps = (uint8_t*)p;
+    if ((rc = dev_alloc(c, "hit.xe", n * 8, &p))) return rc;
+    hits.xe = (double*)p;
+    if ((rc = dev_alloc(c, "hit.ye", n * 8, &p))) return rc;
+    hits.ye = (double*)p;
+    if ((rc = dev_alloc(c, "hit.ze", n * 8, &p))) return rc;
+    hits.ze = (double*)p;
+    if ((rc = dev_alloc(c, "hit.tsum", n * 4, &p))) return rc;
+    hits.tsum = (float*)p;
+    if ((rc = dev_alloc(c, "hit.item", n * 4, &p))) return rc;
+    hits.item = (uint32_t*)p;
+    if ((rc = dev_alloc(c, "peel.ctr", 4 * sizeof(unsigned), &p))) return rc;
+    split_ctr = (unsigned*)p;
+    rays.count = split_ctr;
+    rays.next = split_ctr + 1;
+    hits.count = split_ctr + 2;
+    rays.capacity = (unsigned)n;
+    hits.capacity = (unsigned)n;
+  }
+  unsigned walk_fetch = c->peel_fetch > 0 ? (unsigned)c->peel_fetch : 16u;
+  if (const char* pw = std::getenv("MCRT_PEEL_WALK_FETCH")) {
+    const int v = std::atoi(pw);
+    if (v >= 1 && v <= 64) walk_fetch = (unsigned)v;
+  }
   // Gather thresholds of the peel kernel.  Lanes whose walk reached the
   // observer wait for a tally pass and idle lanes for a fetch pass; while
   // they wait they do not walk, so high thresholds leave most of a
@@ -990,7 +1082,19 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     hipError_t e = fp64 ? launch_transport_fp64(dm, t, rp, c->stream) : launch_transport_mixed(dm, t, rp, c->stream);
     if (e != hipSuccess) return hipfail(c, e, "transport kernel launch");
     HIPCHECK(c, hipEventRecord(c->evr[1], c->stream));
-    if (queued) {
+    if (queued && split) {
+      // the round's record count sizes the chunks
+      HIPCHECK(c, hipMemcpyAsync(c->host_ctl, ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, c->stream));
+      HIPCHECK(c, hipStreamSynchronize(c->stream));
+      const uint64_t nrec = c->host_ctl->q_count < (unsigned)capacity ? c->host_ctl->q_count : (unsigned)capacity;
+      const uint64_t total = nrec * (uint64_t)c->dm.npeel;
+      for (uint64_t i0 = 0; i0 < total; i0 += chunk) {
+        const uint64_t i1 = total - i0 < chunk ? total : i0 + chunk;
+        HIPCHECK(c, launch_fill_i32((int*)split_ctr, 4, 0, c->stream));
+        e = launch_peel_split_mixed(dm, t, rp, rays, hits, i0, i1, sgrid, wgrid, sgrid, walk_fetch, c->stream);
+        if (e != hipSuccess) return hipfail(c, e, "peel kernels launch");
+      }
+    } else if (queued) {
       e = fp64 ? launch_peel_fp64(dm, t, rp, pgrid, c->stream) : launch_peel_mixed(dm, t, rp, pgrid, c->stream);
       if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
     }
@@ -1013,6 +1117,14 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     if (h.live == 0) break;
   }
   st[MCRT_STAT_ROUNDS] = (double)(round + 1);
+  {
+    unsigned long long cnt[C_NCOUNTERS];
+    HIPCHECK(c, hipMemcpyAsync(cnt, c->dt.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    st[MCRT_STAT_WALK_ERRORS] = (double)cnt[C_ERR_WALK];
+    st[MCRT_STAT_PEEL_ERRORS] = (double)cnt[C_ERR_PEEL];
+    st[MCRT_STAT_OTHER_ERRORS] = (double)cnt[C_ERR_OTHER];
+  }
   std::memcpy(c->stats, st, sizeof(st));
   HIPCHECK(c, hipEventRecord(c->ev1, c->stream));
   c->ev_valid = true;
@@ -1451,7 +1563,7 @@ int mcrt_get_batch_tallies(mcrt_ctx* c, double* dtauabs, int32_t* nabs, int32_t*
   if (work)
     for (int k = 0; k < c->nmodels; ++k) work[k] = (double)w[k];
   if (counters)
-    for (int i = 0; i < C_NCOUNTERS; ++i) counters[i] = (double)cnt[i];
+    for (int i = 0; i < C_NPUBLIC; ++i) counters[i] = (double)cnt[i];
   return MCRT_OK;
 }
 
@@ -1562,7 +1674,7 @@ int mcrt_get_tallies(mcrt_ctx* c, double* dtauabs, int32_t* nabs, int32_t* dbig,
   HIPCHECK(c, hipMemcpyAsync(cnt, c->dt.counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
   HIPCHECK(c, hipStreamSynchronize(s));
   if (counters)
-    for (int i = 0; i < C_NCOUNTERS; ++i) counters[i] = (double)cnt[i];
+    for (int i = 0; i < C_NPUBLIC; ++i) counters[i] = (double)cnt[i];
   return MCRT_OK;
 }
 

@@ -112,7 +112,7 @@ __device__ __forceinline__ void stage_shared_tables(DeviceModel& m, const double
 // `regroup` lanes of the wavefront wait for them, or nobody walks) without
 // an event budget: there is no peel-off queue to drain, so nothing is
 // parked and the launch ends when the last packet does.  kComb selects the
-// write-combined dtauabs tally.
+// write-combined cell tallies (tally_abs).
 template <class T, int BS, bool kComb>
 __global__ void __launch_bounds__(BS) k_batch(DeviceModel m0, DeviceTallies t0, BatchDesc bd, BatchRun* run,
                                                const double* inczeta0, LdsPlan lds) {
@@ -128,8 +128,7 @@ __global__ void __launch_bounds__(BS) k_batch(DeviceModel m0, DeviceTallies t0, 
   const unsigned long long total = run->total;
   const uint32_t regroup = run->regroup;
   Life<T> L;
-  L.acc_c = 0xFFFFFFFFu;
-  L.acc_v = 0.0;
+  tally_open(L);
   const bool trial = run->trial != 0;
   int st = L_IDLE;
   int model = 0;
@@ -159,7 +158,7 @@ __global__ void __launch_bounds__(BS) k_batch(DeviceModel m0, DeviceTallies t0, 
     if ((walking | ready | idle) == 0ull) break;
     if (ready && (walking == 0ull || (unsigned)__popcll(ready) >= regroup)) {
       if (st == L_READY) {
-        interact<T>(C, L);
+        interact<T, false, kComb>(C, L);
         st = L_WALK;
       }
     }
@@ -176,7 +175,7 @@ __global__ void __launch_bounds__(BS) k_batch(DeviceModel m0, DeviceTallies t0, 
           rng_init(C.rng, seed + (uint64_t)model, pk);
           L.pid = pk;
           const int e = emit<T>(C, L);
-          if (e == 0) end_packet<T>(C, L, -1);
+          if (e == 0) end_packet<T>(C, L, -1, C_ERR_OTHER);
           else st = (e == 1) ? L_WALK : L_READY;
         }
       }

@@ -29,11 +29,19 @@ constexpr double kR2p = 2.0 * 3.1415926535897932384626433;
 constexpr double kRsol = 6.9598e10;
 constexpr double kTmaxK = 199999.0;
 
-// Counters accumulated per launch (atomic, 64-bit).
+// Counters accumulated per launch (atomic, 64-bit).  The first C_NPUBLIC are
+// mcrt_get_tallies' counters[16]; the error kinds after them (each error also
+// counts in C_ERRORS) go to mcrt_run_stats.
 enum Counter {
   C_TOT = 0, C_FLUX, C_SFLUX, C_AFLUX, C_ABFLUX, C_SCOUNT, C_NSCAT, C_HIT, C_HTOT,
-  C_ESCAPED, C_KILLED, C_SEGMENTS, C_PEELSEG, C_PEELEVT, C_RESETS, C_ERRORS, C_NCOUNTERS
+  C_ESCAPED, C_KILLED, C_SEGMENTS, C_PEELSEG, C_PEELEVT, C_RESETS, C_ERRORS,
+  C_ERR_WALK,      // a transport flight found no wall, or passed the segment guard
+  C_ERR_PEEL,      // a peel-off walk did
+  C_ERR_OTHER,     // an emission placed nowhere, a full peel-off queue
+  C_NCOUNTERS
 };
+constexpr int C_NPUBLIC = C_ERRORS + 1;
+static_assert(C_NPUBLIC == 16, "mcrt_get_tallies returns 16 counters");
 
 struct DeviceModel {
   // grid walls, 1-based reference semantics stored 0-based (wall k at [k-1])
@@ -55,7 +63,11 @@ struct DeviceModel {
   double nud_log0[kMaxOpac], nud_ilog[kMaxOpac];
   double kappav[kMaxOpac];
   int nT, nnu;
-  const double *lTint, *Tint, *nuint, *kapint, *kappap;
+  const double *lTint, *Tint, *nuint, *kappap;
+  // re-emission CDFs, kapint[(iop*nT + i)*nnu + j] (frequency contiguous), and
+  // the same values rounded to FP32 for the mixed path
+  const double* kapint;
+  const float* kapint32;
   // guide table of the re-emission CDFs: kguide[(iop*nT + i)*(kguide_k+1) + k]
   // = the number of frequency entries of temperature row i with
   // kapint <= k/kguide_k (null: none)
@@ -543,14 +555,19 @@ __device__ __forceinline__ T bbfreq(Rng& r, T teff, const double* inczeta) {
   return -RT<T>::log_(z1 * z2 * z3 * z4) * teff / (T)i;
 }
 
+// the re-emission CDF rows in the precision of the path
+template <class T> struct KapRows { __device__ static const double* get(const DeviceModel& m) { return m.kapint; } };
+template <> struct KapRows<float> { __device__ static const float* get(const DeviceModel& m) { return m.kapint32; } };
+template <class T> __device__ __forceinline__ auto kap_rows(const DeviceModel& m) { return KapRows<T>::get(m); }
+
 // lucydustfreq_2_test.f90:19-160 with xi supplied
 template <class T>
 __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop, T xi) {
   const int nT = m.nT, nnu = m.nnu;
   const double* lT = m.lTint;
   const double* nui = m.nuint;
-  const double* K = m.kapint + (size_t)iop * nT * nnu;
-#define KAP(i, j) ((T)K[(size_t)((j) - 1) * nT + ((i) - 1)])
+  const auto* K = kap_rows<T>(m) + (size_t)iop * nT * nnu;
+#define KAP(i, j) ((T)K[(size_t)((i) - 1) * nnu + ((j) - 1)])
   T lT0 = RT<T>::log_(T0);
   int j = (int)((T)(nT - 1) * (lT0 - (T)lT[0]) / ((T)lT[nT - 1] - (T)lT[0])) + 1;
   if (j < 1) j = 1;

@@ -260,9 +260,11 @@ struct Life {
   uint32_t c_rho, c_sp, c_td;
   T rho;
   int sp, spop;        // species code and its opacity table
-  // write-combined dtauabs of the cell visit in progress (walk_step<.., kComb>)
+  // write-combined tallies of the cell visit in progress (kComb): dtauabs,
+  // and the absorption counters dbig and nabs
   uint32_t acc_c;
   double acc_v;
+  uint32_t acc_dbig, acc_nabs;
   T td;
 };
 
@@ -297,27 +299,58 @@ __device__ __forceinline__ T cell_opac_l(const DeviceModel& m, Life<T>& L, size_
   return (T)0;
 }
 
-// The Lucy tally of one segment (tauint_3d_2_fuv.f90:83-90,122-130).  With
-// kComb the lane sums its segments in one cell and adds the sum when it
-// leaves the cell (or ends the packet: tally_flush): a packet trapped in a
-// thick cell then issues one FP64 atomic per visit instead of one per flight.
+// The Lucy tally of one segment (tauint_3d_2_fuv.f90:83-90,122-130) and the
+// absorption counters of an interaction (propagate_5_fuv.f90:139-141).  With
+// kComb the lane sums them over its visit to one cell and adds the sums when
+// it leaves the cell, ends the packet or parks it (tally_flush): a packet
+// trapped in a thick cell then issues a few atomics per visit instead of up
+// to three per interaction.  This matters for the packets whose serial
+// chains end a run: on this GPU a vector memory instruction's completion
+// counter also counts stores and atomics, so every atomic issued before the
+// next load of the chain adds its round trip to that load's wait.  Integer
+// counts are unchanged; dtauabs changes only by the order of its FP64 sums.
+template <class T>
+__device__ __forceinline__ void tally_flush(Ctx<T>& C, Life<T>& L) {
+  if (L.acc_c != 0xFFFFFFFFu) {
+    if (L.acc_v != 0.0) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
+    if (L.acc_dbig) atomicAdd(&C.t.dbig[L.acc_c], (int)L.acc_dbig);
+    if (L.acc_nabs) atomicAdd(&C.t.nabs[L.acc_c], (int)L.acc_nabs);
+  }
+  L.acc_c = 0xFFFFFFFFu;
+  L.acc_v = 0.0;
+  L.acc_dbig = 0u;
+  L.acc_nabs = 0u;
+}
+
+template <class T>
+__device__ __forceinline__ void tally_open(Life<T>& L) {
+  L.acc_c = 0xFFFFFFFFu;
+  L.acc_v = 0.0;
+  L.acc_dbig = 0u;
+  L.acc_nabs = 0u;
+}
+
 template <bool kComb, class T>
 __device__ __forceinline__ void tally_abs(Ctx<T>& C, Life<T>& L, size_t c, double v) {
   if constexpr (kComb) {
-    if (L.acc_c == (uint32_t)c) { L.acc_v += v; return; }
-    if (L.acc_c != 0xFFFFFFFFu) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
-    L.acc_c = (uint32_t)c;
-    L.acc_v = v;
+    if (L.acc_c != (uint32_t)c) { tally_flush<T>(C, L); L.acc_c = (uint32_t)c; }
+    L.acc_v += v;
   } else {
     atomicAdd(&C.t.dtauabs[c], v);
   }
 }
 
-template <class T>
-__device__ __forceinline__ void tally_flush(Ctx<T>& C, Life<T>& L) {
-  if (L.acc_c != 0xFFFFFFFFu) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
-  L.acc_c = 0xFFFFFFFFu;
-  L.acc_v = 0.0;
+// dbig (every absorption) and nabs (when counted) of cell c
+template <bool kComb, class T>
+__device__ __forceinline__ void tally_absorb(Ctx<T>& C, Life<T>& L, size_t c, bool nabs) {
+  if constexpr (kComb) {
+    if (L.acc_c != (uint32_t)c) { tally_flush<T>(C, L); L.acc_c = (uint32_t)c; }
+    L.acc_dbig++;
+    if (nabs) L.acc_nabs++;
+  } else {
+    atomicAdd(&C.t.dbig[c], 1);
+    if (nabs) atomicAdd(&C.t.nabs[c], 1);
+  }
 }
 
 template <class T>
@@ -489,7 +522,7 @@ __device__ __forceinline__ int peel_step(Ctx<T>& C, PeelWalk<T>& w, Rng& rng, do
       found = next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface);
     }
   }
-  if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_PEEL); return 2; }
   C.cnt.add_lanes(C_PEELSEG);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
@@ -670,7 +703,7 @@ __device__ __forceinline__ int peel_ray_first(Ctx<T>& C, PeelWalk<T>& w, PeelRay
     if (testgrid<T>(m, p, rng)) C.cnt.add(C_RESETS);
     found = next_face<T>(m, p, w.ux, w.uy, w.uz, dt, iface);
   }
-  if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  if (!found || ++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_PEEL); return 2; }
   C.cnt.add_lanes(C_PEELSEG);
   size_t c = cell_index(m, p.ir, p.it, p.ip);
   int iop;
@@ -701,7 +734,7 @@ template <class T>
 __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<T>& R) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = w.p;
-  if (++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); return 2; }
+  if (++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_PEEL); return 2; }
   C.cnt.add_lanes(C_PEELSEG);
   // 0 radial, 1 lower polar wall, 2 upper polar wall, 3 azimuthal
   int axis = 0;
@@ -931,7 +964,7 @@ __device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, i
   if (C.q.capacity == 0) return;          // no observer directions
   const unsigned slot = take_slot(C);
   if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, iop, false, (T)0, (T)0);
-  else C.cnt.add(C_ERRORS);              // the host sizes the queue so this cannot happen
+  else { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_OTHER); }   // the host sizes the queue so this cannot happen
 }
 
 // direct stellar peel, weight 4 I cos(norm) (newdisktherm_5_fuv.f90:371-414)
@@ -941,7 +974,7 @@ __device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp,
   if (C.q.capacity == 0) return;
   const unsigned slot = take_slot(C);
   if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, 0, true, coslp, sinlp);
-  else C.cnt.add(C_ERRORS);
+  else { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_OTHER); }
 }
 
 // stokes.f90:46-137 rejection sampling, then the Mueller update
@@ -1105,7 +1138,7 @@ __device__ __forceinline__ void diffuse(Ctx<T>& C, Pkt<T>& p) {
   set_dir<T>(p);
 }
 
-template <class T, bool kAdj = false>
+template <class T, bool kAdj = false, bool kComb = false>
 __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
@@ -1122,13 +1155,15 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
     stokes<T>(C, p, iop);
   } else {
-    atomicAdd(&t.dbig[c], 1);
     p.i_orig = start ? 2 : i_orig_of(L.idust2);
     // propagate_5_fuv.f90:139: a diffusion cell counts only the accretion
     // photons placed in it (initpacc_4.f90); its absorptions re-emit from
     // the diffusion surface (emit_common_2.f90:88)
     const bool diff = diffusing(m, p.ir, p.it);
-    if (start || !diff) atomicAdd(&t.nabs[c], 1);
+    tally_absorb<kComb>(C, L, c, start || !diff);
+    // the surface temperature reads the midplane's nabs: this lane's own
+    // counts are in it first
+    if constexpr (kComb) { if (diff) tally_flush<T>(C, L); }
     if (diff) diffuse<T>(C, p);
     else reemit<T>(C, p, cell_tdust<T>(m, L, c), iop);
     p.iflag = 0;
@@ -1415,12 +1450,14 @@ __device__ __forceinline__ int emit(Ctx<T>& C, Life<T>& L) {
   return 1;
 }
 
-// end of a packet: SED for escaped ones, outcome counters, trace row
+// end of a packet: SED for escaped ones, outcome counters, trace row.
+// code -1 is an error, counted as err_kind (a failed walk, or an emission
+// that could not be placed).
 template <class T>
-__device__ __forceinline__ void end_packet(Ctx<T>& C, Life<T>& L, int code) {
+__device__ __forceinline__ void end_packet(Ctx<T>& C, Life<T>& L, int code, int err_kind = C_ERR_WALK) {
   if (code == 1) { prop_end<T>(C, L); C.cnt.add(C_ESCAPED); }
   else if (code == 2) C.cnt.add(C_KILLED);
-  else C.cnt.add(C_ERRORS);
+  else { C.cnt.add(C_ERRORS); C.cnt.add(err_kind); }
   if (C.t.trace) {
     const Pkt<T>& p = L.p;
     double* tr = C.t.trace + 8 * L.pid;
@@ -1541,8 +1578,14 @@ __device__ __forceinline__ unsigned long long fetch_packets(unsigned long long* 
 // events are spent) is parked in HBM for the next round.
 enum : int { L_IDLE = 0, L_WALK = 1, L_READY = 2 };
 
+// The mixed path write-combines its cell tallies (tally_abs); the FP64 path
+// adds every segment and absorption on its own, as the reference orders them.
+template <class T>
+constexpr bool kCombTally = std::is_same<T, float>::value;
+
 template <class T>
 __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies t, RunParams rp) {
+  constexpr bool kComb = kCombTally<T>;
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
   extern __shared__ double lds_tables[];
   counts_begin(s_cnt);
@@ -1553,6 +1596,7 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   LaneIO io{rp.lane_words + lane, rp.lanes, 0};
   Life<T> L;
+  tally_open(L);
   int st = L_IDLE;
   if (rp.round > 0 && rp.lane_live[lane]) {
     unpark<T>(io, C, L, rp.seed, rp.first_packet);
@@ -1563,11 +1607,15 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
   bool full = false;                // the round's peel-off queue is full
   for (;;) {
     if (st == L_WALK) {
-      const int r = walk_step<T>(C, L);
+      const int r = walk_step<T, false, kComb>(C, L);
       if (r != 0) {
         const int code = (r == 1) ? flight_end<T>(C, L) : ((r == 2) ? 1 : -1);
         if (code == 0) st = L_READY;
-        else { end_packet<T>(C, L, code); st = L_IDLE; }
+        else {
+          if constexpr (kComb) tally_flush<T>(C, L);
+          end_packet<T>(C, L, code);
+          st = L_IDLE;
+        }
       }
     }
     const unsigned long long walking = __ballot(st == L_WALK);
@@ -1578,7 +1626,7 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
       if (st == L_READY && budget > 0 && !full) {
         full = !hold_slot(C);
         if (!full) {
-          interact<T>(C, L);
+          interact<T, false, kComb>(C, L);
           budget--;
           st = L_WALK;
         }
@@ -1597,13 +1645,14 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
           L.pid = k;
           budget--;
           const int e = emit<T>(C, L);
-          if (e == 0) end_packet<T>(C, L, -1);
+          if (e == 0) end_packet<T>(C, L, -1, C_ERR_OTHER);
           else st = (e == 1) ? L_WALK : L_READY;
         }
       }
     }
   }
   const bool live = (st == L_READY);
+  if constexpr (kComb) tally_flush<T>(C, L);
   if (live) { io.w = 0; park<T>(io, C, L); }
   rp.lane_live[lane] = live ? 1 : 0;
   for (unsigned sl = C.slot_next; sl < C.slot_end && sl < C.q.capacity; ++sl)
