@@ -37,6 +37,25 @@ hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const
   return hipGetLastError();
 }
 
+hipError_t launch_peel_split_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const PeelRays& rays,
+                                   const PeelHits& hits, uint64_t item0, uint64_t item1, int grid_start, int grid_walk,
+                                   int grid_hits, unsigned fetch, hipStream_t s) {
+  hipLaunchKernelGGL(k_peel_start<float>, dim3(grid_start), dim3(256), rp.lds.bytes, s, m, t, rp, rays, hits, item0, item1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_peel_walk<float>, dim3(grid_walk), dim3(256), rp.lds.bytes, s, m, t, rp, rays, hits, fetch);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_peel_hits<float>, dim3(grid_hits), dim3(256), rp.lds.bytes, s, m, t, rp, hits);
+  return hipGetLastError();
+}
+
+int peel_walk_blocks_per_cu_mixed(unsigned lds_bytes) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_peel_walk<float>, 256, lds_bytes) != hipSuccess) return 0;
+  return n;
+}
+
 size_t peel_record_bytes_mixed() { return sizeof(PeelRec<float>); }
 
 template <int BS, bool kComb>

@@ -2,6 +2,7 @@
 // translation units (kernels_fp64.hip: -ffp-contract=off; kernels_mixed.hip).
 #pragma once
 #include "mcrt_batch.h"
+#include "mcrt_peel.h"
 
 namespace mcrt {
 
@@ -39,6 +40,15 @@ hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, con
 hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
 hipError_t launch_peel_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);
+// the mixed path's peel-off walks of the round's items [item0, item1) as
+// k_peel_start, k_peel_walk (grid_walk resident blocks, new ray states once
+// `fetch` lanes of a wavefront are idle) and k_peel_hits; item1 - item0 must
+// not exceed the capacity of rays and hits, and their counters must be zero
+hipError_t launch_peel_split_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const PeelRays& rays,
+                                   const PeelHits& hits, uint64_t item0, uint64_t item1, int grid_start, int grid_walk,
+                                   int grid_hits, unsigned fetch, hipStream_t s);
+// resident 256-thread blocks of k_peel_walk per CU
+int peel_walk_blocks_per_cu_mixed(unsigned lds_bytes);
 size_t peel_record_bytes_fp64();
 size_t peel_record_bytes_mixed();
 hipError_t launch_trace_fp64(const DeviceModel& m, const TraceArgs& a, hipStream_t s);
