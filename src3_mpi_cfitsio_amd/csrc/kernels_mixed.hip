@@ -46,7 +46,13 @@ hipError_t launch_peel_split_mixed(const DeviceModel& m, const DeviceTallies& t,
   hipLaunchKernelGGL(k_peel_walk<float>, dim3(grid_walk), dim3(256), rp.lds.bytes, s, m, t, rp, rays, hits, fetch);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_peel_hits<float>, dim3(grid_hits), dim3(256), rp.lds.bytes, s, m, t, rp, hits);
+  // hits: npeel blocks per part of the hit list, the peeled SED of each
+  // block's direction in LDS when that slice fits in 64 KB
+  const unsigned slice = 9u * (unsigned)m.nfreq * (unsigned)m.nap * (unsigned)kNo * 4u;
+  const int sed_lds = slice <= 64u * 1024u ? 1 : 0;
+  const int parts = grid_hits / m.npeel > 0 ? grid_hits / m.npeel : 1;
+  hipLaunchKernelGGL(k_peel_hits<float>, dim3((unsigned)(parts * m.npeel)), dim3(256),
+                     rp.lds.bytes + (sed_lds ? slice : 0u), s, m, t, rp, hits, sed_lds);
   return hipGetLastError();
 }
 

@@ -964,7 +964,11 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   unsigned* split_ctr = nullptr;
   int wgrid = 0;
   size_t chunk = 0;
-  const int sgrid = pgrid * 4;            // blocks of k_peel_start and k_peel_hits
+  const int sgrid = pgrid * 4;            // blocks of k_peel_start
+  // blocks of k_peel_hits: a part of the hit list per two CUs, one block per
+  // direction and part (each block sums its direction's peeled SED in LDS,
+  // so fewer, fuller blocks flush fewer bins)
+  const int hgrid = (c->cu_count / 2 > 0 ? c->cu_count / 2 : 1) * (c->dm.npeel > 0 ? c->dm.npeel : 1);
   // (an item is named by a 32-bit index within its round)
   const size_t round_items = (size_t)capacity * (size_t)c->dm.npeel;
   if (round_items >= (size_t)0xFFFFFFFFu) split = false;
@@ -1091,7 +1095,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
       for (uint64_t i0 = 0; i0 < total; i0 += chunk) {
         const uint64_t i1 = total - i0 < chunk ? total : i0 + chunk;
         HIPCHECK(c, launch_fill_i32((int*)split_ctr, 4, 0, c->stream));
-        e = launch_peel_split_mixed(dm, t, rp, rays, hits, i0, i1, sgrid, wgrid, sgrid, walk_fetch, c->stream);
+        e = launch_peel_split_mixed(dm, t, rp, rays, hits, i0, i1, sgrid, wgrid, hgrid, walk_fetch, c->stream);
         if (e != hipSuccess) return hipfail(c, e, "peel kernels launch");
       }
     } else if (queued) {

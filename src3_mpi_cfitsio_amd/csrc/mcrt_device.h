@@ -139,8 +139,11 @@ struct DeviceTallies {
   unsigned* diff_count;          // photons re-emitted from each diffusion surface cell (diffusion on)
 };
 
+// With at most 1023 walls per axis (mcrt_set_grid) every factor is below
+// 2^24 and the index below 2^30, so 24-bit multiplies give it exactly.
 __device__ __forceinline__ size_t cell_index(const DeviceModel& m, int ir, int it, int ip) {
-  return (size_t)(ir - 1) + (size_t)(m.nrg - 1) * ((size_t)(it - 1) + (size_t)(m.ntg - 1) * (size_t)(ip - 1));
+  const unsigned inner = (unsigned)(it - 1) + __umul24((unsigned)(m.ntg - 1), (unsigned)(ip - 1));
+  return (size_t)((unsigned)(ir - 1) + __umul24((unsigned)(m.nrg - 1), inner));
 }
 
 // cell (ir, it) lies in the diffusion region of its column

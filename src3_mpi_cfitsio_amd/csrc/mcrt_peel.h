@@ -196,17 +196,21 @@ __global__ void __launch_bounds__(256) k_peel_walk(DeviceModel m0, DeviceTallies
   bool dry = false;
   bool walking = false;
   uint32_t item = kSlotHole;
+  unsigned long long segs = 0;    // this lane's segments (the first crossings are counted at the start)
   PeelWalk<T> w;
   PeelRay<T> ray;
   for (;;) {
     int r = 0;
-    if (walking) r = peel_ray_step<T>(C, w, ray);
+    if (walking) r = peel_ray_step<T, false>(C, w, ray);
     const unsigned hs = pool_take(hits.count, walking && r == 1, hbase, hleft);
     if (walking && r == 1) {
       if (hs < hits.capacity) write_hit<T>(hits, hs, w, item);
       else { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_OTHER); }
     }
-    if (r != 0) walking = false;
+    if (r != 0) {
+      segs += (unsigned long long)(w.nseg - 1);
+      walking = false;
+    }
     const unsigned long long walk_mask = __ballot(walking);
     const unsigned long long idle = __ballot(!walking && !dry);
     if ((walk_mask | idle) == 0ull) break;
@@ -224,12 +228,23 @@ __global__ void __launch_bounds__(256) k_peel_walk(DeviceModel m0, DeviceTallies
   }
   const int lane = __lane_id();
   if ((unsigned)lane < hleft && hbase + lane < hits.capacity) hits.item[hbase + lane] = kSlotHole;
+  for (int off = 32; off > 0; off >>= 1) segs += __shfl_xor(segs, off, 64);
+  if (lane == 0 && segs) C.cnt.add(C_PEELSEG, segs);
   counts_flush(t, s_cnt);
 }
 
-// The walks that reached the observer: k_peel's tally, one hit per thread.
+// The walks that reached the observer: k_peel's tally.  Block b takes the
+// hits of direction k = b % npeel in part b / npeel of the hit list (the
+// other directions' hits of that part go to their own blocks).  With
+// sed_lds, the peeled SED of direction k ([9][8][nap][nfreq] floats) is
+// summed in LDS and added to t.psed once per block: every hit adds to 27
+// SED bins, and the hits of a round crowd into the few (frequency,
+// direction) bins where the light is, so global atomics on them serialize
+// in L2 (the kernel ran at 0.7% VALU with 2% L2 hits); in LDS each CU sums
+// its own copy.  The images, spread over nx*nx pixels, stay global atomics.
 template <class T>
-__global__ void __launch_bounds__(256) k_peel_hits(DeviceModel m0, DeviceTallies t, RunParams rp, PeelHits hits) {
+__global__ void __launch_bounds__(256) k_peel_hits(DeviceModel m0, DeviceTallies t, RunParams rp, PeelHits hits,
+                                                   int sed_lds) {
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
   extern __shared__ double lds_tables[];
   counts_begin(s_cnt);
@@ -239,18 +254,43 @@ __global__ void __launch_bounds__(256) k_peel_hits(DeviceModel m0, DeviceTallies
   Ctx<T> C{m, t, {}, {s_cnt}, inczeta, PeelQueue{nullptr, nullptr, 0}};
   const unsigned nh = *hits.count < hits.capacity ? *hits.count : hits.capacity;
   const uint64_t npeel = (uint64_t)m.npeel;
+  const int k = (int)(blockIdx.x % (unsigned)npeel);
+  const unsigned parts = gridDim.x / (unsigned)npeel;
+  const unsigned part = blockIdx.x / (unsigned)npeel;
+  const unsigned per = (nh + parts - 1) / parts;
+  const unsigned h0 = part * per, h1 = (h0 + per < nh) ? h0 + per : nh;
+  const int ns = m.nfreq * m.nap * kNo;            // SED bins of one direction and component
+  float* sed = nullptr;
+  if (sed_lds) {
+    sed = (float*)((char*)lds_tables + rp.lds.bytes);
+    for (int i = threadIdx.x; i < 9 * ns; i += blockDim.x) sed[i] = 0.0f;
+    __syncthreads();
+  }
   const PeelRec<T>* recs = (const PeelRec<T>*)rp.q.recs;
-  for (unsigned h = blockIdx.x * blockDim.x + threadIdx.x; h < nh; h += gridDim.x * blockDim.x) {
+  for (unsigned h = h0 + threadIdx.x; h < h1; h += blockDim.x) {
     const uint32_t item = hits.item[h];
     if (item == kSlotHole) continue;
     const uint64_t ri = item / npeel;
-    const int k = (int)(item - ri * npeel);
+    if ((int)(item - ri * npeel) != k) continue;
     const PeelRec<T>& rec = recs[ri];
     T s0;
     (void)peel_weight<T>(m, rec, k, s0);
     Pkt<T> q;
     rec_to_pkt<T>(rec, q);
-    peel_tally<T>(C, q, s0, k, rec.g, rec.pl, hits.xe[h], hits.ye[h], hits.ze[h], (T)hits.tsum[h]);
+    peel_tally<T>(C, q, s0, k, rec.g, rec.pl, hits.xe[h], hits.ye[h], hits.ze[h], (T)hits.tsum[h], sed);
+  }
+  if (sed_lds) {
+    __syncthreads();
+    // t.psed[c][indx][ia][k][inub]: slice entry (c, j) with j = inub-1 + nfreq*(ia-1 + nap*(indx-1))
+    const size_t nb = (size_t)m.nfreq * npeel * m.nap * kNo;
+    for (int i = threadIdx.x; i < 9 * ns; i += blockDim.x) {
+      const float v = sed[i];
+      if (v != 0.0f) {
+        const int c = i / ns, j = i - c * ns;
+        const int inub = j % m.nfreq, rest = j / m.nfreq;          // rest = ia-1 + nap*(indx-1)
+        atomicAdd(&t.psed[(size_t)c * nb + (size_t)inub + (size_t)m.nfreq * ((size_t)k + npeel * (size_t)rest)], v);
+      }
+    }
   }
   counts_flush(t, s_cnt);
 }

@@ -266,10 +266,17 @@ struct Life {
   double acc_v;
   uint32_t acc_dbig, acc_nabs;
   T td;
+  // wall_clearance at the packet's position, from peel_hidden at an
+  // interaction for the walk_step that starts the next flight there
+  double wclear;
+  bool wclear_ok;
 };
 
 template <class T>
-__device__ __forceinline__ void cell_cache_reset(Life<T>& L) { L.c_rho = L.c_sp = L.c_td = 0xFFFFFFFFu; }
+__device__ __forceinline__ void cell_cache_reset(Life<T>& L) {
+  L.c_rho = L.c_sp = L.c_td = 0xFFFFFFFFu;
+  L.wclear_ok = false;
+}
 
 template <class T>
 __device__ __forceinline__ int cell_species(const DeviceModel& m, Life<T>& L, size_t c) {
@@ -376,7 +383,9 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
   Pkt<T>& p = L.p;
   G dt;
   int iface;
-  if constexpr (!kLazyTestgrid<T>) { if (testgrid<T>(m, p, C.rng)) C.cnt.add(C_RESETS); }
+  bool cached = L.wclear_ok;            // set at this position and cell by peel_hidden
+  L.wclear_ok = false;
+  if constexpr (!kLazyTestgrid<T>) { if (testgrid<T>(m, p, C.rng)) { C.cnt.add(C_RESETS); cached = false; } }
   // In-cell fast path.  At the start of a flight (tsum = 0) off any wall, if
   // the flight length tau / opac is below half the wall clearance, every face
   // root is longer than twice the flight, so the reference's test
@@ -391,7 +400,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
     const T opac = cell_opac_l<T>(m, L, c, iop, rho);
     if (opac > (T)0) {
       const T ds = (L.tau - L.tsum) / opac;
-      if ((G)ds < 0.5 * wall_clearance<T>(m, p)) {
+      if ((G)ds < 0.5 * (cached ? L.wclear : wall_clearance<T>(m, p))) {
         ++L.nseg;
         C.cnt.add_lanes(C_SEGMENTS);
         if (iop >= 0) tally_abs<kComb>(C, L, c, (double)ds * (double)pick(p.kapd, iop) * (double)rho * (1.0 - (double)pick(p.op, iop).albedo));
@@ -730,12 +739,14 @@ __device__ __forceinline__ int peel_ray_first(Ctx<T>& C, PeelWalk<T>& w, PeelRay
 // selection, so a wavefront runs one stream of instructions with every
 // walking lane active instead of one masked stream per axis.  The arithmetic
 // of each lane is that of the wall it crosses.
-template <class T>
+// kCountSeg = false: the caller counts the walk's segments (w.nseg) itself.
+template <class T, bool kCountSeg = true>
 __device__ __forceinline__ int peel_ray_step(Ctx<T>& C, PeelWalk<T>& w, PeelRay<T>& R) {
   const DeviceModel& m = C.m;
   Pkt<T>& p = w.p;
-  if (++w.nseg > m.max_segments) { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_PEEL); return 2; }
-  C.cnt.add_lanes(C_PEELSEG);
+  if (w.nseg >= m.max_segments) { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_PEEL); return 2; }
+  ++w.nseg;                     // w.nseg - 1 ray steps walked (the first crossing is one more)
+  if constexpr (kCountSeg) C.cnt.add_lanes(C_PEELSEG);
   // 0 radial, 1 lower polar wall, 2 upper polar wall, 3 azimuthal
   int axis = 0;
   G tn = R.tr;
@@ -880,10 +891,12 @@ __device__ __forceinline__ bool band_weight(const DeviceModel& m, int b, T freq,
   return true;
 }
 
-// :241-347: image and peeled-SED binning
+// :241-347: image and peeled-SED binning.  sed_k != null: the peeled SED
+// of direction k goes to that slice ([9][8][nap][nfreq], the layout of
+// t.psed without the direction; in LDS) instead of t.psed.
 template <class T>
 __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int k, T gsc, T plsc,
-                                           G xe, G ye, G ze, T tsum) {
+                                           G xe, G ye, G ze, T tsum, float* sed_k = nullptr) {
   const DeviceModel& m = C.m;
   const DeviceTallies& t = C.t;
   C.cnt.add(C_PEELEVT);
@@ -911,6 +924,26 @@ __device__ __forceinline__ void peel_tally(Ctx<T>& C, const Pkt<T>& q, T s0, int
   int inub = freq_bin<T>(m, q.nub);
   int i_inter = (q.iflag == 1) ? 2 : ((q.i_orig == 1) ? 1 : 3);
   int indx[3] = {1, 1 + q.i_orig, 5 + i_inter};
+  if (sed_k) {
+    const int ns = m.nfreq * m.nap * kNo;
+    for (int ia = 1; ia <= m.nap; ++ia) {
+      if (rho2 < m.aperture2[ia - 1]) {
+        for (int j = 0; j < 3; ++j) {
+          float* o = sed_k + (inub - 1) + m.nfreq * ((ia - 1) + m.nap * (indx[j] - 1));
+          atomicAdd(o + 0 * ns, (float)s[0]);
+          atomicAdd(o + 1 * ns, (float)s[1]);
+          atomicAdd(o + 2 * ns, (float)s[2]);
+          atomicAdd(o + 3 * ns, (float)s[3]);
+          atomicAdd(o + 4 * ns, (float)(s[0] * s[0]));
+          atomicAdd(o + 5 * ns, (float)(s[1] * s[1]));
+          atomicAdd(o + 6 * ns, (float)(s[2] * s[2]));
+          atomicAdd(o + 7 * ns, (float)(s[3] * s[3]));
+          atomicAdd(o + 8 * ns, 1.0f);
+        }
+      }
+    }
+    return;
+  }
   size_t nb = (size_t)m.nfreq * m.npeel * m.nap * kNo;
   for (int ia = 1; ia <= m.nap; ++ia) {
     if (rho2 < m.aperture2[ia - 1]) {
@@ -1047,7 +1080,9 @@ __device__ __forceinline__ bool peel_hidden(const DeviceModel& m, Life<T>& L) {
   int iop;
   T rho;
   const T opac = cell_opac_l<T>(m, L, cell_index(m, p.ir, p.it, p.ip), iop, rho);
-  return (double)opac * wall_clearance<T>(m, p) > 1.001 * m.taumax;
+  L.wclear = wall_clearance<T>(m, p);
+  L.wclear_ok = true;
+  return (double)opac * L.wclear > 1.001 * m.taumax;
 }
 
 // propagate_5_fuv.f90:62-198: one interaction (scatter, or absorb and
