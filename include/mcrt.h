@@ -403,7 +403,8 @@ enum {
   MCRT_STAT_WALK_ERRORS = 10,  /* a transport flight found no wall or passed the segment guard */
   MCRT_STAT_PEEL_ERRORS = 11,  /* a peel-off walk did */
   MCRT_STAT_OTHER_ERRORS = 12, /* an emission placed nowhere, a full peel-off queue */
-  MCRT_NSTATS = 13
+  MCRT_STAT_TAIL_ROUNDS = 13,  /* rounds run with one wavefront per remaining packet */
+  MCRT_NSTATS = 14
 };
 int mcrt_run_stats(mcrt_ctx *ctx, double *out);
 

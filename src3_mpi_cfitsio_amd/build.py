@@ -37,9 +37,10 @@ UNITS = [
 # lanes (235 -> 281 VGPRs once took k_transport<float> from 2 waves to 1 and
 # c1 from 1.5 to 1.3 M packets/s).
 MIN_OCCUPANCY = {
-    "_ZN4mcrt11k_transportIfEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
-    "_ZN4mcrt6k_peelIfEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 3,
+    "_ZN4mcrt11k_transportIfLb0EEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
+    "_ZN4mcrt11k_transportIfLb1EEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
     "_ZN4mcrt6k_peelIdEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsE": 2,
+    "_ZN4mcrt11k_peel_walkIfEEvNS_11DeviceModelENS_13DeviceTalliesENS_9RunParamsENS_8PeelRaysENS_8PeelHitsEj": 4,
 }
 
 
@@ -61,8 +62,10 @@ def kernel_resources(remarks):
     return out
 
 
-def check_occupancy(res):
+def check_occupancy(res, unit):
     for name, need in MIN_OCCUPANCY.items():
+        if unit == "kernels_mixed.hip" and ("IfE" in name or "IfL" in name) and name not in res:
+            raise RuntimeError(f"{name}: no resource remark (kernel renamed? update MIN_OCCUPANCY)")
         if name in res:
             got = int(res[name].get("Occupancy [waves/SIMD]", "0"))
             if got < need:
@@ -87,7 +90,7 @@ def build_hip(verbose=False):
         out = _run(cmd)
         res = kernel_resources(out)
         if res:
-            check_occupancy(res)
+            check_occupancy(res, src)
             with open(os.path.join(BUILD, src.replace(".hip", ".resources.txt")), "w") as f:
                 for k, v in sorted(res.items()):
                     f.write(f"{k} " + " ".join(f"{a}={b}" for a, b in v.items()) + "\n")

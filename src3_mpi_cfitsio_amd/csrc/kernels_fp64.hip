@@ -30,7 +30,35 @@ hipError_t launch_setemiss(const EmissArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
-  hipLaunchKernelGGL(k_transport<double>, dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
+  if (plain_model(m))
+    hipLaunchKernelGGL((k_transport<double, true>), dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
+  else
+    hipLaunchKernelGGL((k_transport<double, false>), dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const uint32_t* live,
+                             unsigned nlive, hipStream_t s) {
+  if (nlive == 0) return hipSuccess;
+  const unsigned grid = (nlive + 3u) / 4u;      // four packets per 256-thread block
+  if (plain_model(m))
+    hipLaunchKernelGGL((k_tail<double, true>), dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp, live, nlive);
+  else
+    hipLaunchKernelGGL((k_tail<double, false>), dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp, live, nlive);
+  return hipGetLastError();
+}
+
+int tail_capacity_fp64(const DeviceModel& m, unsigned lds_bytes, int cu_count) {
+  int n = 0;
+  const hipError_t e = plain_model(m)
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tail<double, true>, 256, lds_bytes)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tail<double, false>, 256, lds_bytes);
+  if (e != hipSuccess) return 0;
+  return n * 4 * cu_count;
+}
+
+hipError_t launch_live_list(const uint8_t* lane_live, uint64_t lanes, uint32_t* out, unsigned* count, hipStream_t s) {
+  hipLaunchKernelGGL(k_live_list<0>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, lane_live, lanes, out, count);
   return hipGetLastError();
 }
 
@@ -44,9 +72,12 @@ hipError_t launch_acc_place_fp64(const DeviceModel& m, uint64_t seed, uint64_t f
 
 int lane_words_fp64() { return lane_words<double>(); }
 
-int transport_blocks_per_cu_fp64(unsigned lds_bytes) {
+int transport_blocks_per_cu_fp64(const DeviceModel& m, unsigned lds_bytes) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<double>, 256, lds_bytes) != hipSuccess) return 0;
+  const hipError_t e = plain_model(m)
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<double, true>, 256, lds_bytes)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<double, false>, 256, lds_bytes);
+  if (e != hipSuccess) return 0;
   return n;
 }
 

@@ -7,8 +7,31 @@
 namespace mcrt {
 
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s) {
-  hipLaunchKernelGGL(k_transport<float>, dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
+  if (plain_model(m))
+    hipLaunchKernelGGL((k_transport<float, true>), dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
+  else
+    hipLaunchKernelGGL((k_transport<float, false>), dim3((unsigned)(rp.lanes / 256)), dim3(256), rp.lds.bytes, s, m, t, rp);
   return hipGetLastError();
+}
+
+hipError_t launch_tail_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const uint32_t* live,
+                             unsigned nlive, hipStream_t s) {
+  if (nlive == 0) return hipSuccess;
+  const unsigned grid = (nlive + 3u) / 4u;      // four packets per 256-thread block
+  if (plain_model(m))
+    hipLaunchKernelGGL((k_tail<float, true>), dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp, live, nlive);
+  else
+    hipLaunchKernelGGL((k_tail<float, false>), dim3(grid), dim3(256), rp.lds.bytes, s, m, t, rp, live, nlive);
+  return hipGetLastError();
+}
+
+int tail_capacity_mixed(const DeviceModel& m, unsigned lds_bytes, int cu_count) {
+  int n = 0;
+  const hipError_t e = plain_model(m)
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tail<float, true>, 256, lds_bytes)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tail<float, false>, 256, lds_bytes);
+  if (e != hipSuccess) return 0;
+  return n * 4 * cu_count;
 }
 
 hipError_t launch_acc_place_mixed(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out,
@@ -21,9 +44,12 @@ hipError_t launch_acc_place_mixed(const DeviceModel& m, uint64_t seed, uint64_t 
 
 int lane_words_mixed() { return lane_words<float>(); }
 
-int transport_blocks_per_cu_mixed(unsigned lds_bytes) {
+int transport_blocks_per_cu_mixed(const DeviceModel& m, unsigned lds_bytes) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<float>, 256, lds_bytes) != hipSuccess) return 0;
+  const hipError_t e = plain_model(m)
+      ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<float, true>, 256, lds_bytes)
+      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_transport<float, false>, 256, lds_bytes);
+  if (e != hipSuccess) return 0;
   return n;
 }
 

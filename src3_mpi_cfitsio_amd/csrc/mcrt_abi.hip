@@ -920,7 +920,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   const LdsPlan lds = plan_lds(c->dm);
   int64_t lanes = c->lanes;
   if (lanes <= 0) {
-    int bpc = fp64 ? transport_blocks_per_cu_fp64(lds.bytes) : transport_blocks_per_cu_mixed(lds.bytes);
+    int bpc = fp64 ? transport_blocks_per_cu_fp64(dm, lds.bytes) : transport_blocks_per_cu_mixed(dm, lds.bytes);
     if (bpc < 1) bpc = 1;
     lanes = (int64_t)c->cu_count * bpc * 256;
   }
@@ -1026,6 +1026,21 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     rays.capacity = (unsigned)n;
     hits.capacity = (unsigned)n;
   }
+  // The end of the run: once every packet is handed out and at most
+  // tail_max remain, each runs in a wavefront of its own (k_tail).
+  // (at most as many as k_tail keeps resident at once)
+  int64_t tail_max = fp64 ? tail_capacity_fp64(dm, lds.bytes, c->cu_count) : tail_capacity_mixed(dm, lds.bytes, c->cu_count);
+  if (const char* tl = std::getenv("MCRT_TAIL_LANES")) tail_max = std::atoll(tl);   // 0: never
+  bool tail_mode = false;
+  unsigned nlive = 0;
+  uint32_t* tail_list = nullptr;
+  unsigned* tail_count = nullptr;
+  if (tail_max > 0) {
+    if ((rc = dev_alloc(c, "tail.list", (size_t)lanes * 4, &p))) return rc;
+    tail_list = (uint32_t*)p;
+    if ((rc = dev_alloc(c, "tail.count", 16, &p))) return rc;
+    tail_count = (unsigned*)p;
+  }
   unsigned walk_fetch = c->peel_fetch > 0 ? (unsigned)c->peel_fetch : 16u;
   if (const char* pw = std::getenv("MCRT_PEEL_WALK_FETCH")) {
     const int v = std::atoi(pw);
@@ -1083,7 +1098,16 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     // q_count and live restart every round (a fill kernel, not a memset)
     HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
     HIPCHECK(c, hipEventRecord(c->evr[0], c->stream));
-    hipError_t e = fp64 ? launch_transport_fp64(dm, t, rp, c->stream) : launch_transport_mixed(dm, t, rp, c->stream);
+    hipError_t e;
+    if (tail_mode) {
+      HIPCHECK(c, launch_fill_i32((int*)tail_count, 1, 0, c->stream));
+      HIPCHECK(c, launch_live_list(lane_live, (uint64_t)lanes, tail_list, tail_count, c->stream));
+      e = fp64 ? launch_tail_fp64(dm, t, rp, tail_list, nlive, c->stream)
+               : launch_tail_mixed(dm, t, rp, tail_list, nlive, c->stream);
+      st[MCRT_STAT_TAIL_ROUNDS] += 1.0;
+    } else {
+      e = fp64 ? launch_transport_fp64(dm, t, rp, c->stream) : launch_transport_mixed(dm, t, rp, c->stream);
+    }
     if (e != hipSuccess) return hipfail(c, e, "transport kernel launch");
     HIPCHECK(c, hipEventRecord(c->evr[1], c->stream));
     if (queued && split) {
@@ -1119,6 +1143,8 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
                    (long long)round, ta, tb, h.q_count, h.live, (long long)events, h.next_packet, (long long)npackets);
     if (h.next_packet < (unsigned long long)npackets) continue;
     if (h.live == 0) break;
+    if ((int64_t)h.live <= tail_max) tail_mode = true;
+    nlive = h.live;
   }
   st[MCRT_STAT_ROUNDS] = (double)(round + 1);
   {

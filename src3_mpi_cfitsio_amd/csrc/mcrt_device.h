@@ -441,10 +441,17 @@ __device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
 // flushed to the device counters once per launch.
 struct LocalCounts {
   unsigned long long* s;    // C_NCOUNTERS LDS slots of this workgroup
-  __device__ __forceinline__ void add(int i, unsigned long long v = 1ull) const { atomicAdd(&s[i], v); }
+  // one: every lane of the wavefront carries the same packet (k_tail), which
+  // counts once
+  bool one = false;
+  __device__ __forceinline__ void add(int i, unsigned long long v = 1ull) const {
+    if (one && __lane_id() != 0) return;
+    atomicAdd(&s[i], v);
+  }
   // +1 from every active lane, as one atomic per wavefront: the per-segment
   // counters would otherwise serialize 64 same-address LDS atomics per step
   __device__ __forceinline__ void add_lanes(int i) const {
+    if (one) { if (__lane_id() == 0) atomicAdd(&s[i], 1ull); return; }
     const unsigned long long mask = __ballot(1);
     if (__lane_id() == __ffsll((long long)mask) - 1) atomicAdd(&s[i], (unsigned long long)__popcll(mask));
   }

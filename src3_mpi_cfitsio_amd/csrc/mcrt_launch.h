@@ -28,14 +28,24 @@ struct BatchLaunch {
 // rp.lanes must be a multiple of 256
 hipError_t launch_transport_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
 hipError_t launch_transport_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, hipStream_t s);
+// k_tail: one wavefront per parked packet of the lane slots live[0 .. nlive)
+hipError_t launch_tail_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const uint32_t* live,
+                            unsigned nlive, hipStream_t s);
+hipError_t launch_tail_mixed(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, const uint32_t* live,
+                             unsigned nlive, hipStream_t s);
+// packets k_tail keeps resident at once (one wavefront each)
+int tail_capacity_fp64(const DeviceModel& m, unsigned lds_bytes, int cu_count);
+int tail_capacity_mixed(const DeviceModel& m, unsigned lds_bytes, int cu_count);
+// the lane slots holding a parked packet (count zeroed before)
+hipError_t launch_live_list(const uint8_t* lane_live, uint64_t lanes, uint32_t* out, unsigned* count, hipStream_t s);
 // k_acc_place over the run's accretion photons k < m.acc_to
 hipError_t launch_acc_place_fp64(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out, hipStream_t s);
 hipError_t launch_acc_place_mixed(const DeviceModel& m, uint64_t seed, uint64_t first_packet, AccStart* out, hipStream_t s);
 int lane_words_fp64();
 int lane_words_mixed();
 // resident 256-thread blocks of k_transport per CU with lds_bytes of dynamic LDS (0: query failed)
-int transport_blocks_per_cu_fp64(unsigned lds_bytes);
-int transport_blocks_per_cu_mixed(unsigned lds_bytes);
+int transport_blocks_per_cu_fp64(const DeviceModel& m, unsigned lds_bytes);
+int transport_blocks_per_cu_mixed(const DeviceModel& m, unsigned lds_bytes);
 hipError_t launch_adjoint_fp64(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_adjoint_mixed(const DeviceModel& m, const DeviceTallies& t, const AdjParams& ap, int grid, hipStream_t s);
 hipError_t launch_peel_fp64(const DeviceModel& m, const DeviceTallies& t, const RunParams& rp, int grid, hipStream_t s);

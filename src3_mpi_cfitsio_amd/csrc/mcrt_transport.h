@@ -129,7 +129,15 @@ struct Ctx {
   PeelQueue q;
   unsigned slot_next, slot_end;   // this lane's reserved, not yet written queue slots
   AdjState* adj;                  // null except in k_adjoint (the hooks then fold away)
+  // k_tail: all 64 lanes of a wavefront carry the same packet; lane 0 alone
+  // writes, and the rejection sampling of the scattering angle tries 64
+  // candidates at once
+  bool coop = false;
 };
+
+// the lane that writes the packet's tallies and records
+template <class T>
+__device__ __forceinline__ bool writer(const Ctx<T>& C) { return !C.coop || __lane_id() == 0; }
 
 template <class T> struct Life;
 template <class T> struct Pkt;
@@ -150,8 +158,14 @@ constexpr uint32_t kSlotHole = 0xFFFFFFFFu;
 
 template <class T> __device__ __forceinline__ void atomic_add_dtau(double* a, T v) { atomicAdd(a, (double)v); }
 
-// One atomic per wavefront: each active lane reserves `per` consecutive slots.
-__device__ __forceinline__ unsigned reserve_slots(unsigned* counter, unsigned per) {
+// One atomic per wavefront: each active lane reserves `per` consecutive slots
+// (coop: the wavefront's one packet reserves them, every lane sees the base).
+__device__ __forceinline__ unsigned reserve_slots(unsigned* counter, unsigned per, bool coop = false) {
+  if (coop) {
+    unsigned b = 0;
+    if (__lane_id() == 0) b = atomicAdd(counter, per);
+    return __shfl(b, 0, 64);
+  }
   unsigned long long mask = __ballot(1);
   int lane = __lane_id();
   int leader = __ffsll((long long)mask) - 1;
@@ -166,7 +180,7 @@ __device__ __forceinline__ unsigned reserve_slots(unsigned* counter, unsigned pe
 template <class T>
 __device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
   if (C.slot_next == C.slot_end) {
-    C.slot_next = reserve_slots(C.q.count, kSlotChunk);
+    C.slot_next = reserve_slots(C.q.count, kSlotChunk, C.coop);
     C.slot_end = C.slot_next + kSlotChunk;
   }
   return C.slot_next++;
@@ -178,7 +192,7 @@ __device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
 template <class T>
 __device__ __forceinline__ bool hold_slot(Ctx<T>& C) {
   if (C.q.capacity == 0 || C.slot_next < C.slot_end) return true;
-  const unsigned base = reserve_slots(C.q.count, kSlotChunk);
+  const unsigned base = reserve_slots(C.q.count, kSlotChunk, C.coop);
   if (base >= C.q.capacity) { C.slot_next = C.slot_end = 0u; return false; }
   C.slot_next = base;
   C.slot_end = (base + kSlotChunk < C.q.capacity) ? base + kSlotChunk : C.q.capacity;
@@ -318,7 +332,7 @@ __device__ __forceinline__ T cell_opac_l(const DeviceModel& m, Life<T>& L, size_
 // counts are unchanged; dtauabs changes only by the order of its FP64 sums.
 template <class T>
 __device__ __forceinline__ void tally_flush(Ctx<T>& C, Life<T>& L) {
-  if (L.acc_c != 0xFFFFFFFFu) {
+  if (L.acc_c != 0xFFFFFFFFu && writer(C)) {
     if (L.acc_v != 0.0) atomicAdd(&C.t.dtauabs[L.acc_c], L.acc_v);
     if (L.acc_dbig) atomicAdd(&C.t.dbig[L.acc_c], (int)L.acc_dbig);
     if (L.acc_nabs) atomicAdd(&C.t.nabs[L.acc_c], (int)L.acc_nabs);
@@ -343,7 +357,7 @@ __device__ __forceinline__ void tally_abs(Ctx<T>& C, Life<T>& L, size_t c, doubl
     if (L.acc_c != (uint32_t)c) { tally_flush<T>(C, L); L.acc_c = (uint32_t)c; }
     L.acc_v += v;
   } else {
-    atomicAdd(&C.t.dtauabs[c], v);
+    if (writer(C)) atomicAdd(&C.t.dtauabs[c], v);
   }
 }
 
@@ -354,7 +368,7 @@ __device__ __forceinline__ void tally_absorb(Ctx<T>& C, Life<T>& L, size_t c, bo
     if (L.acc_c != (uint32_t)c) { tally_flush<T>(C, L); L.acc_c = (uint32_t)c; }
     L.acc_dbig++;
     if (nabs) L.acc_nabs++;
-  } else {
+  } else if (writer(C)) {
     atomicAdd(&C.t.dbig[c], 1);
     if (nabs) atomicAdd(&C.t.nabs[c], 1);
   }
@@ -996,7 +1010,7 @@ __device__ __forceinline__ void peel_event(Ctx<T>& C, const Pkt<T>& p, int sp, i
   if constexpr (kAdj) { adj_peel_event<T>(C, p, sp, iop, false, (T)0, (T)0); return; }
   if (C.q.capacity == 0) return;          // no observer directions
   const unsigned slot = take_slot(C);
-  if (slot < C.q.capacity) write_rec<T>(C, p, slot, sp, iop, false, (T)0, (T)0);
+  if (slot < C.q.capacity) { if (writer(C)) write_rec<T>(C, p, slot, sp, iop, false, (T)0, (T)0); }
   else { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_OTHER); }   // the host sizes the queue so this cannot happen
 }
 
@@ -1006,37 +1020,91 @@ __device__ __forceinline__ void peel_direct(Ctx<T>& C, const Pkt<T>& p, T coslp,
   if constexpr (kAdj) { adj_peel_event<T>(C, p, 0, 0, true, coslp, sinlp); return; }
   if (C.q.capacity == 0) return;
   const unsigned slot = take_slot(C);
-  if (slot < C.q.capacity) write_rec<T>(C, p, slot, 0, 0, true, coslp, sinlp);
+  if (slot < C.q.capacity) { if (writer(C)) write_rec<T>(C, p, slot, 0, 0, true, coslp, sinlp); }
   else { C.cnt.add(C_ERRORS); C.cnt.add(C_ERR_OTHER); }
+}
+
+// One trial of stokes.f90:75-136: the candidate scattering angle bmu and
+// rotation ri1 from the trial's first two draws, and its weight rprob for
+// the incoming Stokes vector.
+template <class T>
+__device__ __forceinline__ void stokes_trial(T g, T pl, const Pkt<T>& p, T u1, T u2, T& bmu, T& ri1, T& rprob) {
+  const T PI = (T)kPi, R2P = (T)kR2p, one = (T)1, two = (T)2;
+  T pp[4];
+  bmu = one - two * u1;
+  T cosb2 = bmu * bmu;
+  dustmat<T>(g, pl, bmu, cosb2, pp);
+  ri1 = R2P * u2;
+  if (ri1 > PI) {
+    T ri3 = R2P - ri1, cosi3 = RT<T>::cos_(ri3), sini3 = RT<T>::sin_(ri3);
+    T sin2i3 = two * sini3 * cosi3, cos2i3 = two * (cosi3 * cosi3) - one;
+    rprob = (pp[0] * p.sip + pp[1] * cos2i3 * p.sqp + pp[1] * sin2i3 * p.sup) / p.sip;
+  } else {
+    T cosi1 = RT<T>::cos_(ri1), sini1 = RT<T>::sin_(ri1);
+    T sin2i1 = two * sini1 * cosi1, cos2i1 = two * (cosi1 * cosi1) - one;
+    rprob = (pp[0] * p.sip + pp[1] * cos2i1 * p.sqp + (-pp[1] * sin2i1) * p.sup) / p.sip;
+  }
+}
+
+// The rejection loop of stokes.f90:75-136 with the 64 lanes of a wavefront
+// that all carry the same packet (k_tail): lane j tries trial done + j on
+// its own draws (3 per trial, found by seeking the packet's counter-based
+// stream), the running peak of the serial loop is the prefix maximum of the
+// trials' weights, and the first accepted trial is the serial loop's.  The
+// stream then continues after that trial's draws, as after the serial loop.
+template <class T>
+__device__ __forceinline__ void stokes_coop(Ctx<T>& C, T g, T pl, const Pkt<T>& p, T peak, T& bmu, T& ri1) {
+  const int lane = __lane_id();
+  const uint32_t n0 = C.rng.n;
+  for (uint32_t done = 0;; done += 64u) {
+    Rng r = C.rng;
+    rng_seek(r, n0 + 3u * (done + (uint32_t)lane));
+    const T u1 = uniform<T>(r);
+    const T u2 = uniform<T>(r);
+    T b, ri, rprob;
+    stokes_trial<T>(g, pl, p, u1, u2, b, ri, rprob);
+    const T xran = uniform<T>(r);
+    T pm = rprob;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const T v = __shfl_up(pm, off, 64);
+      if (lane >= off && v > pm) pm = v;
+    }
+    const T pk = (pm > peak) ? pm : peak;
+    const unsigned long long acc = __ballot(!(pk * xran > rprob));
+    if (acc) {
+      const int t = __ffsll((long long)acc) - 1;
+      bmu = __shfl(b, t, 64);
+      ri1 = __shfl(ri, t, 64);
+      C.cnt.add(C_HTOT, (unsigned long long)(done + (uint32_t)t + 1u));
+      C.cnt.add(C_HIT);
+      rng_seek(C.rng, n0 + 3u * (done + (uint32_t)t + 1u));
+      return;
+    }
+    peak = __shfl(pk, 63, 64);
+  }
 }
 
 // stokes.f90:46-137 rejection sampling, then the Mueller update
 template <class T>
 __device__ __forceinline__ void stokes(Ctx<T>& C, Pkt<T>& p, int iop) {
-  const T PI = (T)kPi, R2P = (T)kR2p, one = (T)1, two = (T)2;
   const Opac<T> o = pick(p.op, iop);
   T g = o.g, pl = o.pl;
-  T peak = cs_peak<T>(g), bmu, ri1, rprob, pp[4];
-  for (;;) {
-    C.cnt.add_lanes(C_HTOT);
-    bmu = one - two * uniform<T>(C.rng);
-    T cosb2 = bmu * bmu;
-    dustmat<T>(g, pl, bmu, cosb2, pp);
-    ri1 = R2P * uniform<T>(C.rng);
-    if (ri1 > PI) {
-      T ri3 = R2P - ri1, cosi3 = RT<T>::cos_(ri3), sini3 = RT<T>::sin_(ri3);
-      T sin2i3 = two * sini3 * cosi3, cos2i3 = two * (cosi3 * cosi3) - one;
-      rprob = (pp[0] * p.sip + pp[1] * cos2i3 * p.sqp + pp[1] * sin2i3 * p.sup) / p.sip;
-    } else {
-      T cosi1 = RT<T>::cos_(ri1), sini1 = RT<T>::sin_(ri1);
-      T sin2i1 = two * sini1 * cosi1, cos2i1 = two * (cosi1 * cosi1) - one;
-      rprob = (pp[0] * p.sip + pp[1] * cos2i1 * p.sqp + (-pp[1] * sin2i1) * p.sup) / p.sip;
+  T peak = cs_peak<T>(g), bmu, ri1, rprob;
+  if (C.coop) {
+    stokes_coop<T>(C, g, pl, p, peak, bmu, ri1);
+  } else {
+    for (;;) {
+      C.cnt.add_lanes(C_HTOT);
+      const T u1 = uniform<T>(C.rng);
+      const T u2 = uniform<T>(C.rng);
+      stokes_trial<T>(g, pl, p, u1, u2, bmu, ri1, rprob);
+      if (rprob > peak) peak = rprob;
+      T xran = uniform<T>(C.rng);
+      if (peak * xran > rprob) continue;
+      C.cnt.add_lanes(C_HIT);
+      break;
     }
-    if (rprob > peak) peak = rprob;
-    T xran = uniform<T>(C.rng);
-    if (peak * xran > rprob) continue;
-    C.cnt.add_lanes(C_HIT);
-    break;
   }
   T dir[5] = {p.cost, p.sint, p.phi, p.cosp, p.sinp};
   T s[4] = {p.sip, p.sqp, p.sup, p.svp};
@@ -1055,7 +1123,7 @@ __device__ __forceinline__ int flight_end(Ctx<T>& C, Life<T>& L) {
   if (!L.first) {
     if (p.ir != L.pir || p.it != L.pit || p.ip != L.pip) L.iint = 0;
     if (L.iint > m.iintmax) {
-      atomicAdd(&C.t.killed[cell_index(m, p.ir, p.it, p.ip)], 1);
+      if (writer(C)) atomicAdd(&C.t.killed[cell_index(m, p.ir, p.it, p.ip)], 1);
       C.cnt.add(C_ABFLUX);
       return 2;
     }
@@ -1158,12 +1226,15 @@ __device__ __forceinline__ void diffuse(Ctx<T>& C, Pkt<T>& p) {
   // surface temperature after this photon (:258-270) and the emission temperature (:343-361)
   const double dA = kPi * sw * (m.r2arr[p.ir] - m.r2arr[p.ir - 1]);
   const size_t cs = cell_index(m, p.ir, its, p.ip);
-  const unsigned k = atomicAdd(&C.t.diff_count[cs], 1u);
+  unsigned k = 0;
+  if (writer(C)) k = atomicAdd(&C.t.diff_count[cs], 1u);
+  if (C.coop) k = __shfl(k, 0, 64);
   const double Tcap = kTmaxK / 11605.0, T0 = 0.1 / 11605.0;
   double Ts = quarter_power(T0 * T0 * T0 * T0 + (double)(k + 1u) * m.diff_delta / dA);
   if (Ts > Tcap) Ts = Tcap;
   const int mid = upper ? (ntg - 1) / 2 : (ntg + 1) / 2;
-  const double nacc = (double)*(volatile const int*)&C.t.nabs[cell_index(m, p.ir, mid, p.ip)];
+  double nacc = (double)*(volatile const int*)&C.t.nabs[cell_index(m, p.ir, mid, p.ip)];
+  if (C.coop) nacc = __shfl(nacc, 0, 64);
   const double T4eff = m.diff_delta * nacc / dA;
   const double mu = ux * sux + uy * suy + uz * suz;
   double Te = quarter_power(Ts * Ts * Ts * Ts + 0.75 * mu * T4eff);
@@ -1240,7 +1311,7 @@ __device__ __forceinline__ void prop_end(Ctx<T>& C, Life<T>& L) {
   int i_inter = (p.iflag == 1) ? 2 : ((p.i_orig == 1) ? 1 : 3);
   int indx[3] = {1, 1 + p.i_orig, 5 + i_inter};
   size_t nb = (size_t)m.nfreq * m.nmu * m.nph * m.nap * kNo;
-  for (int ia = 1; ia <= m.nap; ++ia) {
+  for (int ia = 1; ia <= m.nap && writer(C); ++ia) {
     if (rho2 < m.aperture2[ia - 1]) {
       for (int j = 0; j < 3; ++j) {
         size_t o = (size_t)(inub - 1) + (size_t)m.nfreq * ((size_t)(itb - 1) + (size_t)m.nmu * ((size_t)(ipb - 1) + (size_t)m.nph * ((size_t)(ia - 1) + (size_t)m.nap * (size_t)(indx[j] - 1))));
@@ -1493,7 +1564,7 @@ __device__ __forceinline__ void end_packet(Ctx<T>& C, Life<T>& L, int code, int 
   if (code == 1) { prop_end<T>(C, L); C.cnt.add(C_ESCAPED); }
   else if (code == 2) C.cnt.add(C_KILLED);
   else { C.cnt.add(C_ERRORS); C.cnt.add(err_kind); }
-  if (C.t.trace) {
+  if (C.t.trace && writer(C)) {
     const Pkt<T>& p = L.p;
     double* tr = C.t.trace + 8 * L.pid;
     tr[0] = (double)p.nub; tr[1] = (double)p.cost; tr[2] = (double)p.phi; tr[3] = (double)p.sip;
@@ -1618,13 +1689,30 @@ enum : int { L_IDLE = 0, L_WALK = 1, L_READY = 2 };
 template <class T>
 constexpr bool kCombTally = std::is_same<T, float>::value;
 
-template <class T>
+// kPlain: the model has one opacity table and no accretion photons, ISRF,
+// hot spots or diffusion region (plain_model()).  Those fields of the
+// kernel's model copy are then constants, so the compiler drops their code:
+// k_transport<float> keeps ~15k instead of ~24k instructions and spills a
+// third of the scalar registers (each spilled value costs a lane read where
+// it is used).
+__host__ __device__ inline bool plain_model(const DeviceModel& m) {
+  return m.acc_to == 0 && m.nisrf == 0 && m.ispot == 0 && m.diff_itmin == nullptr && m.nopac == 1;
+}
+template <bool kPlain>
+__device__ __forceinline__ void specialize(DeviceModel& m) {
+  if constexpr (kPlain) {
+    m.acc_to = 0; m.nisrf = 0; m.ispot = 0; m.diff_itmin = nullptr; m.nopac = 1;
+  }
+}
+
+template <class T, bool kPlain>
 __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies t, RunParams rp) {
   constexpr bool kComb = kCombTally<T>;
   __shared__ unsigned long long s_cnt[C_NCOUNTERS];
   extern __shared__ double lds_tables[];
   counts_begin(s_cnt);
   DeviceModel m = m0;
+  specialize<kPlain>(m);
   const double* inczeta = rp.inczeta;
   stage_tables(m, inczeta, rp.lds, lds_tables);
   Ctx<T> C{m, t, {}, {s_cnt}, inczeta, rp.q};
@@ -1694,6 +1782,89 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
     ((PeelRec<T>*)C.q.recs)[sl].flags = kSlotHole;
   const unsigned long long mask = __ballot(live);
   if (__lane_id() == 0 && mask) atomicAdd(&rp.ctl->live, (unsigned)__popcll(mask));
+  counts_flush(t, s_cnt);
+}
+
+// The lanes of the last round that hold a parked packet, in any order
+// (count: zeroed by the host).  (A template only so that every translation
+// unit that includes this header may carry it.)
+template <int kUnused = 0>
+__global__ void __launch_bounds__(256) k_live_list(const uint8_t* lane_live, uint64_t lanes, uint32_t* out,
+                                                   unsigned* count) {
+  const uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = l < lanes && lane_live[l] != 0;
+  const unsigned long long mask = __ballot(live);
+  if (!mask) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  unsigned base = 0;
+  if (__lane_id() == leader) base = atomicAdd(count, (unsigned)__popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (live) out[base + (unsigned)__popcll(mask & ((1ull << __lane_id()) - 1ull))] = (uint32_t)l;
+}
+
+// The end of a run: every packet handed out and few still alive, each
+// running a serial chain of interactions (a packet trapped in optically
+// thick cells may have 10^5-10^6 of them), alone in its wavefront, so the
+// chain's latency sets the run's length.  k_tail gives each of the
+// `nlive` parked packets (lane slots live[]) a whole wavefront: all 64
+// lanes carry the packet and compute identically, lane 0 alone writes its
+// tallies, records and parked state, and the scattering angle's rejection
+// sampling tries 64 candidates at once (stokes_coop).  Results are those of
+// k_transport: same draws, same arithmetic, same order.
+template <class T, bool kPlain>
+__global__ void __launch_bounds__(256) k_tail(DeviceModel m0, DeviceTallies t, RunParams rp, const uint32_t* live,
+                                              unsigned nlive) {
+  constexpr bool kComb = kCombTally<T>;
+  __shared__ unsigned long long s_cnt[C_NCOUNTERS];
+  extern __shared__ double lds_tables[];
+  counts_begin(s_cnt);
+  DeviceModel m = m0;
+  specialize<kPlain>(m);
+  const double* inczeta = rp.inczeta;
+  stage_tables(m, inczeta, rp.lds, lds_tables);
+  Ctx<T> C{m, t, {}, {s_cnt}, inczeta, rp.q};
+  C.coop = true;
+  C.cnt.one = true;
+  const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (wave < nlive) {
+    const uint64_t slot = live[wave];
+    LaneIO io{rp.lane_words + slot, rp.lanes, 0};
+    Life<T> L;
+    tally_open(L);
+    unpark<T>(io, C, L, rp.seed, rp.first_packet);
+    int st = L_READY;
+    uint32_t budget = rp.events;
+    for (;;) {
+      if (st == L_WALK) {
+        const int r = walk_step<T, false, kComb>(C, L);
+        if (r != 0) {
+          const int code = (r == 1) ? flight_end<T>(C, L) : ((r == 2) ? 1 : -1);
+          if (code == 0) st = L_READY;
+          else {
+            if constexpr (kComb) tally_flush<T>(C, L);
+            end_packet<T>(C, L, code);
+            st = L_IDLE;
+          }
+        }
+      }
+      if (st == L_IDLE) break;
+      if (st == L_READY) {
+        if (budget == 0 || !hold_slot(C)) break;
+        interact<T, false, kComb>(C, L);
+        budget--;
+        st = L_WALK;
+      }
+    }
+    const bool alive = (st == L_READY);
+    if constexpr (kComb) tally_flush<T>(C, L);
+    if (__lane_id() == 0) {
+      if (alive) { io.w = 0; park<T>(io, C, L); }
+      rp.lane_live[slot] = alive ? 1 : 0;
+      for (unsigned sl = C.slot_next; sl < C.slot_end && sl < C.q.capacity; ++sl)
+        ((PeelRec<T>*)C.q.recs)[sl].flags = kSlotHole;
+      if (alive) atomicAdd(&rp.ctl->live, 1u);
+    }
+  }
   counts_flush(t, s_cnt);
 }
 

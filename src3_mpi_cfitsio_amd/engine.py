@@ -27,7 +27,7 @@ from .model import LSUN, Batch, Model
 COUNTERS = ["tot", "flux", "sflux", "aflux", "abflux", "scount", "nscat", "hit", "htot",
             "escaped", "killed", "segments", "peel_segments", "peel_events", "resets", "errors"]
 RUN_STATS = ["rounds", "transport_ms", "peel_ms", "peel_launches", "records", "parked", "lanes", "events0",
-             "record_bytes", "lane_bytes", "walk_errors", "peel_errors", "other_errors"]
+             "record_bytes", "lane_bytes", "walk_errors", "peel_errors", "other_errors", "tail_rounds"]
 SED_NAMES = ["si", "sq", "su", "sv", "si2", "sq2", "su2", "sv2", "nums", "aveinc"]
 PSED_NAMES = ["ti", "tq", "tu", "tv", "ti2", "tq2", "tu2", "tv2", "numt"]
 IMAGE_NAMES = ["image_b_i", "image_b_q", "image_b_u", "image_b_v",
