@@ -49,24 +49,32 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(c, st, rec_bytes, lane_bytes):
-    """Algorithmic HBM bytes of the two kernels of a run.
+def kernel_bytes(c, st, rec_bytes, lane_bytes, nopac):
+    """Algorithmic HBM bytes of the two kinds of kernel of a run.
 
-    transport: per face crossing the FP32 density (4 B) + species (1 B) +
+    transport (k_transport, k_tail): per face crossing the FP32 density
+    (4 B) + species (1 B, read only with more than one opacity table) +
     FP64 dtauabs read-modify-write (16 B); per escaped packet its SED row,
     3 components x 10 floats read-modify-write (240 B); per queued peel-off
     record its write (rec_bytes); per parked lane-round the lane state
     written and read back (2 x lane_bytes).
-    peel: per face crossing density + species (5 B); per walk that reaches
-    the observer 3 peeled-SED components x 9 floats (216 B) + image I,Q,U,V,
-    origin (40 B) read-modify-write; per record one read (rec_bytes)."""
-    transport = (c["segments"] * 21.0 + c["escaped"] * 240.0 + st["records"] * rec_bytes
+    peel (k_peel_start, k_peel_walk, k_peel_hits): per face crossing the
+    density (+ species); per walk that reaches the observer 3 peeled-SED
+    components x 9 floats (216 B) + image I,Q,U,V, origin (40 B)
+    read-modify-write; per record one read (rec_bytes).  The ray states and
+    hits the three kernels pass to each other are not algorithmic bytes:
+    they are in the measured traffic."""
+    b_rho = 4.0 + (1.0 if nopac > 1 else 0.0)
+    transport = (c["segments"] * (b_rho + 16.0) + c["escaped"] * 240.0 + st["records"] * rec_bytes
                  + st["parked"] * 2.0 * lane_bytes)
-    peel = c["peel_segments"] * 5.0 + c["peel_events"] * 256.0 + st["records"] * rec_bytes
+    peel = c["peel_segments"] * b_rho + c["peel_events"] * 256.0 + st["records"] * rec_bytes
     return transport, peel
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_peel_k_transport_c1_1e8_mixed_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_transport_k_peel_split_c1_hbm_traffic.md")
+# the kernels of a mixed-precision run, named as in the traffic summary
+TRANSPORT_KERNELS = "k_transport+k_tail<float>"
+PEEL_KERNELS = "k_peel_start+k_peel_walk+k_peel_hits<float>"
 
 
 def measured_traffic(kernel):
@@ -432,14 +440,19 @@ def run_single_workload(args, T0):
     kmean = float(np.mean(kern_ms))
     c = {k: float(np.mean([cc[k] for cc in counters])) for k in counters[0]}
     st = {k: float(np.mean([ss[k] for ss in stats])) for k in stats[0]}
-    b_tr, b_pe = kernel_bytes(c, st, st["record_bytes"], st["lane_bytes"])
-    # the dominant kernel's roofline: algorithmic bytes over its summed
-    # HIP-event time (every launch of the step bracketed on the run stream)
-    if st["peel_ms"] >= st["transport_ms"]:
-        kname, kms, kB, nl = "k_peel<float>", st["peel_ms"], b_pe, st["peel_launches"]
-    else:
-        kname, kms, kB, nl = "k_transport<float>", st["transport_ms"], b_tr, st["rounds"]
-    achieved = kB / (kms / 1e3) / 1e9
+    b_tr, b_pe = kernel_bytes(c, st, st["record_bytes"], st["lane_bytes"], m.nopac)
+    # each kind of kernel's roofline: algorithmic bytes over its summed
+    # HIP-event time (every launch of the step bracketed on the run stream);
+    # the one that takes longer is the step's "roofline"
+    def roof(kname, kms, kB, nl):
+        achieved = kB / (kms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname), "kernel": kname,
+                "kernel_ms_per_step": kms, "launches_per_step": nl, "kernel_ms_per_launch": kms / max(nl, 1.0),
+                "algorithmic_bytes_per_launch": kB / max(nl, 1.0)}
+    r_tr = roof(TRANSPORT_KERNELS, st["transport_ms"], b_tr, st["rounds"])
+    r_pe = roof(PEEL_KERNELS, st["peel_ms"], b_pe, st["peel_launches"])
+    dom, other = (r_pe, r_tr) if st["peel_ms"] >= st["transport_ms"] else (r_tr, r_pe)
     log(f"kernels per step: transport {st['transport_ms']:.1f} ms ({b_tr/1e9:.2f} GB algorithmic), "
         f"peel {st['peel_ms']:.1f} ms ({b_pe/1e9:.2f} GB), rounds {st['rounds']:.0f}")
     out = {
@@ -457,13 +470,9 @@ def run_single_workload(args, T0):
                    "budget_s": args.budget, "wall_s": round(elapsed(), 1),
                    "parallelism": "1 GPU", **({"accfrac": args.accfrac} if args.accfrac > 0.0 else {}),
                    **({"diffusion": True} if args.diffusion else {})},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
-                     "kernel": kname, "kernel_ms_per_step": kms, "launches_per_step": nl,
-                     "kernel_ms_per_launch": kms / max(nl, 1.0),
-                     "algorithmic_bytes_per_launch": kB / max(nl, 1.0),
-                     "transport_ms_per_step": st["transport_ms"], "peel_ms_per_step": st["peel_ms"],
-                     "run_ms_per_step": kmean},
+        "roofline": {**dom, "transport_ms_per_step": st["transport_ms"], "peel_ms_per_step": st["peel_ms"],
+                     "run_ms_per_step": kmean, "tail_rounds": st.get("tail_rounds", 0.0)},
+        "roofline_other_kernels": other,
         "cpu_baseline": cpu,
     }
     out.update(extra)
