@@ -436,6 +436,20 @@ __device__ __forceinline__ V pick(const V (&a)[kMaxOpac], int i) {
   }
   return v;
 }
+// The same, field by field: a select between two structures is a copy from
+// a selected address, which would put both into private memory.
+template <class T>
+__device__ __forceinline__ Opac<T> pick(const Opac<T> (&a)[kMaxOpac], int i) {
+  T kap[kMaxOpac], alb[kMaxOpac], g[kMaxOpac], pl[kMaxOpac];
+#pragma unroll
+  for (int k = 0; k < kMaxOpac; ++k) { kap[k] = a[k].kappa; alb[k] = a[k].albedo; g[k] = a[k].g; pl[k] = a[k].pl; }
+  Opac<T> v;
+  v.kappa = pick(kap, i);
+  v.albedo = pick(alb, i);
+  v.g = pick(g, i);
+  v.pl = pick(pl, i);
+  return v;
+}
 
 // Event counters of a workgroup, held in LDS (no registers per lane) and
 // flushed to the device counters once per launch.
@@ -1026,15 +1040,16 @@ __device__ __forceinline__ bool next_face(const DeviceModel& m, const Pkt<T>& p,
   return true;
 }
 
+// iface 1/2: ir -/+ 1, 3/4: it -/+ 1, 5/6: ip -/+ 1 (phi wraps).  Written as
+// three additions rather than a switch: the optimizer turns a switch over
+// three references into one store through a selected pointer, which keeps
+// the caller's cell indices in private memory (a scratch load on every
+// use along the packet's serial chain).
 __device__ __forceinline__ void cross_face(const DeviceModel& m, int& ir, int& it, int& ip, int iface) {
-  switch (iface) {
-    case 1: ir--; break;
-    case 2: ir++; break;
-    case 3: it--; break;
-    case 4: it++; break;
-    case 5: ip--; break;
-    default: ip++; break;
-  }
+  const int s = (iface & 1) ? -1 : 1;
+  ir += (iface <= 2) ? s : 0;
+  it += (iface == 3 || iface == 4) ? s : 0;
+  ip += (iface >= 5) ? s : 0;
   if (ip == 0) ip = m.npg - 1;
   if (ip == m.npg) ip = 1;
 }
