@@ -156,17 +156,27 @@ __device__ __forceinline__ bool diffusing(const DeviceModel& m, int ir, int it) 
 // ---------------------------------------------------------------------------
 // Counter-based RNG: Philox4x32-10, key = run seed, counter = (n/4, packet).
 // ---------------------------------------------------------------------------
+// coop: every lane of the wavefront carries the same stream (k_tail).  The
+// lanes then hold 64 consecutive blocks, lane l block cb + l, computed in
+// one pass, and a draw reads its word from the lane that holds it: one
+// Philox evaluation per 256 draws on the packet's serial chain instead of
+// one per 4.  The words are those of the serial stream.
 struct Rng {
   uint32_t k0, k1, p0, p1, n;
   uint32_t b0, b1, b2, b3;      // current Philox block (scalars: no indexed private memory)
+  bool coop = false;
+  uint32_t cb = 0x80000000u;    // coop: block held by lane 0 (0x80000000: none yet)
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;   // coop: this lane's block
 };
 
 __device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                                        uint32_t k0, uint32_t k1, uint32_t out[4]) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-    uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    // one 32x32 -> 64-bit product gives both halves (v_mad_u64_u32)
+    const uint64_t m0 = (uint64_t)0xD2511F53u * c0, m1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(m0 >> 32), lo0 = (uint32_t)m0;
+    uint32_t hi1 = (uint32_t)(m1 >> 32), lo1 = (uint32_t)m1;
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -179,6 +189,7 @@ __device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet)
   r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32);
   r.p0 = (uint32_t)packet; r.p1 = (uint32_t)(packet >> 32);
   r.n = 0;
+  r.cb = 0x80000000u;
 }
 
 // Move a stream (keys kept) to draw n: the block holding draw n is
@@ -186,7 +197,7 @@ __device__ __forceinline__ void rng_init(Rng& r, uint64_t seed, uint64_t packet)
 // otherwise.
 __device__ __forceinline__ void rng_seek(Rng& r, uint32_t n) {
   r.n = n;
-  if (n & 3u) {
+  if (!r.coop && (n & 3u)) {
     uint32_t o[4];
     philox(n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
     r.b0 = o[0]; r.b1 = o[1]; r.b2 = o[2]; r.b3 = o[3];
@@ -201,6 +212,23 @@ __device__ __forceinline__ void rng_resume(Rng& r, uint64_t seed, uint64_t packe
 
 __device__ __forceinline__ uint32_t rng_word(Rng& r) {
   const uint32_t j = r.n & 3u;
+  if (r.coop) {
+    const uint32_t blk = r.n >> 2;
+    if (blk - r.cb >= 64u) {
+      uint32_t o[4];
+      philox(blk + (uint32_t)__lane_id(), r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
+      r.cb = blk;
+      r.w0 = o[0]; r.w1 = o[1]; r.w2 = o[2]; r.w3 = o[3];
+    }
+    // (the words pass an empty asm, so the select stays one between values:
+    // a select between fields is a load from a selected address, which
+    // would put the whole state into private memory)
+    uint32_t a0 = r.w0, a1 = r.w1, a2 = r.w2, a3 = r.w3;
+    asm volatile("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3));
+    const uint32_t v = (j == 0u) ? a0 : (j == 1u) ? a1 : (j == 2u) ? a2 : a3;
+    r.n++;
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)(blk - r.cb));
+  }
   if (j == 0u) {
     uint32_t o[4];
     philox(r.n >> 2, r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
@@ -520,52 +548,82 @@ __device__ __forceinline__ void set_kapd(Pkt<T>& p, const DeviceModel& m, int or
   }
 }
 
-// dust_5.f90:9-109 (dust species, no gas mixing) for opacity table id
+// (1 - f) a + f b, as the reference interpolates its tables.  The FP32
+// path fuses the second product explicitly, so that each search, serial
+// or over a wavefront, forms the same value at the same index.
+template <class T> __device__ __forceinline__ T lerp2(T fm1, T f, T a, T b) { return fm1 * a + f * b; }
+template <> __device__ __forceinline__ float lerp2<float>(float fm1, float f, float a, float b) {
+  return __builtin_fmaf(f, b, fm1 * a);
+}
+
+// dust_5.f90:9-109 (dust species, no gas mixing) for opacity table id:
+// the table entries at index i (1-based), or interpolated inside the
+// bracket nd(i0) >= nu > nd(i1 = i0 + 1) of the decreasing frequency table
 template <class T>
-__device__ __forceinline__ Opac<T> opac_one(const DeviceModel& m, T nu, int id) {
+__device__ __forceinline__ Opac<T> opac_entry(const DeviceModel& m, int id, int i) {
+  const size_t o0 = (size_t)id * m.nlam + (size_t)(i - 1);
+  Opac<T> o;
+  o.g = (T)m.gdust[o0]; o.pl = (T)m.pldust[o0]; o.kappa = (T)m.kappad[o0]; o.albedo = (T)m.adust[o0];
+  return o;
+}
+template <class T>
+__device__ __forceinline__ Opac<T> opac_interp(const DeviceModel& m, T nu, int id, int i0, int i1) {
   const double* nd = m.nudust + (size_t)id * m.nlam;
   const double *kd = m.kappad + (size_t)id * m.nlam, *ad = m.adust + (size_t)id * m.nlam;
   const double *gd = m.gdust + (size_t)id * m.nlam, *pd = m.pldust + (size_t)id * m.nlam;
-  const int nl = m.nlam;
+  T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
+  T fm1 = (T)1 - f;
   Opac<T> o;
-  if (nu >= (T)nd[0]) { o.g = (T)gd[0]; o.pl = (T)pd[0]; o.kappa = (T)kd[0]; o.albedo = (T)ad[0]; }
-  else if (nu <= (T)nd[nl - 1]) { o.g = (T)gd[nl - 1]; o.pl = (T)pd[nl - 1]; o.kappa = (T)kd[nl - 1]; o.albedo = (T)ad[nl - 1]; }
-  else {
-    // The bisection of dust_5.f90:40-52 ends on the unique bracket
-    // nd(i0) >= nu > nd(i0+1) of the strictly decreasing table.  On a
-    // log-uniform table that index is guessed from log(nu) and stepped to
-    // the bracket, which gives the same i0 in one or two probes.
-    int i0 = 1, i1 = nl, i;
-    if (m.nud_ilog[id] > 0.0) {
-      int g = 1 + (int)(((double)m.nud_log0[id] - (double)RT<T>::log_(nu)) * m.nud_ilog[id]);
-      g = g < 1 ? 1 : (g > nl - 1 ? nl - 1 : g);
-      while (g > 1 && !(nu <= (T)nd[g - 1])) g--;
-      while (g < nl - 1 && nu <= (T)nd[g]) g++;
-      i0 = g; i1 = g + 1;
-    } else {
-      while (i1 - i0 > 1) {
-        i = (i1 + i0) / 2;
-        if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
-      }
-    }
-    T f = (nu - (T)nd[i1 - 1]) / ((T)nd[i0 - 1] - (T)nd[i1 - 1]);
-    T fm1 = (T)1 - f;
-    o.g = fm1 * (T)gd[i1 - 1] + f * (T)gd[i0 - 1];
-    o.pl = fm1 * (T)pd[i1 - 1] + f * (T)pd[i0 - 1];
-    o.kappa = fm1 * (T)kd[i1 - 1] + f * (T)kd[i0 - 1];
-    o.albedo = fm1 * (T)ad[i1 - 1] + f * (T)ad[i0 - 1];
-  }
+  o.g = lerp2<T>(fm1, f, (T)gd[i1 - 1], (T)gd[i0 - 1]);
+  o.pl = lerp2<T>(fm1, f, (T)pd[i1 - 1], (T)pd[i0 - 1]);
+  o.kappa = lerp2<T>(fm1, f, (T)kd[i1 - 1], (T)kd[i0 - 1]);
+  o.albedo = lerp2<T>(fm1, f, (T)ad[i1 - 1], (T)ad[i0 - 1]);
   return o;
+}
+
+template <class T>
+__device__ __forceinline__ Opac<T> opac_one(const DeviceModel& m, T nu, int id, bool coop = false) {
+  const double* nd = m.nudust + (size_t)id * m.nlam;
+  const int nl = m.nlam;
+  if (nu >= (T)nd[0]) return opac_entry<T>(m, id, 1);
+  if (nu <= (T)nd[nl - 1]) return opac_entry<T>(m, id, nl);
+  // The bisection of dust_5.f90:40-52 ends on the unique bracket
+  // nd(i0) >= nu > nd(i0+1) of the decreasing table: i0 + 1 is the first
+  // index with nd < nu.  On a log-uniform table that index is guessed from
+  // log(nu) and stepped to the bracket, which gives the same i0 in one or
+  // two probes.  With every lane of the wavefront carrying the packet
+  // (coop), lane l tests indices 1 + l, 65 + l, ... and a ballot names it.
+  int i0 = 1, i1 = nl, i;
+  if (coop) {
+    for (int base = 1; base <= nl; base += 64) {
+      const int ix = base + __lane_id();
+      const unsigned long long hit = __ballot(ix <= nl && nu > (T)nd[ix <= nl ? ix - 1 : 0]);
+      if (hit) { i1 = base + __ffsll((long long)hit) - 1; break; }
+    }
+    i0 = i1 - 1;
+  } else if (m.nud_ilog[id] > 0.0) {
+    int g = 1 + (int)(((double)m.nud_log0[id] - (double)RT<T>::log_(nu)) * m.nud_ilog[id]);
+    g = g < 1 ? 1 : (g > nl - 1 ? nl - 1 : g);
+    while (g > 1 && !(nu <= (T)nd[g - 1])) g--;
+    while (g < nl - 1 && nu <= (T)nd[g]) g++;
+    i0 = g; i1 = g + 1;
+  } else {
+    while (i1 - i0 > 1) {
+      i = (i1 + i0) / 2;
+      if (nu <= (T)nd[i - 1]) i0 = i; else i1 = i;
+    }
+  }
+  return opac_interp<T>(m, nu, id, i0, i1);
 }
 
 // every opacity table at frequency nu (constant indices: o stays in registers)
 template <class T>
-__device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T> (&o)[kMaxOpac]) {
+__device__ __forceinline__ void opacset(const DeviceModel& m, T nu, Opac<T> (&o)[kMaxOpac], bool coop = false) {
   static_assert(kMaxOpac == 4, "opacset unrolls four tables");
-  o[0] = opac_one<T>(m, nu, 0);
-  if (m.nopac > 1) o[1] = opac_one<T>(m, nu, 1);
-  if (m.nopac > 2) o[2] = opac_one<T>(m, nu, 2);
-  if (m.nopac > 3) o[3] = opac_one<T>(m, nu, 3);
+  o[0] = opac_one<T>(m, nu, 0, coop);
+  if (m.nopac > 1) o[1] = opac_one<T>(m, nu, 1, coop);
+  if (m.nopac > 2) o[2] = opac_one<T>(m, nu, 2, coop);
+  if (m.nopac > 3) o[3] = opac_one<T>(m, nu, 3, coop);
 }
 
 // random.f90:44-92 bbfreq: the zeta(4) partial sums are computed once on the
@@ -584,28 +642,59 @@ template <class T> struct KapRows { __device__ static const double* get(const De
 template <> struct KapRows<float> { __device__ static const float* get(const DeviceModel& m) { return m.kapint32; } };
 template <class T> __device__ __forceinline__ auto kap_rows(const DeviceModel& m) { return KapRows<T>::get(m); }
 
+// lucydustfreq_2_test.f90:34-45: the temperature row j of the CDF tables
+// and the interpolation fraction f toward row j + 1
+template <class T>
+__device__ __forceinline__ void lucy_row(const DeviceModel& m, T T0, int& j, T& f) {
+  const int nT = m.nT;
+  const double* lT = m.lTint;
+  T lT0 = RT<T>::log_(T0);
+  j = (int)((T)(nT - 1) * (lT0 - (T)lT[0]) / ((T)lT[nT - 1] - (T)lT[0])) + 1;
+  if (j < 1) j = 1;
+  if (j > nT - 1) j = nT - 1;
+  f = (lT0 - (T)lT[j - 1]) / ((T)lT[j] - (T)lT[j - 1]);
+  if (f < (T)0) f = (T)0;
+  if (f > (T)1) f = (T)1;
+}
+
+// lucydustfreq_2_test.f90:140-160: the frequency inside the bracket
+// P(i0) <= xi < P(i1); fT is the temperature interpolation fraction
+template <class T>
+__device__ __forceinline__ T lucy_finish(const double* nui, T T0, T xi, int i0, int i1, T P0, T P1, T fT) {
+  T f = fT;                         // (kept where the reference leaves it)
+  if (xi < P0) f = (T)0;
+  else if (xi > P1) f = (T)1;
+  else if (P0 <= (T)0) f = (T)0.5;
+  else if (P1 <= (T)0) { }
+  else if (xi <= (T)0) { }
+  else {
+    // lucydustfreq_2_test.f90:150.  P1/P0 can round to 1 when P0 and P1 are
+    // adjacent floating-point values (the top of a CDF row): f = 0 then,
+    // where the reference formula would divide 0 by 0.
+    const T den = RT<T>::log_acc(P1 / P0);
+    f = den > (T)0 ? RT<T>::log_acc(xi / P0) / den : (T)0;
+  }
+  return T0 * (T)nui[i0 - 1] * RT<T>::pow_((T)nui[i1 - 1] / (T)nui[i0 - 1], f);
+}
+
 // lucydustfreq_2_test.f90:19-160 with xi supplied
 template <class T>
 __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop, T xi) {
   const int nT = m.nT, nnu = m.nnu;
-  const double* lT = m.lTint;
   const double* nui = m.nuint;
   const auto* K = kap_rows<T>(m) + (size_t)iop * nT * nnu;
 #define KAP(i, j) ((T)K[(size_t)((i) - 1) * nnu + ((j) - 1)])
-  T lT0 = RT<T>::log_(T0);
-  int j = (int)((T)(nT - 1) * (lT0 - (T)lT[0]) / ((T)lT[nT - 1] - (T)lT[0])) + 1;
-  if (j < 1) j = 1;
-  if (j > nT - 1) j = nT - 1;
-  T f = (lT0 - (T)lT[j - 1]) / ((T)lT[j] - (T)lT[j - 1]);
-  if (f < (T)0) f = (T)0;
-  if (f > (T)1) f = (T)1;
+#define CDFP(ix) lerp2<T>((T)1 - f, f, KAP(j, ix), KAP(j + 1, ix))
+  int j;
+  T f;
+  lucy_row<T>(m, T0, j, f);
   int i0 = 1, i1 = nnu, i;
-  T P0 = ((T)1 - f) * KAP(j, i0) + f * KAP(j + 1, i0), P1, P = (T)0;
+  T P0 = CDFP(i0), P1, P = (T)0;
   if (xi < P0) {
     if (P0 > (T)0) return T0 * xi * (T)nui[0] / P0;
     return T0 * (T)nui[0];
   }
-  P1 = ((T)1 - f) * KAP(j, i1) + f * KAP(j + 1, i1);
+  P1 = CDFP(i1);
   if (P1 > (T)1) P1 = (T)1;
   if (xi >= P1) {
     if (P1 < (T)1) {
@@ -649,7 +738,7 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
       for (int q = 0; q < 9; ++q) {
         int ix = lo + q * step;
         ix = ix < hi ? ix : hi;
-        T v = ((T)1 - f) * KAP(j, ix) + f * KAP(j + 1, ix);
+        T v = CDFP(ix);
         Pq[q] = (ix > 1 && v > (T)1) ? (T)1 : v;
       }
       if (!(Pq[0] > xi) && Pq[8] > xi) {
@@ -676,7 +765,7 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
     for (int k = 0; k < 7; ++k) {
       const int q = i0 + (k + 1) * step;
       const int qc = q < i1 ? q : i1;
-      T v = ((T)1 - f) * KAP(j, qc) + f * KAP(j + 1, qc);
+      T v = CDFP(qc);
       Pq[k] = v > (T)1 ? (T)1 : v;
     }
     int n0 = i0, n1 = i1;
@@ -696,20 +785,60 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
     i0 = n0; i1 = n1; P0 = p0; P1 = p1;
   }
   (void)P;
-  if (xi < P0) f = (T)0;
-  else if (xi > P1) f = (T)1;
-  else if (P0 <= (T)0) f = (T)0.5;
-  else if (P1 <= (T)0) { }
-  else if (xi <= (T)0) { }
-  else {
-    // lucydustfreq_2_test.f90:150.  P1/P0 can round to 1 when P0 and P1 are
-    // adjacent floating-point values (the top of a CDF row): f = 0 then,
-    // where the reference formula would divide 0 by 0.
-    const T den = RT<T>::log_acc(P1 / P0);
-    f = den > (T)0 ? RT<T>::log_acc(xi / P0) / den : (T)0;
-  }
-  return T0 * (T)nui[i0 - 1] * RT<T>::pow_((T)nui[i1 - 1] / (T)nui[i0 - 1], f);
+  return lucy_finish<T>(nui, T0, xi, i0, i1, P0, P1, f);
 #undef KAP
+#undef CDFP
+}
+
+// The same draw with every lane of the wavefront carrying the packet
+// (k_tail): lane l evaluates P at indices 2 + l, 66 + l, ... of the row and
+// one ballot names the first index with P > xi.  With P non-decreasing
+// (the search above relies on it) that index is the serial search's i1, so
+// the bracket, P0, P1 and the result are the same; the row is read in one
+// pass of coalesced loads instead of a chain of dependent probes.
+template <class T>
+__device__ __forceinline__ T lucydustfreq_coop(const DeviceModel& m, T T0, int iop, T xi) {
+  const int nT = m.nT, nnu = m.nnu;
+  const double* nui = m.nuint;
+  const auto* K = kap_rows<T>(m) + (size_t)iop * nT * nnu;
+  int j;
+  T f;
+  lucy_row<T>(m, T0, j, f);
+  const T fm1 = (T)1 - f;
+  const auto* Ka = K + (size_t)(j - 1) * nnu;
+  const auto* Kb = Ka + nnu;
+  const int lane = __lane_id();
+  // P(1) (never clamped), P(nnu) and the first 64 indices in one pass
+  const T Pfirst = lerp2<T>(fm1, f, (T)Ka[0], (T)Kb[0]);
+  T Plast = lerp2<T>(fm1, f, (T)Ka[nnu - 1], (T)Kb[nnu - 1]);
+  if (Plast > (T)1) Plast = (T)1;
+  int ix = 2 + lane;
+  T v = (ix <= nnu) ? lerp2<T>(fm1, f, (T)Ka[ix - 1], (T)Kb[ix - 1]) : (T)1;
+  if (xi < Pfirst) {
+    if (Pfirst > (T)0) return T0 * xi * (T)nui[0] / Pfirst;
+    return T0 * (T)nui[0];
+  }
+  if (xi >= Plast) {
+    if (Plast < (T)1) {
+      if (xi < (T)1) return T0 * (T)nui[nnu - 1] * ((T)1 - Plast) / ((T)1 - xi);
+      return T0 * (T)nui[nnu - 1];
+    }
+    return T0 * (T)nui[nnu - 1];
+  }
+  T prev = Pfirst;                  // P at the index before this pass's first
+  for (int base = 2;; base += 64) {
+    if (v > (T)1) v = (T)1;
+    const unsigned long long hit = __ballot(ix <= nnu && v > xi);
+    if (hit) {                      // reached within the row: P(nnu) > xi
+      const int t = __ffsll((long long)hit) - 1;
+      const T P1 = __shfl(v, t, 64);
+      const T Pt = __shfl(v, t > 0 ? t - 1 : 0, 64);
+      return lucy_finish<T>(nui, T0, xi, base + t - 1, base + t, t > 0 ? Pt : prev, P1, f);
+    }
+    prev = __shfl(v, 63, 64);
+    ix = base + 64 + lane;
+    v = (ix <= nnu) ? lerp2<T>(fm1, f, (T)Ka[ix - 1], (T)Kb[ix - 1]) : (T)1;
+  }
 }
 
 // dustmat.f90:42-67

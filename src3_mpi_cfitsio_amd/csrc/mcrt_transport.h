@@ -176,12 +176,18 @@ __device__ __forceinline__ unsigned reserve_slots(unsigned* counter, unsigned pe
   return base + rank * per;
 }
 
+// Slots per reservation: kSlotChunk per lane, 64 for a wavefront's one
+// packet (coop), whose reservation is an atomic round trip on its serial
+// chain.
+template <class T>
+__device__ __forceinline__ unsigned slot_chunk(const Ctx<T>& C) { return C.coop ? 64u : kSlotChunk; }
+
 // the next queue slot of this lane (>= capacity: the queue is full)
 template <class T>
 __device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
   if (C.slot_next == C.slot_end) {
-    C.slot_next = reserve_slots(C.q.count, kSlotChunk, C.coop);
-    C.slot_end = C.slot_next + kSlotChunk;
+    C.slot_next = reserve_slots(C.q.count, slot_chunk(C), C.coop);
+    C.slot_end = C.slot_next + slot_chunk(C);
   }
   return C.slot_next++;
 }
@@ -192,10 +198,10 @@ __device__ __forceinline__ unsigned take_slot(Ctx<T>& C) {
 template <class T>
 __device__ __forceinline__ bool hold_slot(Ctx<T>& C) {
   if (C.q.capacity == 0 || C.slot_next < C.slot_end) return true;
-  const unsigned base = reserve_slots(C.q.count, kSlotChunk, C.coop);
+  const unsigned base = reserve_slots(C.q.count, slot_chunk(C), C.coop);
   if (base >= C.q.capacity) { C.slot_next = C.slot_end = 0u; return false; }
   C.slot_next = base;
-  C.slot_end = (base + kSlotChunk < C.q.capacity) ? base + kSlotChunk : C.q.capacity;
+  C.slot_end = (base + slot_chunk(C) < C.q.capacity) ? base + slot_chunk(C) : C.q.capacity;
   return true;
 }
 
@@ -468,7 +474,7 @@ __device__ __forceinline__ int walk_step(Ctx<T>& C, Life<T>& L) {
     C.cnt.add(C_AFLUX);
     p.rtot = sqrt(p.rsq);
     initpabs(C, p);
-    opacset<T>(m, p.nub, p.op);
+    opacset<T>(m, p.nub, p.op, C.coop);
     p.wave = (T)1.2398 / p.nub;
     p.i_orig = 1;
     p.iflag = 0;
@@ -1058,6 +1064,7 @@ __device__ __forceinline__ void stokes_coop(Ctx<T>& C, T g, T pl, const Pkt<T>& 
   const uint32_t n0 = C.rng.n;
   for (uint32_t done = 0;; done += 64u) {
     Rng r = C.rng;
+    r.coop = false;                 // each lane its own trial's draws
     rng_seek(r, n0 + 3u * (done + (uint32_t)lane));
     const T u1 = uniform<T>(r);
     const T u2 = uniform<T>(r);
@@ -1168,7 +1175,10 @@ __device__ __forceinline__ void reemit(Ctx<T>& C, Pkt<T>& p, T tave, int iop) {
   p.cosp = RT<T>::cos_(p.phi);
   p.sinp = RT<T>::sin_(p.phi);
   if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
-  else { T xi = uniform<T>(C.rng); p.nub = lucydustfreq_xi<T>(m, tave, iop, xi); }
+  else {
+    T xi = uniform<T>(C.rng);
+    p.nub = C.coop ? lucydustfreq_coop<T>(m, tave, iop, xi) : lucydustfreq_xi<T>(m, tave, iop, xi);
+  }
   set_dir<T>(p);
 }
 
@@ -1273,7 +1283,7 @@ __device__ __forceinline__ void interact(Ctx<T>& C, Life<T>& L) {
     if (diff) diffuse<T>(C, p);
     else reemit<T>(C, p, cell_tdust<T>(m, L, c), iop);
     p.iflag = 0;
-    opacset<T>(m, p.nub, p.op);
+    opacset<T>(m, p.nub, p.op, C.coop);
     p.wave = (T)1.2398 / p.nub;
     set_kapd(p, m, start ? 0 : 1);
     if (m.ipeel == 1 && (kAdj || !peel_hidden<T>(m, L))) peel_event<T, kAdj>(C, p, L.idust2, iop);
@@ -1637,7 +1647,7 @@ __device__ __forceinline__ void unpark(LaneIO& io, Ctx<T>& C, Life<T>& L, uint64
   L.pid = ((uint64_t)hi << 32) | lo;
   uint32_t n = io.get();
   rng_resume(C.rng, seed, first_packet + L.pid, n);
-  opacset<T>(C.m, p.nub, p.op);
+  opacset<T>(C.m, p.nub, p.op, C.coop);
   L.first = false;
   cell_cache_reset(L);
 }
@@ -1832,6 +1842,8 @@ __global__ void __launch_bounds__(256) k_tail(DeviceModel m0, DeviceTallies t, R
     Life<T> L;
     tally_open(L);
     unpark<T>(io, C, L, rp.seed, rp.first_packet);
+    C.rng.coop = true;
+    C.rng.cb = 0x80000000u;
     int st = L_READY;
     uint32_t budget = rp.events;
     for (;;) {
