@@ -918,6 +918,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     return fail(c, MCRT_E_ARG, "mcrt_run_packets: walls, Planck grids and observer directions exceed the 56 KB LDS budget "
                                "(2*nrg + 4*ntg + 3*npg + nT + nnu + 5*npeel must stay below 7023)");
   const LdsPlan lds = plan_lds(c->dm);
+  dm.dust_lds = lds.n[LT_NUDUST] > 0 ? 1 : 0;
   int64_t lanes = c->lanes;
   if (lanes <= 0) {
     int bpc = fp64 ? transport_blocks_per_cu_fp64(dm, lds.bytes) : transport_blocks_per_cu_mixed(dm, lds.bytes);

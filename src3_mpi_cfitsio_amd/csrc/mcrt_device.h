@@ -118,6 +118,9 @@ struct DeviceModel {
   // surface temperature increment per photon times the surface area.
   const int32_t* diff_itmin;
   double diff_delta;
+  // set per run by the host: the run's LDS plan stages the five dust tables
+  // (the kernels for plain models then address them in LDS only)
+  int dust_lds;
 };
 
 // where k_acc_place put accretion photon k: position, cell, species and the
@@ -791,29 +794,55 @@ __device__ __forceinline__ T lucydustfreq_xi(const DeviceModel& m, T T0, int iop
 }
 
 // The same draw with every lane of the wavefront carrying the packet
-// (k_tail): lane l evaluates P at indices 2 + l, 66 + l, ... of the row and
-// one ballot names the first index with P > xi.  With P non-decreasing
-// (the search above relies on it) that index is the serial search's i1, so
-// the bracket, P0, P1 and the result are the same; the row is read in one
-// pass of coalesced loads instead of a chain of dependent probes.
+// (k_tail): lane l holds P at indices 2 + l, 66 + l, ... of the row and one
+// ballot names the first index with P > xi.  With P non-decreasing (the
+// search above relies on it) that index is the serial search's i1, so the
+// bracket, P0, P1 and the result are the same.  The row (interpolated at
+// the temperature, clamped as the serial search clamps it) is formed once
+// per (temperature, table) in the wavefront's LDS row cache: a packet
+// trapped in a thick cell re-emits there many times at one temperature,
+// and the table rows would otherwise come from memory on every draw.
+constexpr int kCdfCache = 512;      // cached P entries (indices 2..513)
 template <class T>
-__device__ __forceinline__ T lucydustfreq_coop(const DeviceModel& m, T T0, int iop, T xi) {
+struct CdfCache {
+  T* row = nullptr;                 // this wavefront's LDS row, kCdfCache entries
+  T T0 = (T)-1;                     // temperature and table of the cached row
+  int iop = -1;
+  T fT = (T)0, first = (T)0, last = (T)0;   // lucy_row's f, P(1) (unclamped), P(nnu)
+};
+
+template <class T>
+__device__ __forceinline__ T lucydustfreq_coop(const DeviceModel& m, T T0, int iop, T xi, CdfCache<T>& cc) {
   const int nT = m.nT, nnu = m.nnu;
   const double* nui = m.nuint;
-  const auto* K = kap_rows<T>(m) + (size_t)iop * nT * nnu;
-  int j;
-  T f;
-  lucy_row<T>(m, T0, j, f);
-  const T fm1 = (T)1 - f;
-  const auto* Ka = K + (size_t)(j - 1) * nnu;
-  const auto* Kb = Ka + nnu;
   const int lane = __lane_id();
-  // P(1) (never clamped), P(nnu) and the first 64 indices in one pass
-  const T Pfirst = lerp2<T>(fm1, f, (T)Ka[0], (T)Kb[0]);
-  T Plast = lerp2<T>(fm1, f, (T)Ka[nnu - 1], (T)Kb[nnu - 1]);
-  if (Plast > (T)1) Plast = (T)1;
-  int ix = 2 + lane;
-  T v = (ix <= nnu) ? lerp2<T>(fm1, f, (T)Ka[ix - 1], (T)Kb[ix - 1]) : (T)1;
+  const bool cached = cc.row != nullptr && nnu - 1 <= kCdfCache;
+  const auto* Ka = kap_rows<T>(m) + (size_t)iop * nT * nnu;
+  const auto* Kb = Ka;
+  T f, fm1 = (T)0, Pfirst, Plast;
+  if (cached && cc.T0 == T0 && cc.iop == iop) {
+    f = cc.fT; Pfirst = cc.first; Plast = cc.last;
+  } else {
+    int j;
+    lucy_row<T>(m, T0, j, f);
+    fm1 = (T)1 - f;
+    Ka += (size_t)(j - 1) * nnu;
+    Kb = Ka + nnu;
+    Pfirst = lerp2<T>(fm1, f, (T)Ka[0], (T)Kb[0]);
+    Plast = lerp2<T>(fm1, f, (T)Ka[nnu - 1], (T)Kb[nnu - 1]);
+    if (Plast > (T)1) Plast = (T)1;
+    if (cached) {
+      for (int k = 2 + lane; k <= nnu; k += 64) {
+        const T w = lerp2<T>(fm1, f, (T)Ka[k - 1], (T)Kb[k - 1]);
+        cc.row[k - 2] = w > (T)1 ? (T)1 : w;
+      }
+      // the row is read below by other lanes of this wavefront
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      cc.T0 = T0; cc.iop = iop; cc.fT = f; cc.first = Pfirst; cc.last = Plast;
+    }
+  }
   if (xi < Pfirst) {
     if (Pfirst > (T)0) return T0 * xi * (T)nui[0] / Pfirst;
     return T0 * (T)nui[0];
@@ -827,7 +856,15 @@ __device__ __forceinline__ T lucydustfreq_coop(const DeviceModel& m, T T0, int i
   }
   T prev = Pfirst;                  // P at the index before this pass's first
   for (int base = 2;; base += 64) {
-    if (v > (T)1) v = (T)1;
+    const int ix = base + lane;
+    T v = (T)1;
+    if (ix <= nnu) {
+      if (cached) v = cc.row[ix - 2];
+      else {
+        v = lerp2<T>(fm1, f, (T)Ka[ix - 1], (T)Kb[ix - 1]);
+        if (v > (T)1) v = (T)1;
+      }
+    }
     const unsigned long long hit = __ballot(ix <= nnu && v > xi);
     if (hit) {                      // reached within the row: P(nnu) > xi
       const int t = __ffsll((long long)hit) - 1;
@@ -836,8 +873,6 @@ __device__ __forceinline__ T lucydustfreq_coop(const DeviceModel& m, T T0, int i
       return lucy_finish<T>(nui, T0, xi, base + t - 1, base + t, t > 0 ? Pt : prev, P1, f);
     }
     prev = __shfl(v, 63, 64);
-    ix = base + 64 + lane;
-    v = (ix <= nnu) ? lerp2<T>(fm1, f, (T)Ka[ix - 1], (T)Kb[ix - 1]) : (T)1;
   }
 }
 

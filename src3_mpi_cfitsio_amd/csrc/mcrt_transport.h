@@ -77,6 +77,10 @@ struct RunParams {
 
 // Copy the planned tables into this workgroup's LDS and point the kernel's
 // model copy at them.  Every thread of the workgroup must call it.
+// kDustLds: the plan is known to stage the dust tables, so their pointers
+// are LDS addresses at compile time (ds_read, not flat loads that also wait
+// on the wavefront's outstanding global stores).
+template <bool kDustLds = false>
 __device__ __forceinline__ void stage_tables(DeviceModel& m, const double*& inczeta, const LdsPlan& P, double* lds) {
   const double* src[LT_COUNT] = {m.rarr, m.r2arr, m.costarr, m.tan2arr, m.ccone, m.scone, m.phiarr, m.aarr, m.barr,
                                  m.lTint, m.nuint, m.peel, inczeta,
@@ -98,7 +102,7 @@ __device__ __forceinline__ void stage_tables(DeviceModel& m, const double*& incz
   m.nuint = lds + P.off[LT_NUINT];
   m.peel = lds + P.off[LT_PEEL];
   inczeta = lds + P.off[LT_INCZETA];
-  if (P.n[LT_NUDUST]) {
+  if (kDustLds || P.n[LT_NUDUST]) {
     m.nudust = lds + P.off[LT_NUDUST];
     m.kappad = lds + P.off[LT_KAPPAD];
     m.adust = lds + P.off[LT_ADUST];
@@ -133,6 +137,7 @@ struct Ctx {
   // writes, and the rejection sampling of the scattering angle tries 64
   // candidates at once
   bool coop = false;
+  CdfCache<T> cdf;                // k_tail: the re-emission row of the last draw (LDS)
 };
 
 // the lane that writes the packet's tallies and records
@@ -1177,7 +1182,7 @@ __device__ __forceinline__ void reemit(Ctx<T>& C, Pkt<T>& p, T tave, int iop) {
   if (tave >= (T)(kTmaxK / 11605.0)) p.nub = bbfreq<T>(C.rng, tave, C.inczeta);
   else {
     T xi = uniform<T>(C.rng);
-    p.nub = C.coop ? lucydustfreq_coop<T>(m, tave, iop, xi) : lucydustfreq_xi<T>(m, tave, iop, xi);
+    p.nub = C.coop ? lucydustfreq_coop<T>(m, tave, iop, xi, C.cdf) : lucydustfreq_xi<T>(m, tave, iop, xi);
   }
   set_dir<T>(p);
 }
@@ -1700,13 +1705,14 @@ template <class T>
 constexpr bool kCombTally = std::is_same<T, float>::value;
 
 // kPlain: the model has one opacity table and no accretion photons, ISRF,
-// hot spots or diffusion region (plain_model()).  Those fields of the
+// hot spots or diffusion region, and the run stages its dust tables in LDS
+// (plain_model()).  Those fields of the
 // kernel's model copy are then constants, so the compiler drops their code:
 // k_transport<float> keeps ~15k instead of ~24k instructions and spills a
 // third of the scalar registers (each spilled value costs a lane read where
 // it is used).
 __host__ __device__ inline bool plain_model(const DeviceModel& m) {
-  return m.acc_to == 0 && m.nisrf == 0 && m.ispot == 0 && m.diff_itmin == nullptr && m.nopac == 1;
+  return m.acc_to == 0 && m.nisrf == 0 && m.ispot == 0 && m.diff_itmin == nullptr && m.nopac == 1 && m.dust_lds;
 }
 template <bool kPlain>
 __device__ __forceinline__ void specialize(DeviceModel& m) {
@@ -1724,7 +1730,7 @@ __global__ void __launch_bounds__(256) k_transport(DeviceModel m0, DeviceTallies
   DeviceModel m = m0;
   specialize<kPlain>(m);
   const double* inczeta = rp.inczeta;
-  stage_tables(m, inczeta, rp.lds, lds_tables);
+  stage_tables<kPlain>(m, inczeta, rp.lds, lds_tables);
   Ctx<T> C{m, t, {}, {s_cnt}, inczeta, rp.q};
   const uint64_t lane = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   LaneIO io{rp.lane_words + lane, rp.lanes, 0};
@@ -1831,9 +1837,11 @@ __global__ void __launch_bounds__(256) k_tail(DeviceModel m0, DeviceTallies t, R
   DeviceModel m = m0;
   specialize<kPlain>(m);
   const double* inczeta = rp.inczeta;
-  stage_tables(m, inczeta, rp.lds, lds_tables);
+  stage_tables<kPlain>(m, inczeta, rp.lds, lds_tables);
   Ctx<T> C{m, t, {}, {s_cnt}, inczeta, rp.q};
   C.coop = true;
+  __shared__ T s_cdf[4][kCdfCache];
+  C.cdf.row = s_cdf[threadIdx.x >> 6];
   C.cnt.one = true;
   const unsigned wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (wave < nlive) {
