@@ -61,8 +61,12 @@ struct mcrt_ctx {
   int peel_blocks_per_cu = 0;
   int regroup = 0;
   int peel_fetch = 0, peel_tally = 0;   // peel-kernel gather thresholds (0: defaults)
-  RunCtl* host_ctl = nullptr;             // pinned read-back of the run control block
+  RunCtl* host_ctl = nullptr;             // pinned read-back of the run control blocks (two)
   hipEvent_t evr[3] = {nullptr, nullptr, nullptr};   // per-round kernel bracketing
+  // the peel kernels of round r run on peel_stream while round r + 1's
+  // transport runs on stream (two peel-off queues, ordered by events)
+  hipStream_t peel_stream = nullptr;
+  hipEvent_t evp[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // peel start / end per queue
   double stats[MCRT_NSTATS] = {0};        // mcrt_run_stats of the last run
   // batch of axisymmetric models (mcrt_set_batch)
   bool has_batch = false, has_batch_tdust = false;
@@ -286,7 +290,14 @@ int mcrt_create(int device, mcrt_ctx** out) {
   (void)hipEventCreate(&c->ev0);
   (void)hipEventCreate(&c->ev1);
   for (auto& e : c->evr) (void)hipEventCreate(&e);
-  if (hipHostMalloc((void**)&c->host_ctl, sizeof(RunCtl), hipHostMallocDefault) != hipSuccess) {
+  for (auto& q : c->evp)
+    for (auto& e : q) (void)hipEventCreate(&e);
+  if (hipStreamCreateWithFlags(&c->peel_stream, hipStreamNonBlocking) != hipSuccess) {
+    c->peel_stream = nullptr;
+    mcrt_destroy(c);
+    return MCRT_E_HIP;
+  }
+  if (hipHostMalloc((void**)&c->host_ctl, 2 * sizeof(RunCtl), hipHostMallocDefault) != hipSuccess) {
     c->host_ctl = nullptr;
     mcrt_destroy(c);
     return MCRT_E_HIP;
@@ -310,14 +321,18 @@ int mcrt_destroy(mcrt_ctx* c) {
   if (!c) return MCRT_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->peel_stream) (void)hipStreamSynchronize(c->peel_stream);
   for (auto& kv : c->buf) (void)hipFree(kv.second);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (auto& e : c->evr) if (e) (void)hipEventDestroy(e);
+  for (auto& q : c->evp)
+    for (auto& e : q) if (e) (void)hipEventDestroy(e);
   if (c->host_ctl) (void)hipHostFree(c->host_ctl);
   if (c->b_exec) (void)hipGraphExecDestroy(c->b_exec);
   if (c->b_run_h) (void)hipHostFree(c->b_run_h);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (c->peel_stream) (void)hipStreamDestroy(c->peel_stream);
   delete c;
   return MCRT_OK;
 }
@@ -937,18 +952,24 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   const int words = fp64 ? lane_words_fp64() : lane_words_mixed();
   void* p;
   int rc;
-  if ((rc = dev_alloc(c, "run.ctl", sizeof(RunCtl), &p))) return rc;
+  // ctl[0]: packets handed out, lanes alive, queue 0's count; ctl[1]: queue 1's count
+  if ((rc = dev_alloc(c, "run.ctl", 2 * sizeof(RunCtl), &p))) return rc;
   RunCtl* ctl = (RunCtl*)p;
   if ((rc = dev_alloc(c, fp64 ? "lanes.words64" : "lanes.words32", (size_t)lanes * words * 4, &p))) return rc;
   uint32_t* lane_words = (uint32_t*)p;
   if ((rc = dev_alloc(c, "lanes.live", (size_t)lanes, &p))) return rc;
   uint8_t* lane_live = (uint8_t*)p;
-  PeelQueue q{nullptr, &ctl->q_count, 0};
+  // Two peel-off queues: round r fills queue r % 2 while the peel kernels
+  // of round r - 1 walk the other one.
+  PeelQueue qb[2] = {{nullptr, &ctl[0].q_count, 0}, {nullptr, &ctl[1].q_count, 0}};
   if (queued) {
     const size_t rb = fp64 ? peel_record_bytes_fp64() : peel_record_bytes_mixed();
-    if ((rc = dev_alloc(c, fp64 ? "q.recs64" : "q.recs32", (size_t)capacity * rb, &p))) return rc;
-    q.recs = p;
-    q.capacity = (unsigned)capacity;
+    static const char* qn[2][2] = {{"q.recs32", "q.recs32b"}, {"q.recs64", "q.recs64b"}};
+    for (int b = 0; b < 2; ++b) {
+      if ((rc = dev_alloc(c, qn[fp64 ? 1 : 0][b], (size_t)capacity * rb, &p))) return rc;
+      qb[b].recs = p;
+      qb[b].capacity = (unsigned)capacity;
+    }
   }
   if (dm.acc_to > 0) {
     if ((rc = dev_alloc(c, "acc.start", (size_t)dm.acc_to * sizeof(AccStart), &p))) return rc;
@@ -1070,8 +1091,36 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   st[MCRT_STAT_EVENTS0] = (double)events;
   st[MCRT_STAT_RECORD_BYTES] = (double)(fp64 ? peel_record_bytes_fp64() : peel_record_bytes_mixed());
   st[MCRT_STAT_LANE_BYTES] = 4.0 * words;
+  // The peel kernels of a round run on the peel stream, concurrently with
+  // the next round's transport (MCRT_OVERLAP=0: both on the run stream, one
+  // after the other).  Round r writes queue r % 2; its transport first
+  // waits for the peel kernels of round r - 2, which read that queue.
+  hipStream_t ps = c->peel_stream;
+  if (const char* ov = std::getenv("MCRT_OVERLAP")) if (std::atoi(ov) == 0) ps = c->stream;
+  struct Pending {
+    bool on = false;
+    int64_t round = 0;
+    float ta = 0.f;
+    unsigned records = 0, live = 0;
+    unsigned long long handed = 0;
+  } pend[2];
+  // time of the finished peel kernels of queue b (the host has seen a later
+  // event on the same stream complete, or synchronizes here)
+  auto collect = [&](int b) -> int {
+    if (!pend[b].on) return MCRT_OK;
+    HIPCHECK(c, hipEventSynchronize(c->evp[b][1]));
+    float tb = 0.f;
+    HIPCHECK(c, hipEventElapsedTime(&tb, c->evp[b][0], c->evp[b][1]));
+    st[MCRT_STAT_PEEL_MS] += tb;
+    if (verbose)
+      std::fprintf(stderr, "[mcrt] round %lld: transport %.3f ms, peel %.3f ms, records %u, parked %u, events %lld, handed out %llu/%lld\n",
+                   (long long)pend[b].round, pend[b].ta, tb, pend[b].records, pend[b].live, (long long)events,
+                   pend[b].handed, (long long)npackets);
+    pend[b].on = false;
+    return MCRT_OK;
+  };
   HIPCHECK(c, hipEventRecord(c->ev0, c->stream));
-  HIPCHECK(c, launch_fill_i32((int*)ctl, sizeof(RunCtl) / 4, 0, c->stream));
+  HIPCHECK(c, launch_fill_i32((int*)ctl, 2 * sizeof(RunCtl) / 4, 0, c->stream));
   if (dm.acc_to > 0) {
     AccStart* acc = (AccStart*)c->buf["acc.start"];
     HIPCHECK(c, fp64 ? launch_acc_place_fp64(dm, seed, (uint64_t)first_packet, acc, c->stream)
@@ -1080,12 +1129,13 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   int64_t round = 0;
   for (;; ++round) {
     if (round >= ((int64_t)1 << 24)) return fail(c, MCRT_E_HIP, "mcrt_run_packets: packets did not finish");
+    const int b = (int)(round & 1);
     RunParams rp;
     rp.first_packet = (uint64_t)first_packet;
     rp.npackets = (uint64_t)npackets;
     rp.seed = seed;
     rp.inczeta = (const double*)c->buf["inczeta"];
-    rp.q = q;
+    rp.q = qb[b];
     rp.lane_words = lane_words;
     rp.lane_live = lane_live;
     rp.lanes = (uint64_t)lanes;
@@ -1096,8 +1146,14 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     rp.peel_regroup = peel_regroup;
     rp.peel_tally = peel_tally;
     rp.lds = lds;
-    // q_count and live restart every round (a fill kernel, not a memset)
-    HIPCHECK(c, launch_fill_i32((int*)ctl + 2, 2, 0, c->stream));
+    // queue b is free once the peel kernels of round - 2 are done
+    if (pend[b].on) {
+      HIPCHECK(c, hipStreamWaitEvent(c->stream, c->evp[b][1], 0));
+      if (int r2 = collect(b)) return r2;
+    }
+    // queue b's count and the live count restart every round (fill kernels, not memsets)
+    HIPCHECK(c, launch_fill_i32((int*)&ctl[b].q_count, 1, 0, c->stream));
+    HIPCHECK(c, launch_fill_i32((int*)&ctl[0].live, 1, 0, c->stream));
     HIPCHECK(c, hipEventRecord(c->evr[0], c->stream));
     hipError_t e;
     if (tail_mode) {
@@ -1111,41 +1167,57 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
     }
     if (e != hipSuccess) return hipfail(c, e, "transport kernel launch");
     HIPCHECK(c, hipEventRecord(c->evr[1], c->stream));
-    if (queued && split) {
-      // the round's record count sizes the chunks
-      HIPCHECK(c, hipMemcpyAsync(c->host_ctl, ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, c->stream));
-      HIPCHECK(c, hipStreamSynchronize(c->stream));
-      const uint64_t nrec = c->host_ctl->q_count < (unsigned)capacity ? c->host_ctl->q_count : (unsigned)capacity;
-      const uint64_t total = nrec * (uint64_t)c->dm.npeel;
-      for (uint64_t i0 = 0; i0 < total; i0 += chunk) {
-        const uint64_t i1 = total - i0 < chunk ? total : i0 + chunk;
-        HIPCHECK(c, launch_fill_i32((int*)split_ctr, 4, 0, c->stream));
-        e = launch_peel_split_mixed(dm, t, rp, rays, hits, i0, i1, sgrid, wgrid, hgrid, walk_fetch, c->stream);
-        if (e != hipSuccess) return hipfail(c, e, "peel kernels launch");
-      }
-    } else if (queued) {
-      e = fp64 ? launch_peel_fp64(dm, t, rp, pgrid, c->stream) : launch_peel_mixed(dm, t, rp, pgrid, c->stream);
-      if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
-    }
-    HIPCHECK(c, hipEventRecord(c->evr[2], c->stream));
-    HIPCHECK(c, hipMemcpyAsync(c->host_ctl, ctl, sizeof(RunCtl), hipMemcpyDeviceToHost, c->stream));
+    // the round's counts: packets handed out, lanes alive, records queued
+    HIPCHECK(c, hipMemcpyAsync(c->host_ctl, ctl, 2 * sizeof(RunCtl), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
-    const RunCtl h = *c->host_ctl;
-    float ta = 0.f, tb = 0.f;
+    const RunCtl h = c->host_ctl[0];
+    const unsigned qcount = c->host_ctl[b].q_count;
+    const uint64_t nrec = qcount < (unsigned)capacity ? qcount : (unsigned)capacity;
+    float ta = 0.f;
     HIPCHECK(c, hipEventElapsedTime(&ta, c->evr[0], c->evr[1]));
-    HIPCHECK(c, hipEventElapsedTime(&tb, c->evr[1], c->evr[2]));
     st[MCRT_STAT_TRANSPORT_MS] += ta;
-    st[MCRT_STAT_PEEL_MS] += tb;
-    st[MCRT_STAT_RECORDS] += (double)(h.q_count < (unsigned)capacity ? h.q_count : (unsigned)capacity);
+    st[MCRT_STAT_RECORDS] += (double)nrec;
     st[MCRT_STAT_PARKED] += (double)h.live;
-    if (queued) st[MCRT_STAT_PEEL_LAUNCHES] += 1.0;
-    if (verbose)
+    if (queued) {
+      HIPCHECK(c, hipStreamWaitEvent(ps, c->evr[1], 0));
+      HIPCHECK(c, hipEventRecord(c->evp[b][0], ps));
+      if (split) {
+        // the round's record count sizes the chunks
+        const uint64_t total = nrec * (uint64_t)c->dm.npeel;
+        for (uint64_t i0 = 0; i0 < total; i0 += chunk) {
+          const uint64_t i1 = total - i0 < chunk ? total : i0 + chunk;
+          HIPCHECK(c, launch_fill_i32((int*)split_ctr, 4, 0, ps));
+          e = launch_peel_split_mixed(dm, t, rp, rays, hits, i0, i1, sgrid, wgrid, hgrid, walk_fetch, ps);
+          if (e != hipSuccess) return hipfail(c, e, "peel kernels launch");
+        }
+      } else {
+        e = fp64 ? launch_peel_fp64(dm, t, rp, pgrid, ps) : launch_peel_mixed(dm, t, rp, pgrid, ps);
+        if (e != hipSuccess) return hipfail(c, e, "peel kernel launch");
+      }
+      HIPCHECK(c, hipEventRecord(c->evp[b][1], ps));
+      st[MCRT_STAT_PEEL_LAUNCHES] += 1.0;
+      pend[b].on = true;
+      pend[b].round = round;
+      pend[b].ta = ta;
+      pend[b].records = qcount;
+      pend[b].live = h.live;
+      pend[b].handed = h.next_packet;
+    } else if (verbose) {
       std::fprintf(stderr, "[mcrt] round %lld: transport %.3f ms, peel %.3f ms, records %u, parked %u, events %lld, handed out %llu/%lld\n",
-                   (long long)round, ta, tb, h.q_count, h.live, (long long)events, h.next_packet, (long long)npackets);
+                   (long long)round, ta, 0.0, qcount, h.live, (long long)events, h.next_packet, (long long)npackets);
+    }
     if (h.next_packet < (unsigned long long)npackets) continue;
     if (h.live == 0) break;
     if ((int64_t)h.live <= tail_max) tail_mode = true;
     nlive = h.live;
+  }
+  // the run ends when the last peel kernels have: the run stream waits for them
+  for (int k = 1; k <= 2; ++k) {
+    const int b = (int)((round + k) & 1);     // the older pending queue first
+    if (pend[b].on) {
+      HIPCHECK(c, hipStreamWaitEvent(c->stream, c->evp[b][1], 0));
+      if (int r2 = collect(b)) return r2;
+    }
   }
   st[MCRT_STAT_ROUNDS] = (double)(round + 1);
   {
