@@ -1522,14 +1522,22 @@ __device__ __forceinline__ int emit(Ctx<T>& C, Life<T>& L) {
     }
     initp_sample(C, p);
   } else {
-    p.cosb = (T)1 - (T)2 * uniform<T>(C.rng);
-    p.sinb = RT<T>::sqrt_((T)1 - p.cosb * p.cosb);
-    p.cost = -uniform<T>(C.rng);
-    p.sint = RT<T>::sqrt_((T)1 - p.cost * p.cost);
-    p.phi = (T)kR2p * uniform<T>(C.rng);
-    p.cosp = RT<T>::cos_(p.phi);
-    p.sinp = RT<T>::sin_(p.phi);
-    p.lp = (T)kR2p * uniform<T>(C.rng);
+    // (draws in the reference's order; the fields are set in the order the
+    // stellar branch sets them, so that the optimizer does not merge the two
+    // branches' last stores into one store through a selected address,
+    // which would put those fields into private memory)
+    const T cb = (T)1 - (T)2 * uniform<T>(C.rng);
+    const T ct = -uniform<T>(C.rng);
+    const T ph = (T)kR2p * uniform<T>(C.rng);
+    const T lp = (T)kR2p * uniform<T>(C.rng);
+    p.cosb = cb;
+    p.sinb = RT<T>::sqrt_((T)1 - cb * cb);
+    p.lp = lp;
+    p.cost = ct;
+    p.sint = RT<T>::sqrt_((T)1 - ct * ct);
+    p.phi = ph;
+    p.cosp = RT<T>::cos_(ph);
+    p.sinp = RT<T>::sin_(ph);
   }
   initp_transform(p);
   const bool shock = m.ispot != 0 && (spot || m.fspot == 1.0);
