@@ -19,7 +19,11 @@
  *    eV, temperature in K/11605, density in g/cm^3, volume in cm^3;
  *  - a context is used by one host thread at a time (the reference runs one
  *    thread per MPI rank); work is queued on the context's stream and every
- *    call that returns data to the host synchronizes that stream.
+ *    call that returns data to the host synchronizes that stream.  Inside
+ *    mcrt_run_packets the peel-off kernels of each round also run on a
+ *    second stream the context owns (default priority), ordered by events;
+ *    the context's stream waits for them before the call returns, so work
+ *    queued after it sees complete tallies.
  */
 #ifndef MCRT_H
 #define MCRT_H
