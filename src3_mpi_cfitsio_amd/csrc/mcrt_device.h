@@ -213,16 +213,29 @@ __device__ __forceinline__ void rng_resume(Rng& r, uint64_t seed, uint64_t packe
   rng_seek(r, n);
 }
 
+// coop: the lanes take blocks blk .. blk + 63 of the stream
+__device__ __forceinline__ void rng_fill(Rng& r, uint32_t blk) {
+  uint32_t o[4];
+  philox(blk + (uint32_t)__lane_id(), r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
+  r.cb = blk;
+  r.w0 = o[0]; r.w1 = o[1]; r.w2 = o[2]; r.w3 = o[3];
+}
+
+// coop: word p of the stream, p held by the lanes (cb*4 <= p < cb*4 + 256),
+// read by every lane for its own p
+__device__ __forceinline__ uint32_t rng_word_at(const Rng& r, uint32_t p) {
+  const int s = (int)((p >> 2) - r.cb);
+  const uint32_t x0 = (uint32_t)__shfl((int)r.w0, s, 64), x1 = (uint32_t)__shfl((int)r.w1, s, 64);
+  const uint32_t x2 = (uint32_t)__shfl((int)r.w2, s, 64), x3 = (uint32_t)__shfl((int)r.w3, s, 64);
+  const uint32_t c = p & 3u;
+  return (c == 0u) ? x0 : (c == 1u) ? x1 : (c == 2u) ? x2 : x3;
+}
+
 __device__ __forceinline__ uint32_t rng_word(Rng& r) {
   const uint32_t j = r.n & 3u;
   if (r.coop) {
     const uint32_t blk = r.n >> 2;
-    if (blk - r.cb >= 64u) {
-      uint32_t o[4];
-      philox(blk + (uint32_t)__lane_id(), r.p0, r.p1, 0x4D435254u, r.k0, r.k1, o);
-      r.cb = blk;
-      r.w0 = o[0]; r.w1 = o[1]; r.w2 = o[2]; r.w3 = o[3];
-    }
+    if (blk - r.cb >= 64u) rng_fill(r, blk);
     // (the words pass an empty asm, so the select stays one between values:
     // a select between fields is a load from a selected address, which
     // would put the whole state into private memory)
@@ -242,16 +255,18 @@ __device__ __forceinline__ uint32_t rng_word(Rng& r) {
   return w;
 }
 
-template <class T> __device__ __forceinline__ T uniform(Rng& r);
-template <> __device__ __forceinline__ double uniform<double>(Rng& r) {
-  return ((double)rng_word(r) + 0.5) * 2.3283064365386963e-10;
+// a draw in (0, 1) from one word of the stream
+template <class T> __device__ __forceinline__ T u01(uint32_t w);
+template <> __device__ __forceinline__ double u01<double>(uint32_t w) {
+  return ((double)w + 0.5) * 2.3283064365386963e-10;
 }
 // 23 bits: (k + 1/2) 2^-23 is exact in FP32 for every k < 2^23, so the
 // draw is strictly inside (0, 1).  (With 24 bits, k + 1/2 is not
 // representable for k >= 2^23: it rounds to even and k = 2^24 - 1 gives 1.)
-template <> __device__ __forceinline__ float uniform<float>(Rng& r) {
-  return ((float)(rng_word(r) >> 9) + 0.5f) * 1.1920928955078125e-7f;
+template <> __device__ __forceinline__ float u01<float>(uint32_t w) {
+  return ((float)(w >> 9) + 0.5f) * 1.1920928955078125e-7f;
 }
+template <class T> __device__ __forceinline__ T uniform(Rng& r) { return u01<T>(rng_word(r)); }
 
 // ---------------------------------------------------------------------------
 // Real-type helpers

@@ -1068,14 +1068,25 @@ __device__ __forceinline__ void stokes_coop(Ctx<T>& C, T g, T pl, const Pkt<T>& 
   const int lane = __lane_id();
   const uint32_t n0 = C.rng.n;
   for (uint32_t done = 0;; done += 64u) {
-    Rng r = C.rng;
-    r.coop = false;                 // each lane its own trial's draws
-    rng_seek(r, n0 + 3u * (done + (uint32_t)lane));
-    const T u1 = uniform<T>(r);
-    const T u2 = uniform<T>(r);
+    // this pass's 192 draws from the lane-held blocks (refilled when they
+    // do not hold them all), each lane reading its trial's three words
+    const uint32_t pa = n0 + 3u * (done + (uint32_t)lane);
+    T u1, u2, xran;
+    if (C.rng.coop) {
+      const uint32_t lo = (n0 + 3u * done) >> 2, hi = (n0 + 3u * (done + 64u) - 1u) >> 2;
+      if (lo - C.rng.cb >= 64u || hi - C.rng.cb >= 64u) rng_fill(C.rng, lo);
+      u1 = u01<T>(rng_word_at(C.rng, pa));
+      u2 = u01<T>(rng_word_at(C.rng, pa + 1u));
+      xran = u01<T>(rng_word_at(C.rng, pa + 2u));
+    } else {
+      Rng r = C.rng;
+      rng_seek(r, pa);
+      u1 = uniform<T>(r);
+      u2 = uniform<T>(r);
+      xran = uniform<T>(r);
+    }
     T b, ri, rprob;
     stokes_trial<T>(g, pl, p, u1, u2, b, ri, rprob);
-    const T xran = uniform<T>(r);
     T pm = rprob;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
