@@ -1102,6 +1102,14 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
   // waits for the peel kernels of round r - 2, which read that queue.
   hipStream_t ps = c->peel_stream;
   if (const char* ov = std::getenv("MCRT_OVERLAP")) if (std::atoi(ov) == 0) ps = c->stream;
+  // A run that returns an error part-way drains the peel stream first, so
+  // no peel kernel of it is left running into the context's next call; a
+  // run that completes has made the context's stream wait for them instead.
+  struct PeelDrain {
+    hipStream_t s;
+    bool armed;
+    ~PeelDrain() { if (armed && s) (void)hipStreamSynchronize(s); }
+  } drain{ps, true};
   struct Pending {
     bool on = false;
     int64_t round = 0;
@@ -1228,6 +1236,7 @@ int mcrt_run_packets(mcrt_ctx* c, int64_t first_packet, int64_t npackets, uint64
       if (int r2 = collect(b)) return r2;
     }
   }
+  drain.armed = false;
   st[MCRT_STAT_ROUNDS] = (double)(round + 1);
   {
     unsigned long long cnt[C_NCOUNTERS];
