@@ -513,7 +513,7 @@ def main():
         log(f"copied {x.numel() * 4} bytes")
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path would shard by packet ranges with one tally all-reduce per Lucy iteration)",
               file=sys.stderr)
         sys.exit(2)
